@@ -1,0 +1,298 @@
+// Oracle shim: a C-ABI around the reference CPU BatchSim (include/rocketgpu/rocketgpu.h:988-1089), compiled against
+// the reference's own headers by oracle/Makefile into oracle/_ref/librgref.so.  TEST INFRASTRUCTURE ONLY: tests,
+// the fixture generator (tests/golden/make_fixtures.py) and bench.py's CPU baseline load it as the checker; the
+// product never links it.
+//
+// The shim builds the procedural arena (include/rocketgpu_amd/workload.h) as a custom ArenaLayout, the way a
+// reference user would (ArenaLayout::meshes, rocketgpu.h:943-959), and exposes the BatchSim calls the parity tests
+// need plus a seeded multi-tick driver that records per-tick state digests.
+#include <cstdio>
+#include <cstring>
+#include <exception>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "rocketgpu/rocketgpu.h"
+#include "rocketgpu_amd/workload.h"
+
+using namespace rocketgpu;
+
+namespace {
+
+struct RefSim {
+    std::unique_ptr<BatchSim> sim;
+    int env_count = 0;
+    int max_cars = 0;
+    double scale = 1.0;
+};
+
+thread_local std::string g_error;
+
+int Fail(const std::exception& e) {
+    g_error = e.what();
+    return -1;
+}
+
+CarConfig PresetConfig(int id) {
+    switch (id) {
+    case 1: return DominusConfig();
+    case 2: return PlankConfig();
+    case 3: return BreakoutConfig();
+    case 4: return HybridConfig();
+    case 5: return MercConfig();
+    default: return OctaneConfig();
+    }
+}
+
+ArenaLayout ProceduralLayout(double scale, double fillet_radius) {
+    rgw::ArenaParams p;
+    p.scale = scale;
+    p.fillet_radius = fillet_radius;
+    const rgw::ArenaGeometry g = rgw::BuildProceduralArena(p);
+    const rgw::ArenaRulesSpec rules = rgw::ProceduralArenaRules(p);
+    ArenaLayout layout;
+    ArenaMesh mesh;
+    mesh.vertices.reserve(g.vertices.size());
+    for (const rgw::V3& v : g.vertices) mesh.vertices.push_back(Vec3(v.x, v.y, v.z));
+    mesh.indices = g.indices;
+    layout.meshes.push_back(std::move(mesh));
+    layout.boost_pads.clear();
+    for (int k = 0; k < rules.pad_count; ++k) {
+        BoostPadConfig pad;
+        pad.pos = Vec3(rules.pad_pos[k].x, rules.pad_pos[k].y, rules.pad_pos[k].z);
+        pad.is_big = rules.pad_big[k];
+        layout.boost_pads.push_back(pad);
+    }
+    const auto v3 = [](const rgw::V3& v) { return Vec3(v.x, v.y, v.z); };
+    for (int t = 0; t < 2; ++t) {
+        layout.goals[t].point = v3(rules.goals[t].point);
+        layout.goals[t].normal = v3(rules.goals[t].normal);
+        layout.goals[t].min = v3(rules.goals[t].min);
+        layout.goals[t].max = v3(rules.goals[t].max);
+        for (int k = 0; k < 5; ++k) {
+            const rgw::SpawnSpot& s = rules.kickoff[t][k];
+            layout.kickoff_spawns[t][k] = ArenaSpawn{Vec3(s.x, s.y, s.z), s.yaw};
+        }
+        for (int k = 0; k < 4; ++k) {
+            const rgw::SpawnSpot& s = rules.respawn[t][k];
+            layout.respawn_spawns[t][k] = ArenaSpawn{Vec3(s.x, s.y, s.z), s.yaw};
+        }
+    }
+    layout.ball_spawn = v3(rules.ball_spawn);
+    return layout;
+}
+
+uint64_t EnvDigest(const BallState& ball, const CarState* cars, int count) {
+    uint64_t h = rgw::kFnvOffset;
+    const float brot[4] = {ball.rot.w, ball.rot.x, ball.rot.y, ball.rot.z};
+    h = rgw::BodyDigest(h, &ball.pos.x, &ball.vel.x, &ball.ang_vel.x, brot);
+    for (int i = 0; i < count; ++i) {
+        const CarState& c = cars[i];
+        const float rot[4] = {c.rot.w, c.rot.x, c.rot.y, c.rot.z};
+        h = rgw::BodyDigest(h, &c.pos.x, &c.vel.x, &c.ang_vel.x, rot);
+    }
+    return h;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* rgref_last_error() { return g_error.c_str(); }
+
+// Struct sizes of the reference build: 0 CarState, 1 BallState, 2 EnvState, 3 BoostPadState, 4 CarControls,
+// 5 CarConfig, 6 EnvCommand.
+long rgref_sizeof(int which) {
+    switch (which) {
+    case 0: return sizeof(CarState);
+    case 1: return sizeof(BallState);
+    case 2: return sizeof(EnvState);
+    case 3: return sizeof(BoostPadState);
+    case 4: return sizeof(CarControls);
+    case 5: return sizeof(CarConfig);
+    case 6: return sizeof(EnvCommand);
+    default: return -1;
+    }
+}
+
+// Field offsets used by the layout-equality test: which 0 CarState, 1 BallState, 2 EnvState, 3 BoostPadState.
+long rgref_offsetof(int which, const char* field) {
+    const std::string f(field);
+#define RGREF_FIELD(T, name) if (f == #name) return static_cast<long>(offsetof(T, name));
+    if (which == 0) {
+        RGREF_FIELD(CarState, pos) RGREF_FIELD(CarState, rot) RGREF_FIELD(CarState, vel) RGREF_FIELD(CarState, ang_vel)
+        RGREF_FIELD(CarState, boost) RGREF_FIELD(CarState, is_on_ground) RGREF_FIELD(CarState, is_demoed)
+        RGREF_FIELD(CarState, pos_bt) RGREF_FIELD(CarState, vel_bt) RGREF_FIELD(CarState, basis_forward)
+        RGREF_FIELD(CarState, tick_count_since_update) RGREF_FIELD(CarState, ball_hit_info)
+        RGREF_FIELD(CarState, last_controls) RGREF_FIELD(CarState, auto_flip_torque_scale)
+    } else if (which == 1) {
+        RGREF_FIELD(BallState, pos) RGREF_FIELD(BallState, rot) RGREF_FIELD(BallState, vel)
+        RGREF_FIELD(BallState, ang_vel) RGREF_FIELD(BallState, pos_bt) RGREF_FIELD(BallState, vel_bt)
+        RGREF_FIELD(BallState, basis_forward) RGREF_FIELD(BallState, sleeping)
+        RGREF_FIELD(BallState, deactivation_time) RGREF_FIELD(BallState, constrained_2d)
+        RGREF_FIELD(BallState, tick_count_since_update) RGREF_FIELD(BallState, restored_inv_inertia_valid)
+    } else if (which == 2) {
+        RGREF_FIELD(EnvState, tick) RGREF_FIELD(EnvState, blue_score) RGREF_FIELD(EnvState, goal_scored_this_tick)
+        RGREF_FIELD(EnvState, car_visit_order_mode) RGREF_FIELD(EnvState, car_visit_order_packed)
+        RGREF_FIELD(EnvState, frame_frac_prev) RGREF_FIELD(EnvState, respawn_match_order)
+    } else if (which == 3) {
+        RGREF_FIELD(BoostPadState, cooldown) RGREF_FIELD(BoostPadState, is_active)
+        RGREF_FIELD(BoostPadState, pickup_frac_q) RGREF_FIELD(BoostPadState, cur_locked_car_id)
+    }
+#undef RGREF_FIELD
+    return -1;
+}
+
+void* rgref_create(int env_count, int max_cars, double scale, double fillet_radius, uint64_t respawn_seed) {
+    try {
+        auto h = std::make_unique<RefSim>();
+        SimConfig cfg;
+        cfg.env_count = env_count;
+        cfg.max_cars_per_env = max_cars;
+        cfg.respawn_rng_seed = respawn_seed;
+        h->sim = std::make_unique<BatchSim>(cfg, ProceduralLayout(scale, fillet_radius));
+        h->env_count = env_count;
+        h->max_cars = max_cars;
+        h->scale = scale;
+        return h.release();
+    } catch (const std::exception& e) {
+        Fail(e);
+        return nullptr;
+    }
+}
+
+void rgref_destroy(void* p) { delete static_cast<RefSim*>(p); }
+
+int rgref_set_env_car_count(void* p, const int32_t* counts, long n) {
+    try { static_cast<RefSim*>(p)->sim->set_env_car_count(counts, static_cast<size_t>(n)); return 0; }
+    catch (const std::exception& e) { return Fail(e); }
+}
+
+int rgref_set_teams(void* p, const uint8_t* teams, long n) {
+    try { static_cast<RefSim*>(p)->sim->set_teams(reinterpret_cast<const Team*>(teams), static_cast<size_t>(n)); return 0; }
+    catch (const std::exception& e) { return Fail(e); }
+}
+
+int rgref_set_car_presets(void* p, const uint8_t* presets, long n) {
+    try {
+        std::vector<CarConfig> cfgs(static_cast<size_t>(n));
+        for (long i = 0; i < n; ++i) cfgs[static_cast<size_t>(i)] = PresetConfig(presets[i]);
+        static_cast<RefSim*>(p)->sim->set_car_configs(cfgs.data(), cfgs.size());
+        return 0;
+    } catch (const std::exception& e) { return Fail(e); }
+}
+
+int rgref_reset_to_kickoff(void* p, uint64_t seed) {
+    try { static_cast<RefSim*>(p)->sim->reset_to_kickoff(seed); return 0; }
+    catch (const std::exception& e) { return Fail(e); }
+}
+
+// which: 0 CarState, 1 BallState, 2 EnvState, 3 BoostPadState, 4 CarControls (upload only).
+int rgref_upload(void* p, int which, const void* src, long n) {
+    try {
+        BatchSim& s = *static_cast<RefSim*>(p)->sim;
+        const size_t c = static_cast<size_t>(n);
+        switch (which) {
+        case 0: s.upload_car_states(static_cast<const CarState*>(src), c); break;
+        case 1: s.upload_ball_states(static_cast<const BallState*>(src), c); break;
+        case 2: s.upload_env_states(static_cast<const EnvState*>(src), c); break;
+        case 3: s.upload_boost_pad_states(static_cast<const BoostPadState*>(src), c); break;
+        case 4: s.upload_controls(static_cast<const CarControls*>(src), c); break;
+        default: throw std::invalid_argument("unknown state array");
+        }
+        return 0;
+    } catch (const std::exception& e) { return Fail(e); }
+}
+
+int rgref_download(void* p, int which, void* dst, long n) {
+    try {
+        const BatchSim& s = *static_cast<RefSim*>(p)->sim;
+        const size_t c = static_cast<size_t>(n);
+        switch (which) {
+        case 0: s.download_car_states(static_cast<CarState*>(dst), c); break;
+        case 1: s.download_ball_states(static_cast<BallState*>(dst), c); break;
+        case 2: s.download_env_states(static_cast<EnvState*>(dst), c); break;
+        case 3: s.download_boost_pad_states(static_cast<BoostPadState*>(dst), c); break;
+        default: throw std::invalid_argument("unknown state array");
+        }
+        return 0;
+    } catch (const std::exception& e) { return Fail(e); }
+}
+
+int rgref_apply_env_commands(void* p, const void* cmds, long n) {
+    try { static_cast<RefSim*>(p)->sim->apply_env_commands(static_cast<const EnvCommand*>(cmds), static_cast<size_t>(n)); return 0; }
+    catch (const std::exception& e) { return Fail(e); }
+}
+
+int rgref_step(void* p, int ticks) {
+    try { static_cast<RefSim*>(p)->sim->step(ticks); return 0; }
+    catch (const std::exception& e) { return Fail(e); }
+}
+
+int rgref_step_with_controls(void* p, const void* controls_by_tick, int ticks, int stride) {
+    try {
+        static_cast<RefSim*>(p)->sim->step_with_device_controls(static_cast<const CarControls*>(controls_by_tick),
+                                                                 ticks, stride);
+        return 0;
+    } catch (const std::exception& e) { return Fail(e); }
+}
+
+// Per-env body digests (ball, then cars in slot order up to the env's car count) of the current state.
+int rgref_env_digests(void* p, const int32_t* car_counts, uint64_t* out) {
+    try {
+        RefSim& h = *static_cast<RefSim*>(p);
+        std::vector<BallState> balls(static_cast<size_t>(h.env_count));
+        std::vector<CarState> cars(static_cast<size_t>(h.env_count) * static_cast<size_t>(h.max_cars));
+        h.sim->download_ball_states(balls.data(), balls.size());
+        if (!cars.empty()) h.sim->download_car_states(cars.data(), cars.size());
+        for (int e = 0; e < h.env_count; ++e) {
+            const int n = car_counts ? car_counts[e] : 0;
+            out[e] = EnvDigest(balls[static_cast<size_t>(e)], cars.data() + static_cast<size_t>(e) * h.max_cars, n);
+        }
+        return 0;
+    } catch (const std::exception& e) { return Fail(e); }
+}
+
+// Seeded driver: steps `ticks` ticks one at a time with controls drawn from per-car ControlStreams (seeded by
+// ctrl_seed, arena, slot; boost probability p_boost), and after each tick records the FNV-1a of the per-env
+// digests into batch_digest[t].  Every `checkpoint_every` ticks (if > 0) the per-env digests are appended to
+// env_digests (env_count values per checkpoint).  Returns the number of checkpoints written, or -1.
+long rgref_run(void* p, const int32_t* car_counts, uint64_t ctrl_seed, float p_boost, int ticks,
+               uint64_t* batch_digest, int checkpoint_every, uint64_t* env_digests) {
+    try {
+        RefSim& h = *static_cast<RefSim*>(p);
+        const size_t cap = static_cast<size_t>(h.env_count) * static_cast<size_t>(h.max_cars);
+        std::vector<rgw::ControlStream> streams(cap);
+        for (int e = 0; e < h.env_count; ++e)
+            for (int s = 0; s < h.max_cars; ++s)
+                streams[static_cast<size_t>(e) * h.max_cars + s].seed(ctrl_seed, e, s, p_boost);
+        std::vector<CarControls> ctrl(cap);
+        std::vector<uint64_t> digests(static_cast<size_t>(h.env_count));
+        long checkpoints = 0;
+        for (int t = 0; t < ticks; ++t) {
+            for (size_t i = 0; i < cap; ++i) {
+                const rgw::Controls c = streams[i].next();
+                std::memcpy(&ctrl[i], &c, sizeof(CarControls));
+            }
+            if (cap > 0) {
+                h.sim->step_with_device_controls(ctrl.data(), 1, static_cast<int>(cap));
+            } else {
+                h.sim->step(1);
+            }
+            if (rgref_env_digests(p, car_counts, digests.data()) != 0) return -1;
+            batch_digest[t] = rgw::Fnv1a(rgw::kFnvOffset, digests.data(), digests.size() * sizeof(uint64_t));
+            if (checkpoint_every > 0 && (t + 1) % checkpoint_every == 0) {
+                std::memcpy(env_digests + static_cast<size_t>(checkpoints) * h.env_count, digests.data(),
+                            digests.size() * sizeof(uint64_t));
+                ++checkpoints;
+            }
+        }
+        return checkpoints;
+    } catch (const std::exception& e) {
+        Fail(e);
+        return -1;
+    }
+}
+
+}  // extern "C"

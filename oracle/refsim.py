@@ -1,0 +1,149 @@
+"""ctypes wrapper of the oracle: the reference CPU BatchSim built by oracle/Makefile into oracle/_ref/librgref.so.
+
+TEST INFRASTRUCTURE ONLY.  Imported by tests/, tests/golden/make_fixtures.py, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg as the checker / CPU baseline; never by the product package.
+
+The reference API it mirrors is include/rocketgpu/rocketgpu.h:988-1089 (BatchSim); the shim is oracle/ref_shim.cpp.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_ref", "librgref.so")
+
+CAR, BALL, ENV, PAD, CONTROLS = 0, 1, 2, 3, 4
+
+_lib = None
+
+
+def available() -> bool:
+    return os.path.exists(LIB_PATH)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not available():
+            raise RuntimeError(f"oracle library missing: {LIB_PATH} (run `make -C oracle`)")
+        L = ctypes.CDLL(LIB_PATH)
+        L.rgref_last_error.restype = ctypes.c_char_p
+        L.rgref_sizeof.restype = ctypes.c_long
+        L.rgref_sizeof.argtypes = [ctypes.c_int]
+        L.rgref_offsetof.restype = ctypes.c_long
+        L.rgref_offsetof.argtypes = [ctypes.c_int, ctypes.c_char_p]
+        L.rgref_create.restype = ctypes.c_void_p
+        L.rgref_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_uint64]
+        L.rgref_destroy.argtypes = [ctypes.c_void_p]
+        vp, cp, lg = ctypes.c_void_p, ctypes.c_void_p, ctypes.c_long
+        for name, args in [
+            ("rgref_set_env_car_count", [vp, cp, lg]),
+            ("rgref_set_teams", [vp, cp, lg]),
+            ("rgref_set_car_presets", [vp, cp, lg]),
+            ("rgref_reset_to_kickoff", [vp, ctypes.c_uint64]),
+            ("rgref_upload", [vp, ctypes.c_int, cp, lg]),
+            ("rgref_download", [vp, ctypes.c_int, cp, lg]),
+            ("rgref_apply_env_commands", [vp, cp, lg]),
+            ("rgref_step", [vp, ctypes.c_int]),
+            ("rgref_step_with_controls", [vp, cp, ctypes.c_int, ctypes.c_int]),
+            ("rgref_env_digests", [vp, cp, cp]),
+        ]:
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = ctypes.c_int
+        L.rgref_run.restype = ctypes.c_long
+        L.rgref_run.argtypes = [vp, cp, ctypes.c_uint64, ctypes.c_float, ctypes.c_int, cp, ctypes.c_int, cp]
+        _lib = L
+    return _lib
+
+
+def sizeof(which: int) -> int:
+    return int(lib().rgref_sizeof(which))
+
+
+def offsetof(which: int, field: str) -> int:
+    return int(lib().rgref_offsetof(which, field.encode()))
+
+
+def _ptr(a: np.ndarray):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+class RefSim:
+    """One reference BatchSim(cfg, procedural layout)."""
+
+    def __init__(self, env_count: int, max_cars: int, scale: float = 1.0, fillet_radius: float = 256.0,
+                 respawn_seed: int = 0):
+        self._L = lib()
+        self.h = self._L.rgref_create(env_count, max_cars, scale, fillet_radius, respawn_seed)
+        if not self.h:
+            raise RuntimeError(self._L.rgref_last_error().decode())
+        self.env_count = env_count
+        self.max_cars = max_cars
+
+    def close(self):
+        if self.h:
+            self._L.rgref_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _check(self, rc):
+        if rc != 0:
+            raise RuntimeError(self._L.rgref_last_error().decode())
+
+    def set_env_car_count(self, counts):
+        a = np.ascontiguousarray(counts, dtype=np.int32)
+        self._check(self._L.rgref_set_env_car_count(self.h, _ptr(a), a.size))
+
+    def set_teams(self, teams):
+        a = np.ascontiguousarray(teams, dtype=np.uint8)
+        self._check(self._L.rgref_set_teams(self.h, _ptr(a), a.size))
+
+    def set_car_presets(self, presets):
+        a = np.ascontiguousarray(presets, dtype=np.uint8)
+        self._check(self._L.rgref_set_car_presets(self.h, _ptr(a), a.size))
+
+    def reset_to_kickoff(self, seed: int = 0):
+        self._check(self._L.rgref_reset_to_kickoff(self.h, seed))
+
+    def download(self, which: int, count: int) -> np.ndarray:
+        out = np.zeros((count, sizeof(which)), dtype=np.uint8)
+        self._check(self._L.rgref_download(self.h, which, _ptr(out), count))
+        return out
+
+    def upload(self, which: int, rows: np.ndarray):
+        rows = np.ascontiguousarray(rows, dtype=np.uint8)
+        assert rows.shape[1] == sizeof(which)
+        self._check(self._L.rgref_upload(self.h, which, _ptr(rows), rows.shape[0]))
+
+    def apply_env_commands(self, cmds: np.ndarray):
+        cmds = np.ascontiguousarray(cmds, dtype=np.uint8)
+        self._check(self._L.rgref_apply_env_commands(self.h, _ptr(cmds), cmds.shape[0]))
+
+    def step(self, ticks: int = 1):
+        self._check(self._L.rgref_step(self.h, ticks))
+
+    def step_with_controls(self, controls: np.ndarray, ticks: int, stride: int):
+        controls = np.ascontiguousarray(controls, dtype=np.uint8)
+        self._check(self._L.rgref_step_with_controls(self.h, _ptr(controls), ticks, stride))
+
+    def env_digests(self, car_counts) -> np.ndarray:
+        a = np.ascontiguousarray(car_counts, dtype=np.int32)
+        out = np.zeros(self.env_count, dtype=np.uint64)
+        self._check(self._L.rgref_env_digests(self.h, _ptr(a), _ptr(out)))
+        return out
+
+    def run(self, car_counts, ctrl_seed: int, ticks: int, checkpoint_every: int = 0, p_boost: float = 0.3):
+        a = np.ascontiguousarray(car_counts, dtype=np.int32)
+        batch = np.zeros(ticks, dtype=np.uint64)
+        n_ck = ticks // checkpoint_every if checkpoint_every > 0 else 0
+        env = np.zeros((max(n_ck, 1), self.env_count), dtype=np.uint64)
+        got = self._L.rgref_run(self.h, _ptr(a), ctrl_seed, p_boost, ticks, _ptr(batch), checkpoint_every, _ptr(env))
+        if got < 0:
+            raise RuntimeError(self._L.rgref_last_error().decode())
+        return batch, env[:got]
