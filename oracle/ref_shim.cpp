@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "rocketgpu/rocketgpu.h"
+#include "rocketgpu/detail/arena/mesh_bvh.hpp"
 #include "rocketgpu_amd/workload.h"
 
 using namespace rocketgpu;
@@ -252,6 +253,81 @@ int rgref_env_digests(void* p, const int32_t* car_counts, uint64_t* out) {
         }
         return 0;
     } catch (const std::exception& e) { return Fail(e); }
+}
+
+// Mesh tables the reference built for this sim's layout (host memory in the CPU build): counts into counts[0..2]
+// (triangles, BVH nodes, meshes).  With non-null outputs: per triangle 16 floats (a, b, c, normal, the three edge
+// angles, edge flags as an integer value); per node 4 int32 (qmin[a] | qmax[a] << 16 for a = 0..2, then the
+// escape/triangle index); per mesh 15 floats (aabb min/max, quantization min/max/scale) and 4 ints (triangle
+// start/count, node offset/count).
+int rgref_mesh_tables(void* p, long* counts, float* tris, int32_t* nodes, float* meshes, int32_t* mesh_ints) {
+    try {
+        const SimConfig& c = static_cast<RefSim*>(p)->sim->config();
+        counts[0] = c.arena_mesh_tri_count;
+        counts[1] = c.arena_mesh_bvh_node_count;
+        counts[2] = c.arena_mesh_bvh_mesh_count;
+        if (tris) {
+            for (int i = 0; i < c.arena_mesh_tri_count; ++i) {
+                const detail::ArenaMeshTriangle& t = c.arena_mesh_tris[i];
+                float* o = tris + 16 * static_cast<size_t>(i);
+                const Vec3 v[4] = {t.a, t.b, t.c, t.normal};
+                for (int k = 0; k < 4; ++k) {
+                    o[3 * k] = v[k].x;
+                    o[3 * k + 1] = v[k].y;
+                    o[3 * k + 2] = v[k].z;
+                }
+                o[12] = t.edge_v0v1_angle;
+                o[13] = t.edge_v1v2_angle;
+                o[14] = t.edge_v2v0_angle;
+                o[15] = static_cast<float>(t.edge_flags);
+            }
+        }
+        if (nodes) {
+            for (int i = 0; i < c.arena_mesh_bvh_node_count; ++i) {
+                const detail::ArenaMeshBvhNode& n = c.arena_mesh_bvh_nodes[i];
+                int32_t* o = nodes + 4 * static_cast<size_t>(i);
+                o[0] = n.quantized_aabb_min[0] | (n.quantized_aabb_max[0] << 16);
+                o[1] = n.quantized_aabb_min[1] | (n.quantized_aabb_max[1] << 16);
+                o[2] = n.quantized_aabb_min[2] | (n.quantized_aabb_max[2] << 16);
+                o[3] = n.escape_index_or_triangle_index;
+            }
+        }
+        if (meshes) {
+            for (int i = 0; i < c.arena_mesh_bvh_mesh_count; ++i) {
+                const detail::ArenaMeshBvhMeshInfo& m = c.arena_mesh_bvh_meshes[i];
+                const detail::ArenaMeshBvhQuantization& q = c.arena_mesh_bvh_quantization[m.mesh_index];
+                float* o = meshes + 15 * static_cast<size_t>(i);
+                const Vec3 v[5] = {m.aabb_min, m.aabb_max, q.bvh_aabb_min, q.bvh_aabb_max, q.bvh_quantization};
+                for (int k = 0; k < 5; ++k) {
+                    o[3 * k] = v[k].x;
+                    o[3 * k + 1] = v[k].y;
+                    o[3 * k + 2] = v[k].z;
+                }
+                int32_t* oi = mesh_ints + 4 * static_cast<size_t>(i);
+                oi[0] = m.triangle_start;
+                oi[1] = m.triangle_count;
+                oi[2] = m.node_offset;
+                oi[3] = m.node_count;
+            }
+        }
+        return 0;
+    } catch (const std::exception& e) { return Fail(e); }
+}
+
+// The reference's float kernels (include/rocketgpu/detail/math.hpp:36-492, and its sinf/cosf wrappers that use the
+// host libm beyond the UCRT range): fn 0 sin, 1 cos, 2 atan2(x, y), 3 pow(x, y), 4 acos.
+int rgref_math(int fn, const float* x, const float* y, float* out, long n) {
+    for (long i = 0; i < n; ++i) {
+        switch (fn) {
+        case 0: out[i] = RocketGpuSinf(x[i]); break;
+        case 1: out[i] = RocketGpuCosf(x[i]); break;
+        case 2: out[i] = RocketGpuUcrtAtan2f(x[i], y[i]); break;
+        case 3: out[i] = RocketGpuUcrtPowf(x[i], y[i]); break;
+        case 4: out[i] = RocketGpuUcrtAcosf(x[i]); break;
+        default: return -1;
+        }
+    }
+    return 0;
 }
 
 // Seeded driver: steps `ticks` ticks one at a time with controls drawn from per-car ControlStreams (seeded by

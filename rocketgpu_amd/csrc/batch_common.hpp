@@ -1,0 +1,71 @@
+// Host-side pieces shared by the HIP BatchSim and the host-compiled test twin: step parameters from the reference's
+// default MutatorConfig, the procedural layout's rules, and upload canonicalization.
+#pragma once
+
+#include <cmath>
+#include <vector>
+
+#include "core/rg_ball_step.hpp"
+#include "mesh_build.hpp"
+#include "rocketgpu_amd/workload.h"
+
+namespace rg {
+
+// MutatorConfig defaults (include/rocketgpu/rocketgpu.h:708-840) used by the step, with the ball damping scale
+// cached the way BatchSim caches it (src/rocketgpu.cpp:259-270).
+inline core::StepParams DefaultStepParams(const rgw::ArenaRulesSpec& rules) {
+    core::StepParams p{};
+    p.tick_dt = 1.0f / 120.0f;
+    p.gravity_z = -650.0f;
+    p.ball_radius = 91.24999f;
+    p.ball_mass = 30.000002f;
+    p.ball_drag = 0.03f;
+    p.ball_damping_scale = p.ball_drag <= 0.0f ? 1.0f : core::UcrtPowf(core::Max(0.0f, 1.0f - p.ball_drag), p.tick_dt);
+    p.ball_friction = 0.35f;
+    p.ball_restitution = 0.60f;
+    p.ball_max_speed = 6000.0f;
+    p.ball_max_ang_speed = 6.0f;
+    p.frame_frac = 0.5f;
+    p.enable_scoring = 1;
+    for (int t = 0; t < 2; ++t) {
+        const rgw::GoalSpec& g = rules.goals[t];
+        const float len = std::sqrt(g.normal.x * g.normal.x + g.normal.y * g.normal.y + g.normal.z * g.normal.z);
+        p.goal_point[t] = Vec3(g.point.x, g.point.y, g.point.z);
+        p.goal_normal[t] = Vec3(g.normal.x / len, g.normal.y / len, g.normal.z / len);
+        p.goal_min[t] = Vec3(g.min.x, g.min.y, g.min.z);
+        p.goal_max[t] = Vec3(g.max.x, g.max.y, g.max.z);
+    }
+    return p;
+}
+
+inline ArenaMeshHost BuildProceduralArenaMesh(const rgw::ArenaParams& params) {
+    const rgw::ArenaGeometry g = rgw::BuildProceduralArena(params);
+    std::vector<std::vector<Vec3>> verts(1);
+    for (const rgw::V3& v : g.vertices) verts[0].push_back(Vec3(v.x, v.y, v.z));
+    std::vector<std::vector<int32_t>> idx(1, g.indices);
+    return BuildArenaMesh(GroupLayoutMeshes(verts, idx));
+}
+
+// uu -> bt for an uploaded linear quantity, preferring the bt value whose x50 reproduces the uu value exactly
+// (src/rocketgpu.cpp:321-338).
+inline float ImportedScalarBt(float value) {
+    const float bt = value * (1.0f / 50.0f);
+    if (bt * 50.0f == value) return bt;
+    const float up = std::nextafter(bt, INFINITY);
+    const float down = std::nextafter(bt, -INFINITY);
+    if (up * 50.0f == value) return up;
+    if (down * 50.0f == value) return down;
+    return bt;
+}
+
+// Public ball state upload (CanonicalizeImportedBallStatesInPlace, src/rocketgpu.cpp:373-383).
+inline void CanonicalizeBall(BallState& b) {
+    b.pos_bt = Vec3(ImportedScalarBt(b.pos.x), ImportedScalarBt(b.pos.y), ImportedScalarBt(b.pos.z));
+    b.vel_bt = Vec3(ImportedScalarBt(b.vel.x), ImportedScalarBt(b.vel.y), ImportedScalarBt(b.vel.z));
+    b.pos = Vec3(b.pos_bt.x * 50.0f, b.pos_bt.y * 50.0f, b.pos_bt.z * 50.0f);
+    b.vel = Vec3(b.vel_bt.x * 50.0f, b.vel_bt.y * 50.0f, b.vel_bt.z * 50.0f);
+    b.sleeping = 2;
+    b.deactivation_time = 0.0f;
+}
+
+}  // namespace rg

@@ -1,0 +1,162 @@
+// Host-compiled twin of the step core, for tests only.  It runs the exact same core sources as the gfx950 kernels,
+// compiled for the host, so that the CPU test suite can compare the core against the oracle (the reference CPU
+// build) without a GPU, and so a GPU mismatch can be localized to the compiler or the device path.  It is not part
+// of the product: BatchSim never calls it and the Python package never loads it.
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "batch_common.hpp"
+
+using namespace rg;
+
+namespace {
+
+struct Twin {
+    ArenaMeshHost mesh;
+    core::ArenaMeshView view{};
+    core::StepParams params{};
+    std::vector<BallState> balls;
+    std::vector<EnvState> envs;
+    std::vector<core::ArenaInternal> internal;
+};
+
+thread_local std::string g_err;
+
+}  // namespace
+
+extern "C" {
+
+const char* rgt_last_error() { return g_err.c_str(); }
+
+void* rgt_create(int env_count, double scale, double fillet_radius) {
+    try {
+        auto t = std::make_unique<Twin>();
+        rgw::ArenaParams ap;
+        ap.scale = scale;
+        ap.fillet_radius = fillet_radius;
+        t->mesh = BuildProceduralArenaMesh(ap);
+        t->view.tris = t->mesh.tris.data();
+        t->view.nodes = t->mesh.nodes.data();
+        t->view.meshes = t->mesh.meshes.data();
+        t->view.mesh_count = static_cast<int32_t>(t->mesh.meshes.size());
+        t->view.tri_count = static_cast<int32_t>(t->mesh.tris.size());
+        t->params = DefaultStepParams(rgw::ProceduralArenaRules(ap));
+        t->balls.assign(static_cast<size_t>(env_count), BallState{});
+        t->envs.assign(static_cast<size_t>(env_count), EnvState{});
+        t->internal.assign(static_cast<size_t>(env_count), core::ArenaInternal{});
+        std::memset(t->internal.data(), 0, t->internal.size() * sizeof(core::ArenaInternal));
+        return t.release();
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return nullptr;
+    }
+}
+
+void rgt_destroy(void* p) { delete static_cast<Twin*>(p); }
+
+// Same table format as oracle/ref_shim.cpp rgref_mesh_tables.
+int rgt_mesh_tables(void* p, long* counts, float* tris, int32_t* nodes, float* meshes, int32_t* mesh_ints) {
+    const Twin& t = *static_cast<Twin*>(p);
+    counts[0] = static_cast<long>(t.mesh.tris.size());
+    counts[1] = static_cast<long>(t.mesh.nodes.size());
+    counts[2] = static_cast<long>(t.mesh.meshes.size());
+    if (tris) {
+        for (size_t i = 0; i < t.mesh.tris.size(); ++i) {
+            const core::MeshTriangle& m = t.mesh.tris[i];
+            float* o = tris + 16 * i;
+            const Vec3 v[4] = {m.a, m.b, m.c, m.normal};
+            for (int k = 0; k < 4; ++k) {
+                o[3 * k] = v[k].x;
+                o[3 * k + 1] = v[k].y;
+                o[3 * k + 2] = v[k].z;
+            }
+            o[12] = m.edge_angle_v0v1;
+            o[13] = m.edge_angle_v1v2;
+            o[14] = m.edge_angle_v2v0;
+            o[15] = static_cast<float>(m.flags & 0xffu);
+        }
+    }
+    if (nodes) {
+        for (size_t i = 0; i < t.mesh.nodes.size(); ++i) {
+            const core::MeshBvhNode& n = t.mesh.nodes[i];
+            int32_t* o = nodes + 4 * i;
+            for (int a = 0; a < 3; ++a) o[a] = n.qmin[a] | (n.qmax[a] << 16);
+            o[3] = n.escape_or_tri;
+        }
+    }
+    if (meshes) {
+        for (size_t i = 0; i < t.mesh.meshes.size(); ++i) {
+            const core::MeshInfo& m = t.mesh.meshes[i];
+            float* o = meshes + 15 * i;
+            const Vec3 v[5] = {m.aabb_min, m.aabb_max, m.q.bvh_min, m.q.bvh_max, m.q.scale};
+            for (int k = 0; k < 5; ++k) {
+                o[3 * k] = v[k].x;
+                o[3 * k + 1] = v[k].y;
+                o[3 * k + 2] = v[k].z;
+            }
+            int32_t* oi = mesh_ints + 4 * i;
+            oi[0] = m.tri_start;
+            oi[1] = m.tri_count;
+            oi[2] = m.node_start;
+            oi[3] = m.node_count;
+        }
+    }
+    return 0;
+}
+
+// which: 1 BallState, 2 EnvState (public upload: balls are canonicalized like BatchSim::upload_ball_states).
+int rgt_upload(void* p, int which, const void* src, long n) {
+    Twin& t = *static_cast<Twin*>(p);
+    if (which == 1) {
+        std::memcpy(t.balls.data(), src, static_cast<size_t>(n) * sizeof(BallState));
+        for (long i = 0; i < n; ++i) CanonicalizeBall(t.balls[static_cast<size_t>(i)]);
+        return 0;
+    }
+    if (which == 2) {
+        std::memcpy(t.envs.data(), src, static_cast<size_t>(n) * sizeof(EnvState));
+        return 0;
+    }
+    return -1;
+}
+
+int rgt_download(void* p, int which, void* dst, long n) {
+    Twin& t = *static_cast<Twin*>(p);
+    if (which == 1) {
+        std::memcpy(dst, t.balls.data(), static_cast<size_t>(n) * sizeof(BallState));
+        return 0;
+    }
+    if (which == 2) {
+        std::memcpy(dst, t.envs.data(), static_cast<size_t>(n) * sizeof(EnvState));
+        return 0;
+    }
+    return -1;
+}
+
+int rgt_step(void* p, int ticks) {
+    Twin& t = *static_cast<Twin*>(p);
+    auto scratch = std::make_unique<core::BallTickScratch>();
+    for (int k = 0; k < ticks; ++k) {
+        for (size_t e = 0; e < t.balls.size(); ++e) {
+            core::StepBallArena(t.balls[e], t.envs[e], t.internal[e], t.view, t.params, *scratch);
+        }
+    }
+    return 0;
+}
+
+int rgt_math(int fn, const float* x, const float* y, float* out, long n) {
+    for (long i = 0; i < n; ++i) {
+        switch (fn) {
+        case 0: out[i] = core::UcrtSinf(x[i]); break;
+        case 1: out[i] = core::UcrtCosf(x[i]); break;
+        case 2: out[i] = core::UcrtAtan2f(x[i], y[i]); break;
+        case 3: out[i] = core::UcrtPowf(x[i], y[i]); break;
+        case 4: out[i] = core::UcrtAcosf(x[i]); break;
+        default: return -1;
+        }
+    }
+    return 0;
+}
+
+}  // extern "C"
