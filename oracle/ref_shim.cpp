@@ -314,6 +314,32 @@ int rgref_mesh_tables(void* p, long* counts, float* tris, int32_t* nodes, float*
     } catch (const std::exception& e) { return Fail(e); }
 }
 
+// Default-constructed record bytes of the reference's structs (same ids as rgref_sizeof).
+int rgref_default_state(int which, void* out) {
+    switch (which) {
+    case 0: { const CarState v{}; std::memcpy(out, &v, sizeof(v)); return 0; }
+    case 1: { const BallState v{}; std::memcpy(out, &v, sizeof(v)); return 0; }
+    case 2: { const EnvState v{}; std::memcpy(out, &v, sizeof(v)); return 0; }
+    case 3: { const BoostPadState v{}; std::memcpy(out, &v, sizeof(v)); return 0; }
+    case 4: { const CarControls v{}; std::memcpy(out, &v, sizeof(v)); return 0; }
+    default: return -1;
+    }
+}
+
+// Ball-launch workload: default BallState records with pos / vel / ang_vel from rgw::SeededBallLaunch.
+int rgref_workload_ball_states(uint64_t seed, long first_arena, long count, double scale, void* out) {
+    BallState* b = static_cast<BallState*>(out);
+    for (long i = 0; i < count; ++i) {
+        const rgw::BallLaunch l = rgw::SeededBallLaunch(seed, static_cast<uint64_t>(first_arena + i), scale);
+        BallState s{};
+        s.pos = Vec3(l.pos[0], l.pos[1], l.pos[2]);
+        s.vel = Vec3(l.vel[0], l.vel[1], l.vel[2]);
+        s.ang_vel = Vec3(l.ang_vel[0], l.ang_vel[1], l.ang_vel[2]);
+        b[i] = s;
+    }
+    return 0;
+}
+
 // The reference's float kernels (include/rocketgpu/detail/math.hpp:36-492, and its sinf/cosf wrappers that use the
 // host libm beyond the UCRT range): fn 0 sin, 1 cos, 2 atan2(x, y), 3 pow(x, y), 4 acos.
 int rgref_math(int fn, const float* x, const float* y, float* out, long n) {

@@ -54,6 +54,10 @@ def lib():
             f = getattr(L, name)
             f.argtypes = args
             f.restype = ctypes.c_int
+        L.rgref_default_state.argtypes = [ctypes.c_int, cp]
+        L.rgref_workload_ball_states.argtypes = [ctypes.c_uint64, ctypes.c_long, ctypes.c_long, ctypes.c_double, cp]
+        L.rgref_math.argtypes = [ctypes.c_int, cp, cp, cp, ctypes.c_long]
+        L.rgref_mesh_tables.argtypes = [vp, cp, cp, cp, cp, cp]
         L.rgref_run.restype = ctypes.c_long
         L.rgref_run.argtypes = [vp, cp, ctypes.c_uint64, ctypes.c_float, ctypes.c_int, cp, ctypes.c_int, cp]
         _lib = L
@@ -66,6 +70,27 @@ def sizeof(which: int) -> int:
 
 def offsetof(which: int, field: str) -> int:
     return int(lib().rgref_offsetof(which, field.encode()))
+
+
+def default_state(which: int) -> np.ndarray:
+    out = np.zeros(sizeof(which), dtype=np.uint8)
+    if lib().rgref_default_state(which, out.ctypes.data) != 0:
+        raise ValueError(which)
+    return out
+
+
+def workload_ball_states(seed: int, first_arena: int, count: int, scale: float = 1.0) -> np.ndarray:
+    out = np.zeros((count, sizeof(BALL)), dtype=np.uint8)
+    lib().rgref_workload_ball_states(seed, first_arena, count, scale, out.ctypes.data)
+    return out
+
+
+def math(fn: int, x, y=None) -> np.ndarray:
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    y = np.ascontiguousarray(x if y is None else y, dtype=np.float32)
+    out = np.zeros_like(x)
+    lib().rgref_math(fn, x.ctypes.data, y.ctypes.data, out.ctypes.data, x.size)
+    return out
 
 
 def _ptr(a: np.ndarray):

@@ -1,0 +1,204 @@
+"""rocketgpu_amd: the batched arena step of RocketGPU on MI355X (gfx950).
+
+The native library (rocketgpu_amd/_build/librocketgpu_amd.so) holds the HIP kernels, BatchSim and the C-ABI of
+include/rocketgpu_amd/rg_capi.h.  This module is the Python face of that C-ABI, mirroring the reference's Python
+surface (python/rocketgpu_bindings.cpp: BatchSim, state_layouts, download_state_array, device arrays):
+
+    sim = BatchSim(env_count=4096, max_cars_per_env=6)        # procedural arena PA(1, 256)
+    sim.upload("ball_states", balls)                            # numpy uint8 [n, 288] or a structured array
+    sim.step(8)
+    balls = sim.download_state_array("ball_states")
+
+There is no CPU fallback: importing works anywhere, but constructing a BatchSim loads the HIP library and needs a
+GPU; a missing library raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import json
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "librocketgpu_amd.so")
+
+RG_ARRAY = {"car_states": 0, "ball_states": 1, "env_states": 2, "boost_pad_states": 3, "controls": 4,
+            "env_car_count": 5}
+ITEMSIZE = {"car_states": 880, "ball_states": 288, "env_states": 44, "boost_pad_states": 16, "controls": 24,
+            "env_car_count": 4}
+
+_lib = None
+
+
+class RocketGpuError(RuntimeError):
+    pass
+
+
+class _SimConfig(ctypes.Structure):
+    _fields_ = [("env_count", ctypes.c_int32), ("max_cars_per_env", ctypes.c_int32), ("tick_dt", ctypes.c_float),
+                ("enable_boost_pads", ctypes.c_uint8), ("enable_car_car_collisions", ctypes.c_uint8),
+                ("enable_demos", ctypes.c_uint8), ("enable_scoring", ctypes.c_uint8),
+                ("respawn_rng_seed", ctypes.c_uint64)]
+
+
+class _DeviceView(ctypes.Structure):
+    _fields_ = [(name, ctypes.c_void_p) for name in
+                ("car_states", "controls", "car_configs", "teams", "ball_states", "env_states", "boost_pad_states",
+                 "env_car_count")]
+
+
+def library_path() -> str:
+    return LIB_PATH
+
+
+def lib():
+    """Load the HIP library (raises when it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RocketGpuError(f"librocketgpu_amd.so is not built ({LIB_PATH}); run __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        vp = ctypes.c_void_p
+        L.rg_last_error.restype = ctypes.c_char_p
+        L.rg_version.restype = ctypes.c_char_p
+        L.rg_batch_create_procedural.argtypes = [ctypes.POINTER(_SimConfig), ctypes.c_double, ctypes.c_double,
+                                                 ctypes.POINTER(vp)]
+        L.rg_batch_destroy.argtypes = [vp]
+        L.rg_batch_destroy.restype = None
+        L.rg_batch_env_count.argtypes = [vp]
+        L.rg_batch_env_count.restype = ctypes.c_int64
+        L.rg_batch_car_capacity.argtypes = [vp]
+        L.rg_batch_car_capacity.restype = ctypes.c_int64
+        L.rg_batch_set_env_car_count.argtypes = [vp, vp, ctypes.c_size_t, vp]
+        L.rg_batch_upload.argtypes = [vp, ctypes.c_int, vp, ctypes.c_size_t, vp]
+        L.rg_batch_download.argtypes = [vp, ctypes.c_int, vp, ctypes.c_size_t, vp]
+        L.rg_batch_step.argtypes = [vp, ctypes.c_int32, vp]
+        L.rg_batch_step_with_device_controls.argtypes = [vp, vp, ctypes.c_int32, ctypes.c_int32, vp]
+        L.rg_batch_synchronize.argtypes = [vp, vp]
+        L.rg_batch_device_view.argtypes = [vp, ctypes.POINTER(_DeviceView)]
+        L.rg_batch_env_digests.argtypes = [vp, vp, vp]
+        L.rg_workload_ball_states.argtypes = [ctypes.c_uint64, ctypes.c_int64, ctypes.c_int64, ctypes.c_double, vp]
+        L.rg_state_layouts_json.argtypes = [ctypes.c_char_p, ctypes.c_size_t]
+        L.rg_state_layouts_json.restype = ctypes.c_int64
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise RocketGpuError(lib().rg_last_error().decode())
+
+
+def state_layouts() -> dict:
+    """Field layouts of the state records (the reference's state_layouts())."""
+    L = lib()
+    n = L.rg_state_layouts_json(None, 0)
+    buf = ctypes.create_string_buffer(int(n) + 1)
+    L.rg_state_layouts_json(buf, n + 1)
+    return json.loads(buf.value.decode())
+
+
+def workload_ball_states(seed: int, first_arena: int, count: int, scale: float = 1.0) -> np.ndarray:
+    """Seeded ball-launch BallState records ([count, 288] uint8) for the ball-arena workload."""
+    out = np.zeros((count, ITEMSIZE["ball_states"]), dtype=np.uint8)
+    _check(lib().rg_workload_ball_states(seed, first_arena, count, scale, out.ctypes.data))
+    return out
+
+
+def _stream_ptr(stream):
+    if stream is None:
+        return None
+    if isinstance(stream, int):
+        return ctypes.c_void_p(stream)
+    return ctypes.c_void_p(stream.cuda_stream)
+
+
+class BatchSim:
+    """BatchSim over the procedural arena PA(scale, fillet_radius) (include/rocketgpu/rocketgpu.h:988-1089)."""
+
+    def __init__(self, env_count: int, max_cars_per_env: int = 6, scale: float = 1.0, fillet_radius: float = 256.0,
+                 tick_dt: float = 1.0 / 120.0, enable_scoring: bool = True):
+        L = lib()
+        cfg = _SimConfig(env_count, max_cars_per_env, tick_dt, 1, 1, 1, 1 if enable_scoring else 0, 0)
+        h = ctypes.c_void_p()
+        _check(L.rg_batch_create_procedural(ctypes.byref(cfg), scale, fillet_radius, ctypes.byref(h)))
+        self._h = h
+        self.env_count = int(L.rg_batch_env_count(h))
+        self.car_capacity = int(L.rg_batch_car_capacity(h))
+        self.max_cars_per_env = max_cars_per_env
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().rg_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _count(self, name):
+        return {"car_states": self.car_capacity, "controls": self.car_capacity}.get(name, self.env_count)
+
+    def set_env_car_count(self, counts, stream=None):
+        a = np.ascontiguousarray(counts, dtype=np.int32)
+        _check(lib().rg_batch_set_env_car_count(self._h, a.ctypes.data, a.size, _stream_ptr(stream)))
+
+    def upload(self, name: str, rows, stream=None):
+        a = np.ascontiguousarray(rows)
+        n = a.nbytes // ITEMSIZE[name]
+        if n * ITEMSIZE[name] != a.nbytes:
+            raise ValueError(f"{name}: byte size is not a multiple of {ITEMSIZE[name]}")
+        _check(lib().rg_batch_upload(self._h, RG_ARRAY[name], a.ctypes.data, n, _stream_ptr(stream)))
+
+    def download_state_array(self, name: str, stream=None) -> np.ndarray:
+        n = self._count(name)
+        out = np.zeros((n, ITEMSIZE[name]), dtype=np.uint8)
+        _check(lib().rg_batch_download(self._h, RG_ARRAY[name], out.ctypes.data, n, _stream_ptr(stream)))
+        return out
+
+    def step(self, ticks: int = 1, stream=None):
+        _check(lib().rg_batch_step(self._h, ticks, _stream_ptr(stream)))
+
+    def step_with_device_controls(self, device_controls_ptr: int, ticks: int, control_stride: int, stream=None):
+        _check(lib().rg_batch_step_with_device_controls(self._h, ctypes.c_void_p(device_controls_ptr), ticks,
+                                                        control_stride, _stream_ptr(stream)))
+
+    def synchronize(self, stream=None):
+        _check(lib().rg_batch_synchronize(self._h, _stream_ptr(stream)))
+
+    def device_view(self) -> dict:
+        v = _DeviceView()
+        _check(lib().rg_batch_device_view(self._h, ctypes.byref(v)))
+        return {name: getattr(v, name) for name, _ in _DeviceView._fields_}
+
+    def env_digests_into(self, device_ptr: int, stream=None):
+        """Per-env body digests (uint64) into a device buffer of env_count entries."""
+        _check(lib().rg_batch_env_digests(self._h, ctypes.c_void_p(device_ptr), _stream_ptr(stream)))
+
+    def device_arrays(self):
+        """Zero-copy torch views of the state arrays as raw bytes ([n, itemsize] uint8 tensors on cuda:0)."""
+        import torch
+
+        view = self.device_view()
+        out = {}
+        for name in ("ball_states", "env_states", "car_states", "controls"):
+            ptr = view[name]
+            if not ptr:
+                continue
+            n = self._count(name)
+            out[name] = _torch_from_ptr(ptr, (n, ITEMSIZE[name]), torch.uint8)
+        return out
+
+
+def _torch_from_ptr(ptr: int, shape, dtype):
+    import torch
+
+    class _Holder:
+        def __init__(self):
+            self.__cuda_array_interface__ = {"shape": tuple(shape), "typestr": "|u1", "data": (ptr, False),
+                                             "version": 3, "strides": None}
+
+    return torch.as_tensor(_Holder(), device="cuda")

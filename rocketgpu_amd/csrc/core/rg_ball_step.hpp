@@ -486,13 +486,16 @@ RG_HD void RefreshManifold(BallManifoldPoint* c, int& count, const BallState& ba
 // ------------------------------------------------------------------------------------------------------------
 // The tick.
 
+// With the ball alone the manifold list holds at most one manifold per static mesh plus the predictive one.
+constexpr int kMaxBallManifoldEntries = kMaxArenaMeshes + 2;
+
 struct BallTickScratch {
     BodyProxy ball_proxy;
     Vec3 predict_vel_bt;
     uint8_t predictive_hit;
     Vec3 predictive_normal;
     float predictive_distance_bt;
-    ManifoldEntry entries[kMaxManifoldEntries];
+    ManifoldEntry entries[kMaxBallManifoldEntries];
     int entry_count;
     BallRows rows;
 };
@@ -611,9 +614,9 @@ RG_HD void BallBroadphase(ArenaInternal& st, const ArenaMeshView& mesh, BallTick
     int8_t tag[kMaxBodies];
     for (int b = 0; b < kMaxBodies; ++b) tag[b] = -1;
     for (int obj = 0; obj < h.obj_count; ++obj) tag[h.obj[obj]] = static_cast<int8_t>(obj);
-    const int n = bp.manifold_count < kMaxManifoldEntries ? bp.manifold_count : kMaxManifoldEntries;
-    int8_t island[kMaxManifoldEntries];
-    int8_t order[kMaxManifoldEntries];
+    const int n = bp.manifold_count < kMaxBallManifoldEntries ? bp.manifold_count : kMaxBallManifoldEntries;
+    int8_t island[kMaxBallManifoldEntries];
+    int8_t order[kMaxBallManifoldEntries];
     for (int i = 0; i < n; ++i) {
         const uint32_t key = bp.manifolds[i];
         island[i] = tag[KeyKind(key) <= 1 ? 0 : KeyBodyA(key) + 1];

@@ -145,6 +145,18 @@ int rgt_step(void* p, int ticks) {
     return 0;
 }
 
+// Per-env body digests of the ball arenas (no cars), then their fold into `batch` (may be null).
+int rgt_env_digests(void* p, uint64_t* out, uint64_t* batch) {
+    const Twin& t = *static_cast<Twin*>(p);
+    for (size_t e = 0; e < t.balls.size(); ++e) {
+        const BallState& b = t.balls[e];
+        const float rot[4] = {b.rot.w, b.rot.x, b.rot.y, b.rot.z};
+        out[e] = rgw::BodyDigest(rgw::kFnvOffset, &b.pos.x, &b.vel.x, &b.ang_vel.x, rot);
+    }
+    if (batch) *batch = rgw::Fnv1a(rgw::kFnvOffset, out, t.balls.size() * sizeof(uint64_t));
+    return 0;
+}
+
 int rgt_math(int fn, const float* x, const float* y, float* out, long n) {
     for (long i = 0; i < n; ++i) {
         switch (fn) {

@@ -1,0 +1,479 @@
+// BatchSim on MI355X: device buffers, the gfx950 step kernels and the C-ABI (include/rocketgpu_amd/rg_capi.h).
+//
+// Layout in HBM.  Arena geometry (triangles, BVH nodes, mesh infos) is uploaded once per BatchSim and read by every
+// arena.  Per-arena state is three arrays indexed by arena: BallState (288 B, the public record), EnvState (44 B)
+// and ArenaInternal (the broadphase sweep, pair/manifold lists and persistent ball-mesh manifolds that Bullet keeps
+// across ticks).  One lane steps one arena; a launch covers every arena.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "batch_common.hpp"
+#include "rocketgpu_amd/rg_capi.h"
+
+namespace rg {
+namespace {
+
+thread_local std::string g_last_error;
+
+struct InvalidArgument : std::invalid_argument {
+    using std::invalid_argument::invalid_argument;
+};
+
+void Check(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <typename T>
+class DeviceArray {
+public:
+    DeviceArray() = default;
+    DeviceArray(const DeviceArray&) = delete;
+    DeviceArray& operator=(const DeviceArray&) = delete;
+    ~DeviceArray() {
+        if (ptr_) (void)hipFree(ptr_);
+    }
+    void Resize(size_t n) {
+        if (ptr_) Check(hipFree(ptr_), "hipFree");
+        ptr_ = nullptr;
+        size_ = n;
+        if (n) Check(hipMalloc(reinterpret_cast<void**>(&ptr_), n * sizeof(T)), "hipMalloc");
+    }
+    void Upload(const T* src, size_t n, hipStream_t s) {
+        if (n > size_) throw std::out_of_range("upload larger than the device array");
+        if (n) Check(hipMemcpyAsync(ptr_, src, n * sizeof(T), hipMemcpyHostToDevice, s), "hipMemcpyAsync H2D");
+    }
+    void Download(T* dst, size_t n, hipStream_t s) const {
+        if (n > size_) throw std::out_of_range("download larger than the device array");
+        if (n) Check(hipMemcpyAsync(dst, ptr_, n * sizeof(T), hipMemcpyDeviceToHost, s), "hipMemcpyAsync D2H");
+    }
+    T* data() const { return ptr_; }
+    size_t size() const { return size_; }
+
+private:
+    T* ptr_ = nullptr;
+    size_t size_ = 0;
+};
+
+struct StepArgs {
+    BallState* balls;
+    EnvState* envs;
+    core::ArenaInternal* internal;
+    const int32_t* env_car_count;
+    core::ArenaMeshView mesh;
+    core::StepParams params;
+    int32_t env_count;
+};
+
+// One lane per arena; `ticks` ticks back to back with the arena's state in HBM (L2-resident between ticks).
+__global__ void __launch_bounds__(64) StepBallArenasKernel(StepArgs a, int32_t ticks) {
+    const int e = static_cast<int>(blockIdx.x) * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x);
+    if (e >= a.env_count) return;
+    core::BallTickScratch scratch;
+    BallState& ball = a.balls[e];
+    EnvState& env = a.envs[e];
+    core::ArenaInternal& st = a.internal[e];
+    for (int t = 0; t < ticks; ++t) core::StepBallArena(ball, env, st, a.mesh, a.params, scratch);
+}
+
+__device__ inline uint64_t Fnv(uint64_t h, const void* data, int n) {
+    const unsigned char* p = static_cast<const unsigned char*>(data);
+    for (int i = 0; i < n; ++i) {
+        h ^= p[i];
+        h *= rgw::kFnvPrime;
+    }
+    return h;
+}
+
+__global__ void EnvDigestKernel(const BallState* balls, const CarState* cars, const int32_t* env_car_count,
+                                int32_t env_count, int32_t max_cars, uint64_t* out) {
+    const int e = static_cast<int>(blockIdx.x) * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x);
+    if (e >= env_count) return;
+    const BallState& b = balls[e];
+    uint64_t h = rgw::kFnvOffset;
+    const float brot[4] = {b.rot.w, b.rot.x, b.rot.y, b.rot.z};
+    h = Fnv(h, &b.pos, 12);
+    h = Fnv(h, &b.vel, 12);
+    h = Fnv(h, &b.ang_vel, 12);
+    h = Fnv(h, brot, 16);
+    const int n = env_car_count[e];
+    for (int i = 0; i < n; ++i) {
+        const CarState& c = cars[static_cast<size_t>(e) * max_cars + i];
+        const float rot[4] = {c.rot.w, c.rot.x, c.rot.y, c.rot.z};
+        h = Fnv(h, &c.pos, 12);
+        h = Fnv(h, &c.vel, 12);
+        h = Fnv(h, &c.ang_vel, 12);
+        h = Fnv(h, rot, 16);
+    }
+    out[e] = h;
+}
+
+}  // namespace
+
+class BatchSim {
+public:
+    BatchSim(const rg_sim_config& cfg, const std::vector<MeshSource>& meshes, const rgw::ArenaRulesSpec& rules)
+        : cfg_(cfg) {
+        if (cfg.env_count <= 0) throw InvalidArgument("env_count must be positive");
+        if (cfg.max_cars_per_env < 0 || cfg.max_cars_per_env > 8) {
+            throw InvalidArgument("max_cars_per_env must be 0..8");
+        }
+        if (!(cfg.tick_dt > 0.0f)) throw InvalidArgument("tick_dt must be positive");
+        int device = 0;
+        Check(hipGetDevice(&device), "hipGetDevice");
+        mesh_host_ = BuildArenaMesh(meshes);
+        tris_.Resize(mesh_host_.tris.size());
+        nodes_.Resize(mesh_host_.nodes.size());
+        infos_.Resize(mesh_host_.meshes.size());
+        tris_.Upload(mesh_host_.tris.data(), mesh_host_.tris.size(), nullptr);
+        nodes_.Upload(mesh_host_.nodes.data(), mesh_host_.nodes.size(), nullptr);
+        infos_.Upload(mesh_host_.meshes.data(), mesh_host_.meshes.size(), nullptr);
+        params_ = DefaultStepParams(rules);
+        params_.tick_dt = cfg.tick_dt;
+        params_.ball_damping_scale =
+            params_.ball_drag <= 0.0f ? 1.0f
+                                      : core::UcrtPowf(core::Max(0.0f, 1.0f - params_.ball_drag), params_.tick_dt);
+        params_.enable_scoring = cfg.enable_scoring;
+        const size_t n = static_cast<size_t>(cfg.env_count);
+        const size_t cap = n * static_cast<size_t>(cfg.max_cars_per_env);
+        balls_.Resize(n);
+        envs_.Resize(n);
+        internal_.Resize(n);
+        car_count_.Resize(n);
+        cars_.Resize(cap);
+        controls_.Resize(cap);
+        digests_.Resize(n);
+        const std::vector<BallState> balls(n);
+        const std::vector<EnvState> envs(n);
+        const std::vector<int32_t> counts(n, 0);
+        const std::vector<CarState> cars(cap);
+        const std::vector<CarControls> controls(cap);
+        balls_.Upload(balls.data(), n, nullptr);
+        envs_.Upload(envs.data(), n, nullptr);
+        car_count_.Upload(counts.data(), n, nullptr);
+        cars_.Upload(cars.data(), cap, nullptr);
+        controls_.Upload(controls.data(), cap, nullptr);
+        Check(hipMemset(internal_.data(), 0, n * sizeof(core::ArenaInternal)), "hipMemset");
+        Check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    }
+
+    int64_t env_count() const { return cfg_.env_count; }
+    int64_t car_capacity() const { return static_cast<int64_t>(cfg_.env_count) * cfg_.max_cars_per_env; }
+
+    void SetEnvCarCount(const int32_t* counts, size_t n, hipStream_t s) {
+        if (n != static_cast<size_t>(cfg_.env_count)) throw InvalidArgument("set_env_car_count needs env_count values");
+        for (size_t i = 0; i < n; ++i) {
+            if (counts[i] < 0 || counts[i] > cfg_.max_cars_per_env) {
+                throw std::out_of_range("env car count outside 0..max_cars_per_env");
+            }
+            if (counts[i] != 0) {
+                throw InvalidArgument("arenas with cars are not supported by this build yet (ball-only arenas only)");
+            }
+        }
+        car_count_.Upload(counts, n, s);
+        Check(hipStreamSynchronize(s), "hipStreamSynchronize");
+    }
+
+    void Upload(int id, const void* src, size_t n, hipStream_t s) {
+        switch (id) {
+        case RG_ARRAY_BALL_STATES: {
+            Expect(n, balls_.size(), "ball states");
+            std::vector<BallState> tmp(n);
+            std::memcpy(tmp.data(), src, n * sizeof(BallState));
+            for (BallState& b : tmp) CanonicalizeBall(b);
+            balls_.Upload(tmp.data(), n, s);
+            Check(hipStreamSynchronize(s), "hipStreamSynchronize");
+            break;
+        }
+        case RG_ARRAY_ENV_STATES:
+            Expect(n, envs_.size(), "env states");
+            envs_.Upload(static_cast<const EnvState*>(src), n, s);
+            Check(hipStreamSynchronize(s), "hipStreamSynchronize");
+            break;
+        case RG_ARRAY_CONTROLS:
+            Expect(n, controls_.size(), "controls");
+            controls_.Upload(static_cast<const CarControls*>(src), n, s);
+            Check(hipStreamSynchronize(s), "hipStreamSynchronize");
+            break;
+        case RG_ARRAY_CAR_STATES:
+            Expect(n, cars_.size(), "car states");
+            cars_.Upload(static_cast<const CarState*>(src), n, s);
+            Check(hipStreamSynchronize(s), "hipStreamSynchronize");
+            break;
+        default:
+            throw InvalidArgument("unknown or read-only state array id");
+        }
+    }
+
+    void Download(int id, void* dst, size_t n, hipStream_t s) const {
+        switch (id) {
+        case RG_ARRAY_BALL_STATES: balls_.Download(static_cast<BallState*>(dst), n, s); break;
+        case RG_ARRAY_ENV_STATES: envs_.Download(static_cast<EnvState*>(dst), n, s); break;
+        case RG_ARRAY_CAR_STATES: cars_.Download(static_cast<CarState*>(dst), n, s); break;
+        case RG_ARRAY_CONTROLS: controls_.Download(static_cast<CarControls*>(dst), n, s); break;
+        case RG_ARRAY_ENV_CAR_COUNT: car_count_.Download(static_cast<int32_t*>(dst), n, s); break;
+        default: throw InvalidArgument("unknown state array id");
+        }
+        Check(hipStreamSynchronize(s), "hipStreamSynchronize");
+    }
+
+    void Step(int32_t ticks, hipStream_t s) {
+        if (ticks < 0) throw InvalidArgument("ticks must be non-negative");
+        if (ticks == 0) return;
+        StepArgs a{};
+        a.balls = balls_.data();
+        a.envs = envs_.data();
+        a.internal = internal_.data();
+        a.env_car_count = car_count_.data();
+        a.mesh.tris = tris_.data();
+        a.mesh.nodes = nodes_.data();
+        a.mesh.meshes = infos_.data();
+        a.mesh.mesh_count = static_cast<int32_t>(mesh_host_.meshes.size());
+        a.mesh.tri_count = static_cast<int32_t>(mesh_host_.tris.size());
+        a.params = params_;
+        a.env_count = cfg_.env_count;
+        const int block = 64;
+        const int grid = (cfg_.env_count + block - 1) / block;
+        hipLaunchKernelGGL(StepBallArenasKernel, dim3(grid), dim3(block), 0, s, a, ticks);
+        Check(hipGetLastError(), "StepBallArenasKernel launch");
+    }
+
+    void StepWithDeviceControls(const void* controls, int32_t ticks, int32_t stride, hipStream_t s) {
+        if (stride != 0 && stride < car_capacity()) {
+            throw InvalidArgument("control_stride must be 0 or at least car_capacity");
+        }
+        if (ticks > 0 && car_capacity() > 0 && controls == nullptr) throw InvalidArgument("controls pointer is null");
+        Step(ticks, s);
+    }
+
+    void DeviceView(rg_device_view* v) const {
+        std::memset(v, 0, sizeof(*v));
+        v->car_states = cars_.data();
+        v->controls = controls_.data();
+        v->ball_states = balls_.data();
+        v->env_states = envs_.data();
+        v->env_car_count = car_count_.data();
+    }
+
+    void EnvDigests(uint64_t* out, hipStream_t s) {
+        const int block = 256;
+        const int grid = (cfg_.env_count + block - 1) / block;
+        hipLaunchKernelGGL(EnvDigestKernel, dim3(grid), dim3(block), 0, s, balls_.data(), cars_.data(),
+                           car_count_.data(), cfg_.env_count, cfg_.max_cars_per_env, out);
+        Check(hipGetLastError(), "EnvDigestKernel launch");
+    }
+
+private:
+    static void Expect(size_t n, size_t want, const char* what) {
+        if (n != want) throw InvalidArgument(std::string(what) + ": count must equal the array size");
+    }
+
+    rg_sim_config cfg_;
+    ArenaMeshHost mesh_host_;
+    core::StepParams params_{};
+    DeviceArray<core::MeshTriangle> tris_;
+    DeviceArray<core::MeshBvhNode> nodes_;
+    DeviceArray<core::MeshInfo> infos_;
+    DeviceArray<BallState> balls_;
+    DeviceArray<EnvState> envs_;
+    DeviceArray<core::ArenaInternal> internal_;
+    DeviceArray<int32_t> car_count_;
+    DeviceArray<CarState> cars_;
+    DeviceArray<CarControls> controls_;
+    DeviceArray<uint64_t> digests_;
+};
+
+}  // namespace rg
+
+struct rg_batch {
+    std::unique_ptr<rg::BatchSim> sim;
+};
+
+namespace {
+
+template <typename F>
+int Guard(F&& f) {
+    try {
+        f();
+        return RG_OK;
+    } catch (const rg::InvalidArgument& e) {
+        rg::g_last_error = e.what();
+        return RG_EINVAL;
+    } catch (const std::invalid_argument& e) {
+        rg::g_last_error = e.what();
+        return RG_EINVAL;
+    } catch (const std::out_of_range& e) {
+        rg::g_last_error = e.what();
+        return RG_ERANGE;
+    } catch (const std::exception& e) {
+        rg::g_last_error = e.what();
+        return RG_ERUNTIME;
+    }
+}
+
+hipStream_t S(void* s) { return static_cast<hipStream_t>(s); }
+
+rgw::ArenaRulesSpec RulesFromLayout(const rg_arena_layout& l) {
+    rgw::ArenaRulesSpec r{};
+    for (int t = 0; t < 2; ++t) {
+        const float* g = l.goals[t];
+        r.goals[t].point = rgw::V3{g[0], g[1], g[2]};
+        r.goals[t].normal = rgw::V3{g[3], g[4], g[5]};
+        r.goals[t].min = rgw::V3{g[6], g[7], g[8]};
+        r.goals[t].max = rgw::V3{g[9], g[10], g[11]};
+        const float len = std::sqrt(g[3] * g[3] + g[4] * g[4] + g[5] * g[5]);
+        if (!(len > 0.0f) || !std::isfinite(len)) throw rg::InvalidArgument("ArenaGoal normal must be nonzero");
+    }
+    return r;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* rg_last_error(void) { return rg::g_last_error.c_str(); }
+const char* rg_version(void) { return "rocketgpu_amd 0.1 (gfx950; ball-only arenas)"; }
+
+int rg_batch_create(const rg_sim_config* config, const rg_arena_layout* layout, rg_batch** out) {
+    return Guard([&] {
+        if (!config || !layout || !out) throw rg::InvalidArgument("null argument");
+        std::vector<std::vector<rg::Vec3>> verts(static_cast<size_t>(layout->mesh_count));
+        std::vector<std::vector<int32_t>> idx(static_cast<size_t>(layout->mesh_count));
+        size_t vo = 0, io = 0;
+        for (int m = 0; m < layout->mesh_count; ++m) {
+            for (int v = 0; v < layout->vertex_counts[m]; ++v, ++vo) {
+                const float* p = layout->vertices_uu + 3 * vo;
+                if (!(std::abs(p[0]) <= 11500.0f && std::abs(p[1]) <= 11500.0f && std::abs(p[2]) <= 11500.0f)) {
+                    throw rg::InvalidArgument(
+                        "ArenaLayout: a mesh vertex must be finite and within kArenaLayoutMaxCoord (11500 uu)");
+                }
+                verts[static_cast<size_t>(m)].push_back(rg::Vec3(p[0], p[1], p[2]));
+            }
+            for (int i = 0; i < layout->index_counts[m]; ++i, ++io) idx[static_cast<size_t>(m)].push_back(layout->indices[io]);
+        }
+        auto b = std::make_unique<rg_batch>();
+        b->sim = std::make_unique<rg::BatchSim>(*config, rg::GroupLayoutMeshes(verts, idx), RulesFromLayout(*layout));
+        *out = b.release();
+    });
+}
+
+int rg_batch_create_procedural(const rg_sim_config* config, double scale, double fillet_radius, rg_batch** out) {
+    return Guard([&] {
+        if (!config || !out) throw rg::InvalidArgument("null argument");
+        rgw::ArenaParams ap;
+        ap.scale = scale;
+        ap.fillet_radius = fillet_radius;
+        const rgw::ArenaGeometry g = rgw::BuildProceduralArena(ap);
+        std::vector<std::vector<rg::Vec3>> verts(1);
+        for (const rgw::V3& v : g.vertices) verts[0].push_back(rg::Vec3(v.x, v.y, v.z));
+        std::vector<std::vector<int32_t>> idx(1, g.indices);
+        auto b = std::make_unique<rg_batch>();
+        b->sim = std::make_unique<rg::BatchSim>(*config, rg::GroupLayoutMeshes(verts, idx),
+                                                rgw::ProceduralArenaRules(ap));
+        *out = b.release();
+    });
+}
+
+void rg_batch_destroy(rg_batch* batch) { delete batch; }
+
+int64_t rg_batch_env_count(const rg_batch* b) { return b ? b->sim->env_count() : -1; }
+int64_t rg_batch_car_capacity(const rg_batch* b) { return b ? b->sim->car_capacity() : -1; }
+
+int rg_batch_set_env_car_count(rg_batch* b, const int32_t* counts, size_t count, void* stream) {
+    return Guard([&] { b->sim->SetEnvCarCount(counts, count, S(stream)); });
+}
+
+int rg_batch_upload(rg_batch* b, int id, const void* src, size_t count, void* stream) {
+    return Guard([&] { b->sim->Upload(id, src, count, S(stream)); });
+}
+
+int rg_batch_download(rg_batch* b, int id, void* dst, size_t count, void* stream) {
+    return Guard([&] { b->sim->Download(id, dst, count, S(stream)); });
+}
+
+int rg_batch_step(rg_batch* b, int32_t ticks, void* stream) {
+    return Guard([&] { b->sim->Step(ticks, S(stream)); });
+}
+
+int rg_batch_step_with_device_controls(rg_batch* b, const void* controls, int32_t ticks, int32_t stride,
+                                       void* stream) {
+    return Guard([&] { b->sim->StepWithDeviceControls(controls, ticks, stride, S(stream)); });
+}
+
+int rg_batch_synchronize(rg_batch* b, void* stream) {
+    (void)b;
+    return Guard([&] { rg::Check(hipStreamSynchronize(S(stream)), "hipStreamSynchronize"); });
+}
+
+int rg_batch_device_view(rg_batch* b, rg_device_view* out) {
+    return Guard([&] { b->sim->DeviceView(out); });
+}
+
+int rg_batch_env_digests(rg_batch* b, uint64_t* device_out, void* stream) {
+    return Guard([&] { b->sim->EnvDigests(device_out, S(stream)); });
+}
+
+uint64_t rg_fold_digests(const uint64_t* d, int64_t n) {
+    return n > 0 && d ? rgw::Fnv1a(rgw::kFnvOffset, d, static_cast<size_t>(n) * sizeof(uint64_t)) : rgw::kFnvOffset;
+}
+
+int rg_workload_ball_states(uint64_t seed, int64_t first_arena, int64_t count, double scale, void* out) {
+    return Guard([&] {
+        if (count < 0 || (count > 0 && !out)) throw rg::InvalidArgument("bad workload request");
+        rg::BallState* b = static_cast<rg::BallState*>(out);
+        for (int64_t i = 0; i < count; ++i) {
+            const rgw::BallLaunch l = rgw::SeededBallLaunch(seed, static_cast<uint64_t>(first_arena + i), scale);
+            rg::BallState s{};
+            s.pos = rg::Vec3(l.pos[0], l.pos[1], l.pos[2]);
+            s.vel = rg::Vec3(l.vel[0], l.vel[1], l.vel[2]);
+            s.ang_vel = rg::Vec3(l.ang_vel[0], l.ang_vel[1], l.ang_vel[2]);
+            b[i] = s;
+        }
+    });
+}
+
+int64_t rg_state_layouts_json(char* buf, size_t capacity) {
+    std::string j = "{";
+    const auto field = [&](const char* name, const char* type, size_t off, int n, bool last) {
+        char tmp[160];
+        std::snprintf(tmp, sizeof(tmp), "[\"%s\", \"%s\", %zu, %d]%s", name, type, off, n, last ? "" : ", ");
+        j += tmp;
+    };
+    j += "\"ball_states\": {\"itemsize\": " + std::to_string(sizeof(rg::BallState)) + ", \"fields\": [";
+    field("pos", "<f4", offsetof(rg::BallState, pos), 3, false);
+    field("rot", "<f4", offsetof(rg::BallState, rot), 4, false);
+    field("vel", "<f4", offsetof(rg::BallState, vel), 3, false);
+    field("ang_vel", "<f4", offsetof(rg::BallState, ang_vel), 3, false);
+    field("tick_count_since_update", "<u8", offsetof(rg::BallState, tick_count_since_update), 1, false);
+    field("pos_bt", "<f4", offsetof(rg::BallState, pos_bt), 3, false);
+    field("vel_bt", "<f4", offsetof(rg::BallState, vel_bt), 3, false);
+    field("sleeping", "<u4", offsetof(rg::BallState, sleeping), 1, false);
+    field("deactivation_time", "<f4", offsetof(rg::BallState, deactivation_time), 1, true);
+    j += "]}, \"env_states\": {\"itemsize\": " + std::to_string(sizeof(rg::EnvState)) + ", \"fields\": [";
+    field("tick", "<u4", offsetof(rg::EnvState, tick), 1, false);
+    field("blue_score", "<u4", offsetof(rg::EnvState, blue_score), 1, false);
+    field("orange_score", "<u4", offsetof(rg::EnvState, orange_score), 1, false);
+    field("last_scored_team", "<i4", offsetof(rg::EnvState, last_scored_team), 1, false);
+    field("goal_scored_this_tick", "|u1", offsetof(rg::EnvState, goal_scored_this_tick), 1, true);
+    j += "]}, \"car_states\": {\"itemsize\": " + std::to_string(sizeof(rg::CarState)) + ", \"fields\": [";
+    field("pos", "<f4", offsetof(rg::CarState, pos), 3, false);
+    field("rot", "<f4", offsetof(rg::CarState, rot), 4, false);
+    field("vel", "<f4", offsetof(rg::CarState, vel), 3, false);
+    field("ang_vel", "<f4", offsetof(rg::CarState, ang_vel), 3, false);
+    field("boost", "<f4", offsetof(rg::CarState, boost), 1, true);
+    j += "]}, \"controls\": {\"itemsize\": " + std::to_string(sizeof(rg::CarControls)) + "}}";
+    if (buf && capacity > 0) {
+        const size_t n = std::min(capacity - 1, j.size());
+        std::memcpy(buf, j.data(), n);
+        buf[n] = 0;
+    }
+    return static_cast<int64_t>(j.size());
+}
+
+}  // extern "C"
