@@ -10,6 +10,7 @@
 #include <cstring>
 #include <exception>
 #include <memory>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -315,13 +316,26 @@ int rgref_mesh_tables(void* p, long* counts, float* tris, int32_t* nodes, float*
 }
 
 // Default-constructed record bytes of the reference's structs (same ids as rgref_sizeof).
+// Default-constructed records with every padding byte zero (the records are built in zeroed storage), so that
+// fixtures written from them are deterministic.
+extern "C++" {
+template <typename T>
+static void DefaultInto(void* out) {
+    alignas(T) unsigned char buf[sizeof(T)];
+    std::memset(buf, 0, sizeof(T));
+    T* v = new (buf) T;
+    std::memcpy(out, v, sizeof(T));
+    v->~T();
+}
+}
+
 int rgref_default_state(int which, void* out) {
     switch (which) {
-    case 0: { const CarState v{}; std::memcpy(out, &v, sizeof(v)); return 0; }
-    case 1: { const BallState v{}; std::memcpy(out, &v, sizeof(v)); return 0; }
-    case 2: { const EnvState v{}; std::memcpy(out, &v, sizeof(v)); return 0; }
-    case 3: { const BoostPadState v{}; std::memcpy(out, &v, sizeof(v)); return 0; }
-    case 4: { const CarControls v{}; std::memcpy(out, &v, sizeof(v)); return 0; }
+    case 0: DefaultInto<CarState>(out); return 0;
+    case 1: DefaultInto<BallState>(out); return 0;
+    case 2: DefaultInto<EnvState>(out); return 0;
+    case 3: DefaultInto<BoostPadState>(out); return 0;
+    case 4: DefaultInto<CarControls>(out); return 0;
     default: return -1;
     }
 }
@@ -331,11 +345,10 @@ int rgref_workload_ball_states(uint64_t seed, long first_arena, long count, doub
     BallState* b = static_cast<BallState*>(out);
     for (long i = 0; i < count; ++i) {
         const rgw::BallLaunch l = rgw::SeededBallLaunch(seed, static_cast<uint64_t>(first_arena + i), scale);
-        BallState s{};
-        s.pos = Vec3(l.pos[0], l.pos[1], l.pos[2]);
-        s.vel = Vec3(l.vel[0], l.vel[1], l.vel[2]);
-        s.ang_vel = Vec3(l.ang_vel[0], l.ang_vel[1], l.ang_vel[2]);
-        b[i] = s;
+        DefaultInto<BallState>(&b[i]);
+        b[i].pos = Vec3(l.pos[0], l.pos[1], l.pos[2]);
+        b[i].vel = Vec3(l.vel[0], l.vel[1], l.vel[2]);
+        b[i].ang_vel = Vec3(l.ang_vel[0], l.ang_vel[1], l.ang_vel[2]);
     }
     return 0;
 }

@@ -79,6 +79,8 @@ def lib():
         L.rg_batch_device_view.argtypes = [vp, ctypes.POINTER(_DeviceView)]
         L.rg_batch_env_digests.argtypes = [vp, vp, vp]
         L.rg_workload_ball_states.argtypes = [ctypes.c_uint64, ctypes.c_int64, ctypes.c_int64, ctypes.c_double, vp]
+        L.rg_fold_digests.argtypes = [vp, ctypes.c_int64]
+        L.rg_fold_digests.restype = ctypes.c_uint64
         L.rg_state_layouts_json.argtypes = [ctypes.c_char_p, ctypes.c_size_t]
         L.rg_state_layouts_json.restype = ctypes.c_int64
         _lib = L

@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <new>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -429,45 +430,90 @@ int rg_workload_ball_states(uint64_t seed, int64_t first_arena, int64_t count, d
         rg::BallState* b = static_cast<rg::BallState*>(out);
         for (int64_t i = 0; i < count; ++i) {
             const rgw::BallLaunch l = rgw::SeededBallLaunch(seed, static_cast<uint64_t>(first_arena + i), scale);
-            rg::BallState s{};
-            s.pos = rg::Vec3(l.pos[0], l.pos[1], l.pos[2]);
-            s.vel = rg::Vec3(l.vel[0], l.vel[1], l.vel[2]);
-            s.ang_vel = rg::Vec3(l.ang_vel[0], l.ang_vel[1], l.ang_vel[2]);
-            b[i] = s;
+            std::memset(static_cast<void*>(&b[i]), 0, sizeof(rg::BallState));
+            rg::BallState* s = new (&b[i]) rg::BallState;
+            s->pos = rg::Vec3(l.pos[0], l.pos[1], l.pos[2]);
+            s->vel = rg::Vec3(l.vel[0], l.vel[1], l.vel[2]);
+            s->ang_vel = rg::Vec3(l.ang_vel[0], l.ang_vel[1], l.ang_vel[2]);
         }
     });
 }
 
 int64_t rg_state_layouts_json(char* buf, size_t capacity) {
+    // {"name": {"itemsize": n, "fields": [[field, typestr, offset, [shape...]], ...]}, ...}; scalars have shape [],
+    // as the reference's state_layouts() gives them (python/rocketgpu_bindings.cpp:543-560).
     std::string j = "{";
-    const auto field = [&](const char* name, const char* type, size_t off, int n, bool last) {
-        char tmp[160];
-        std::snprintf(tmp, sizeof(tmp), "[\"%s\", \"%s\", %zu, %d]%s", name, type, off, n, last ? "" : ", ");
-        j += tmp;
+    bool first_field = true;
+    const auto begin = [&](const char* name, size_t itemsize, bool first) {
+        j += std::string(first ? "" : "]}, ") + "\"" + name + "\": {\"itemsize\": " + std::to_string(itemsize) +
+             ", \"fields\": [";
+        first_field = true;
     };
-    j += "\"ball_states\": {\"itemsize\": " + std::to_string(sizeof(rg::BallState)) + ", \"fields\": [";
-    field("pos", "<f4", offsetof(rg::BallState, pos), 3, false);
-    field("rot", "<f4", offsetof(rg::BallState, rot), 4, false);
-    field("vel", "<f4", offsetof(rg::BallState, vel), 3, false);
-    field("ang_vel", "<f4", offsetof(rg::BallState, ang_vel), 3, false);
-    field("tick_count_since_update", "<u8", offsetof(rg::BallState, tick_count_since_update), 1, false);
-    field("pos_bt", "<f4", offsetof(rg::BallState, pos_bt), 3, false);
-    field("vel_bt", "<f4", offsetof(rg::BallState, vel_bt), 3, false);
-    field("sleeping", "<u4", offsetof(rg::BallState, sleeping), 1, false);
-    field("deactivation_time", "<f4", offsetof(rg::BallState, deactivation_time), 1, true);
-    j += "]}, \"env_states\": {\"itemsize\": " + std::to_string(sizeof(rg::EnvState)) + ", \"fields\": [";
-    field("tick", "<u4", offsetof(rg::EnvState, tick), 1, false);
-    field("blue_score", "<u4", offsetof(rg::EnvState, blue_score), 1, false);
-    field("orange_score", "<u4", offsetof(rg::EnvState, orange_score), 1, false);
-    field("last_scored_team", "<i4", offsetof(rg::EnvState, last_scored_team), 1, false);
-    field("goal_scored_this_tick", "|u1", offsetof(rg::EnvState, goal_scored_this_tick), 1, true);
-    j += "]}, \"car_states\": {\"itemsize\": " + std::to_string(sizeof(rg::CarState)) + ", \"fields\": [";
-    field("pos", "<f4", offsetof(rg::CarState, pos), 3, false);
-    field("rot", "<f4", offsetof(rg::CarState, rot), 4, false);
-    field("vel", "<f4", offsetof(rg::CarState, vel), 3, false);
-    field("ang_vel", "<f4", offsetof(rg::CarState, ang_vel), 3, false);
-    field("boost", "<f4", offsetof(rg::CarState, boost), 1, true);
-    j += "]}, \"controls\": {\"itemsize\": " + std::to_string(sizeof(rg::CarControls)) + "}}";
+    const auto field = [&](const char* name, const char* type, size_t off, int n) {
+        char tmp[192];
+        std::snprintf(tmp, sizeof(tmp), "%s[\"%s\", \"%s\", %zu, [%s]]", first_field ? "" : ", ", name, type, off,
+                      n == 0 ? "" : std::to_string(n).c_str());
+        j += tmp;
+        first_field = false;
+    };
+#define RG_F(T, f, type, n) field(#f, type, offsetof(T, f), n)
+    begin("ball_states", sizeof(rg::BallState), true);
+    RG_F(rg::BallState, pos, "<f4", 3);
+    RG_F(rg::BallState, rot, "<f4", 4);
+    RG_F(rg::BallState, vel, "<f4", 3);
+    RG_F(rg::BallState, ang_vel, "<f4", 3);
+    RG_F(rg::BallState, vel_impulse_cache, "<f4", 3);
+    RG_F(rg::BallState, tick_count_when_extra_impulse_applied, "<u8", 0);
+    RG_F(rg::BallState, tick_count_since_update, "<u8", 0);
+    RG_F(rg::BallState, pos_bt, "<f4", 3);
+    RG_F(rg::BallState, vel_bt, "<f4", 3);
+    RG_F(rg::BallState, basis_forward, "<f4", 3);
+    RG_F(rg::BallState, basis_right, "<f4", 3);
+    RG_F(rg::BallState, basis_up, "<f4", 3);
+    RG_F(rg::BallState, sleeping, "<u4", 0);
+    RG_F(rg::BallState, deactivation_time, "<f4", 0);
+    RG_F(rg::BallState, constrained_2d, "<u2", 0);
+    begin("env_states", sizeof(rg::EnvState), false);
+    RG_F(rg::EnvState, tick, "<u4", 0);
+    RG_F(rg::EnvState, blue_score, "<u4", 0);
+    RG_F(rg::EnvState, orange_score, "<u4", 0);
+    RG_F(rg::EnvState, last_scored_team, "<i4", 0);
+    RG_F(rg::EnvState, goal_scored_this_tick, "|u1", 0);
+    RG_F(rg::EnvState, car_visit_order_mode, "|u1", 0);
+    RG_F(rg::EnvState, car_visit_present_mask, "|u1", 0);
+    RG_F(rg::EnvState, kickoff_resets, "|u1", 0);
+    RG_F(rg::EnvState, car_visit_order_packed, "<u4", 0);
+    RG_F(rg::EnvState, countdown_phase, "|u1", 0);
+    RG_F(rg::EnvState, countdown_fires, "|u1", 0);
+    RG_F(rg::EnvState, frame_ticks, "<u2", 0);
+    RG_F(rg::EnvState, countdown_count, "<f4", 0);
+    RG_F(rg::EnvState, frame_frac_prev, "<f4", 0);
+    RG_F(rg::EnvState, kickoff_match_order, "<u2", 0);
+    RG_F(rg::EnvState, kickoff_match_index, "|u1", 0);
+    RG_F(rg::EnvState, goal_latched, "|u1", 0);
+    RG_F(rg::EnvState, respawn_match_order, "|u1", 0);
+    begin("car_states", sizeof(rg::CarState), false);
+    RG_F(rg::CarState, pos, "<f4", 3);
+    RG_F(rg::CarState, rot, "<f4", 4);
+    RG_F(rg::CarState, vel, "<f4", 3);
+    RG_F(rg::CarState, ang_vel, "<f4", 3);
+    RG_F(rg::CarState, boost, "<f4", 0);
+    RG_F(rg::CarState, is_on_ground, "|u1", 0);
+    RG_F(rg::CarState, is_demoed, "|u1", 0);
+    RG_F(rg::CarState, pos_bt, "<f4", 3);
+    RG_F(rg::CarState, vel_bt, "<f4", 3);
+    RG_F(rg::CarState, tick_count_since_update, "<u8", 0);
+    begin("controls", sizeof(rg::CarControls), false);
+    RG_F(rg::CarControls, throttle, "<f4", 0);
+    RG_F(rg::CarControls, steer, "<f4", 0);
+    RG_F(rg::CarControls, pitch, "<f4", 0);
+    RG_F(rg::CarControls, yaw, "<f4", 0);
+    RG_F(rg::CarControls, roll, "<f4", 0);
+    RG_F(rg::CarControls, jump, "|u1", 0);
+    RG_F(rg::CarControls, boost, "|u1", 0);
+    RG_F(rg::CarControls, handbrake, "|u1", 0);
+#undef RG_F
+    j += "]}}";
     if (buf && capacity > 0) {
         const size_t n = std::min(capacity - 1, j.size());
         std::memcpy(buf, j.data(), n);
