@@ -1,7 +1,8 @@
 """Generates rg_math_tables.inc: the lookup tables of the FP32 trigonometric / power kernels in rg_math.hpp.
 
-Every entry is a mathematical constant rounded to the nearest double (computed here with 80-digit decimal
-arithmetic, then rounded exactly through fractions.Fraction):
+Every entry is a mathematical constant computed here with 80-digit decimal arithmetic and rounded exactly to a
+double through fractions.Fraction: to nearest, except the atan table, which the UCRT kernel stores rounded
+toward zero:
 
   atan_tab[k]  = atan((k + 16) / 256)          k = 0..240   (the atan2f table lookup; rounded toward zero)
   exp2_tab[k]  = 2 ** (k / 64)                 k = 0..63    (the powf exp step)
