@@ -5,13 +5,17 @@ with the state resident in HBM.  Timing: HIP events recorded on the stream the s
 each step; the timed region (K steps) is bracketed by device synchronization.
 
 The workload is 16,384 ball arenas on the procedural arena PA(1, 256) (20,332 triangles) with seeded ball launches
-and T = 8 ticks per step; cars are not simulated by this build yet, so the arenas carry no cars (named as such in
-config.workload).  The roofline object prices the step kernel's algorithmic bytes per arena-tick
-(2 x (BallState 288 + EnvState 44 + 34 pads x 16 + world solver state 11,712) + 4 = 25,180 B at 0 cars) against
-the 8 TB/s HBM peak.  cpu_baseline times the reference CPU build (oracle/_ref) on a bounded sample of the same
-workload on this machine's host cores.
+and T = 8 ticks per step.  Cars are not simulated by this build yet, so the arenas carry no cars; config.workload
+says so.  The roofline object prices the step kernel's algorithmic bytes per arena-tick against the 8 TB/s HBM
+peak: 2 x (BallState 288 + EnvState 44 + 34 pads x 16 + world solver state 11,712) + 4 = 25,180 B at 0 cars.
+cpu_baseline times the reference CPU build (oracle/_ref) on this machine's host cores over the same tick window of
+the same workload.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--arenas N] [--ticks T] [--cpu-ticks C]
+Beside the headline: the launch block chosen by a short sweep (64/128/256 lanes), the other step forms on the same
+batch (T launches of one tick, one hipGraph replay), and the serving loop (simulated clients, Poisson arrivals,
+continuous batching over 262,144 resident arenas) with p50/p99 request latency.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--arenas N] [--ticks T] [--no-serve] [--cpu-ticks C]
 """
 from __future__ import annotations
 
@@ -25,6 +29,8 @@ REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, REPO)
 
 HBM_PEAK_GBS = 8000.0
+
+
 # Algorithmic bytes per arena-tick at n cars: B(n) = 2 * (12588 + 2456 n + 130 n (n - 1)) + 144 n + 4.
 def bytes_per_arena_tick(n_cars: float) -> float:
     return 2.0 * (12588 + 2456 * n_cars + 130 * n_cars * (n_cars - 1)) + 144 * n_cars + 4
@@ -39,17 +45,46 @@ def parse():
     p.add_argument("--ticks", type=int, default=8, help="ticks per step (T of step_with_device_controls)")
     p.add_argument("--max-cars", type=int, default=6)
     p.add_argument("--seed", type=int, default=2024)
-    p.add_argument("--cpu-ticks", type=int, default=240, help="ticks of the reference CPU sample (0: skip)")
+    p.add_argument("--block", type=int, default=0, help="launch block (0: choose by a short sweep)")
+    p.add_argument("--no-forms", action="store_true")
+    p.add_argument("--no-serve", action="store_true")
+    p.add_argument("--serve-arenas", type=int, default=262144)
+    p.add_argument("--cpu-ticks", type=int, default=-1,
+                   help="ticks of the reference CPU sample (-1: the GPU's timed window; 0: skip)")
     p.add_argument("--traffic", default=os.path.join(REPO, "profiles", "step_ball_arenas_16384_traffic.json"))
+    p.add_argument("--calibrate-copy-mib", type=int, default=0,
+                   help="first run one plain device-to-device copy of this many MiB (a known-size reference for "
+                        "the FETCH_SIZE / WRITE_SIZE counters)")
     return p.parse_args()
 
 
-def cpu_baseline(args, init_ball, init_env):
-    """The reference CPU build on this host: the same arenas, args.cpu_ticks ticks after an 8-tick warm-up."""
-    if args.cpu_ticks <= 0:
+def timed(torch, stream, fn, reps):
+    """Average ms per call of fn() from HIP events recorded on `stream` around each call."""
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
+    for a, b in ev:
+        a.record(stream)
+        fn()
+        b.record(stream)
+    stream.synchronize()
+    return sum(a.elapsed_time(b) for a, b in ev) / reps
+
+
+def make_batch(rga, np, n, max_cars, seed):
+    sim = rga.BatchSim(n, max_cars)
+    sim.set_env_car_count(np.zeros(n, np.int32))
+    init_env = sim.download_state_array("env_states")
+    init_ball = rga.workload_ball_states(seed, 0, n, 1.0)
+    sim.upload("ball_states", init_ball)
+    return sim, init_ball, init_env
+
+
+def cpu_baseline(args, init_ball, init_env, start_tick, ticks):
+    """The reference CPU build on this host: the same arenas over the same tick window as the GPU's timed steps."""
+    if ticks <= 0:
         return None
     cores = min(16, os.cpu_count() or 1)
     os.environ["ROCKETGPU_CPU_THREADS"] = str(cores)
+    import numpy as np
     from oracle import refsim as R
 
     if not R.available():
@@ -57,16 +92,17 @@ def cpu_baseline(args, init_ball, init_env):
                 "sample": "reference CPU build (oracle/_ref) not present on this machine"}
     n = init_ball.shape[0]
     ref = R.RefSim(n, 2, 1.0, 256.0)
-    ref.set_env_car_count(__import__("numpy").zeros(n, "int32"))
+    ref.set_env_car_count(np.zeros(n, np.int32))
     ref.upload(R.ENV, init_env)
     ref.upload(R.BALL, init_ball)
-    ref.step(8)
+    if start_tick > 0:
+        ref.step(start_tick)
     t0 = time.perf_counter()
-    ref.step(args.cpu_ticks)
+    ref.step(ticks)
     dt = time.perf_counter() - t0
-    return {"value": n * args.cpu_ticks / dt, "unit": "arena-ticks/s", "cores": cores, "kind": "reference",
-            "sample": f"{n} ball arenas x {args.cpu_ticks} ticks (after 8 warm-up ticks) in {dt:.2f} s, "
-                      f"ROCKETGPU_CPU_THREADS={cores}"}
+    return {"value": n * ticks / dt, "unit": "arena-ticks/s", "cores": cores, "kind": "reference",
+            "sample": f"{n} ball arenas, ticks {start_tick}..{start_tick + ticks} (the GPU's timed window) in "
+                      f"{dt:.2f} s, ROCKETGPU_CPU_THREADS={cores}"}
 
 
 def main():
@@ -82,18 +118,31 @@ def main():
 
     torch.cuda.set_device(0)
     n, T = args.arenas, args.ticks
-    sim = rga.BatchSim(n, args.max_cars)
-    sim.set_env_car_count(np.zeros(n, np.int32))
-    init_env = np.zeros((n, 44), np.uint8)
-    # Default EnvState records (tick 0, no score, last_scored_team -1, frame fraction 0.5).
-    init_env[:] = sim.download_state_array("env_states")[0]
-    init_ball = rga.workload_ball_states(args.seed, 0, n, 1.0)
-    sim.upload("env_states", init_env)
-    sim.upload("ball_states", init_ball)
-
+    if args.calibrate_copy_mib > 0:
+        src = torch.ones(args.calibrate_copy_mib << 18, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     stream = torch.cuda.Stream()
     g = torch.Generator(device="cuda")
     g.manual_seed(args.seed)
+
+    # Launch block: a short sweep on a scratch copy of the workload (same trajectory window as the warm-up).
+    block = args.block
+    sweep = {}
+    if block == 0:
+        for b in (64, 128, 256):
+            sim_b, _, _ = make_batch(rga, np, n, args.max_cars, args.seed)
+            sim_b.set_launch_block(b)
+            sim_b.step(T, stream=stream)
+            sweep[b] = timed(torch, stream, lambda: sim_b.step(T, stream=stream), 4)
+            sim_b.close()
+        block = min(sweep, key=sweep.get)
+
+    sim, init_ball, init_env = make_batch(rga, np, n, args.max_cars, args.seed)
+    sim.set_launch_block(block)
     ctrl = torch.randint(0, 256, (T, sim.car_capacity * 24), dtype=torch.uint8, device="cuda", generator=g)
     torch.cuda.synchronize()
     for _ in range(args.warmup):
@@ -128,20 +177,49 @@ def main():
     envs = sim.download_state_array("env_states")
     ticks_done = envs[:, 0:4].copy().view(np.uint32)[:, 0]
     assert np.all(ticks_done == (args.warmup + args.steps) * T), "tick counters do not match the steps run"
+
+    forms = None
+    if not args.no_forms:
+        # The same batch, continuing its trajectory: T single-tick launches, one T-tick launch, one graph replay.
+        per_tick = timed(torch, stream, lambda: [sim.step(1, stream=stream) for _ in range(T)], 10)
+        multi = timed(torch, stream, lambda: sim.step(T, stream=stream), 10)
+        sim.step_graph(1, T, stream=stream)
+        graph_1xT = timed(torch, stream, lambda: sim.step_graph(1, T, stream=stream), 10)
+        sim.step_graph(T, 1, stream=stream)
+        graph_T = timed(torch, stream, lambda: sim.step_graph(T, 1, stream=stream), 10)
+        forms = {"ms_per_T_ticks": {"plain_T_launches_of_1_tick": per_tick, "one_launch_of_T_ticks": multi,
+                                    "hipgraph_T_launches_of_1_tick": graph_1xT, "hipgraph_one_launch_of_T": graph_T},
+                 "T": T}
     sim.close()
 
-    cpu = cpu_baseline(args, init_ball, init_env)
+    serving = None
+    if not args.no_serve:
+        ssim, _, _ = make_batch(rga, np, args.serve_arenas, 2, args.seed + 1)
+        ssim.set_launch_block(block)
+        serving = ssim.serve(clients=256, min_arenas=16, max_arenas=256, horizons=(1, 4, 8),
+                             max_arenas_per_launch=65536, max_in_flight=2, request_rate_hz=2000.0, duration_s=2.0,
+                             seed=args.seed)
+        serving["config"] = ("256 clients x U{16..256} resident ball arenas, Poisson 2,000 req/s aggregate for 2 s, "
+                             "k in {1, 4, 8} ticks, <= 65,536 arenas per launch, 2 launches in flight; "
+                             f"{args.serve_arenas} arenas resident in HBM")
+        ssim.close()
+
+    cpu_ticks = args.steps * T if args.cpu_ticks < 0 else args.cpu_ticks
+    cpu = cpu_baseline(args, init_ball, init_env, args.warmup * T, cpu_ticks)
     line = {
         "metric": "arena-ticks/s", "value": value, "unit": "arena-ticks/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded ball launches, procedural arena)",
         "config": {"workload": f"{n} ball arenas (no cars), procedural arena PA(1, 256) 20,332 triangles, "
                                f"step_with_device_controls T={T}",
-                   "arenas": n, "ticks_per_step": T, "cars_per_arena": 0, "parallelism": "single GPU"},
+                   "arenas": n, "ticks_per_step": T, "cars_per_arena": 0, "parallelism": "single GPU",
+                   "launch_block": block, "launch_block_sweep_ms": sweep or None},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "StepBallArenasKernel", "kernel_avg_ms": avg_kernel_s * 1e3,
                      "bytes_per_arena_tick": bytes_per_arena_tick(0)},
+        "forms": forms,
+        "serving": serving,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

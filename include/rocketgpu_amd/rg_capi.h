@@ -105,6 +105,17 @@ int rg_batch_step(rg_batch* batch, int32_t ticks, void* stream);
 /* step_with_device_controls(device_controls_by_tick, ticks, control_stride, stream)  (rocketgpu.h:1075). */
 int rg_batch_step_with_device_controls(rg_batch* batch, const void* device_controls_by_tick, int32_t ticks,
                                        int32_t control_stride, void* stream);
+/* Steps `count` distinct arenas listed in device_arena_ids (device int32), arena i for device_ticks[i] ticks, or
+ * for `ticks` ticks each when device_ticks is null.  No reference counterpart: it is the launch of the serving loop
+ * below (the reference steps whole batches only, rocketgpu.h:1068). */
+int rg_batch_step_arenas(rg_batch* batch, const int32_t* device_arena_ids, const int32_t* device_ticks,
+                         int32_t count, int32_t ticks, void* stream);
+/* launches × step(ticks_per_launch) captured once into a hipGraph and replayed on `stream` (the reference captures
+ * its multi-kernel step into CUDA graphs keyed on pointers, src/rocketgpu_cuda.cu:3816-3819). */
+int rg_batch_step_graph(rg_batch* batch, int32_t ticks_per_launch, int32_t launches, void* stream);
+/* Lanes per workgroup of the step kernel: 64, 128 or 256 (launch-shape tuning; default 64). */
+int rg_batch_set_launch_block(rg_batch* batch, int32_t block);
+
 /* synchronize(stream)  (rocketgpu.h:1084). */
 int rg_batch_synchronize(rg_batch* batch, void* stream);
 
@@ -124,6 +135,42 @@ uint64_t rg_fold_digests(const uint64_t* host_env_digests, int64_t n);
  * launches of rocketgpu_amd/workload.h) for arenas first_arena.. on PA(scale, .).  No reference counterpart; the
  * reference's benchmark draws its own inputs (tools/bench_cpp.cpp:300-360). */
 int rg_workload_ball_states(uint64_t seed, int64_t first_arena, int64_t count, double scale, void* out_ball_states);
+
+/* Serving loop (no reference counterpart; the reference has no server).  `clients` simulated actors own disjoint
+ * ranges of min_arenas..max_arenas resident arenas and send "step my arenas k ticks" requests (k drawn from
+ * horizons) with Poisson arrivals at request_rate_hz in aggregate for duration_s.  Requests are admitted into the
+ * next launch while its arena budget (max_arenas_per_launch) allows; up to max_in_flight launches overlap on
+ * separate streams.  Latency is completion minus arrival. */
+typedef struct rg_serve_config {
+    int32_t clients;
+    int32_t min_arenas;
+    int32_t max_arenas;
+    int32_t horizon_count;
+    int32_t horizons[4];
+    int32_t max_arenas_per_launch;
+    int32_t max_in_flight;
+    double request_rate_hz;
+    double duration_s;
+    uint64_t seed;
+} rg_serve_config;
+
+typedef struct rg_serve_report {
+    int64_t requests;
+    int64_t arena_ticks;
+    int64_t launches;
+    int64_t arenas_resident;
+    double seconds;
+    double arena_ticks_per_s;
+    double p50_ms;
+    double p99_ms;
+    double mean_ms;
+    double max_ms;
+} rg_serve_report;
+
+/* Runs the loop to completion; host_ticks_per_arena (nullable, env_count entries) receives the ticks each arena
+ * was stepped. */
+int rg_serve_run(rg_batch* batch, const rg_serve_config* config, rg_serve_report* report,
+                 int32_t* host_ticks_per_arena);
 
 /* state_layouts()  (python/rocketgpu_bindings.cpp:541-806) as JSON into buf; returns the length needed. */
 int64_t rg_state_layouts_json(char* buf, size_t capacity);

@@ -48,6 +48,24 @@ class _DeviceView(ctypes.Structure):
                  "env_car_count")]
 
 
+class ServeConfig(ctypes.Structure):
+    """rg_serve_config: simulated clients, Poisson arrivals, horizons, launch budget (include/rocketgpu_amd/rg_capi.h)."""
+    _fields_ = [("clients", ctypes.c_int32), ("min_arenas", ctypes.c_int32), ("max_arenas", ctypes.c_int32),
+                ("horizon_count", ctypes.c_int32), ("horizons", ctypes.c_int32 * 4),
+                ("max_arenas_per_launch", ctypes.c_int32), ("max_in_flight", ctypes.c_int32),
+                ("request_rate_hz", ctypes.c_double), ("duration_s", ctypes.c_double), ("seed", ctypes.c_uint64)]
+
+
+class ServeReport(ctypes.Structure):
+    _fields_ = [("requests", ctypes.c_int64), ("arena_ticks", ctypes.c_int64), ("launches", ctypes.c_int64),
+                ("arenas_resident", ctypes.c_int64), ("seconds", ctypes.c_double),
+                ("arena_ticks_per_s", ctypes.c_double), ("p50_ms", ctypes.c_double), ("p99_ms", ctypes.c_double),
+                ("mean_ms", ctypes.c_double), ("max_ms", ctypes.c_double)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 def library_path() -> str:
     return LIB_PATH
 
@@ -79,6 +97,10 @@ def lib():
         L.rg_batch_device_view.argtypes = [vp, ctypes.POINTER(_DeviceView)]
         L.rg_batch_env_digests.argtypes = [vp, vp, vp]
         L.rg_workload_ball_states.argtypes = [ctypes.c_uint64, ctypes.c_int64, ctypes.c_int64, ctypes.c_double, vp]
+        L.rg_batch_step_arenas.argtypes = [vp, vp, vp, ctypes.c_int32, ctypes.c_int32, vp]
+        L.rg_batch_step_graph.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, vp]
+        L.rg_batch_set_launch_block.argtypes = [vp, ctypes.c_int32]
+        L.rg_serve_run.argtypes = [vp, ctypes.POINTER(ServeConfig), ctypes.POINTER(ServeReport), vp]
         L.rg_fold_digests.argtypes = [vp, ctypes.c_int64]
         L.rg_fold_digests.restype = ctypes.c_uint64
         L.rg_state_layouts_json.argtypes = [ctypes.c_char_p, ctypes.c_size_t]
@@ -167,6 +189,32 @@ class BatchSim:
     def step_with_device_controls(self, device_controls_ptr: int, ticks: int, control_stride: int, stream=None):
         _check(lib().rg_batch_step_with_device_controls(self._h, ctypes.c_void_p(device_controls_ptr), ticks,
                                                         control_stride, _stream_ptr(stream)))
+
+    def step_arenas(self, device_ids_ptr: int, count: int, ticks: int = 0, device_ticks_ptr: int = 0, stream=None):
+        """Step `count` distinct arenas (device int32 ids), each for device_ticks[i] ticks or `ticks` ticks."""
+        _check(lib().rg_batch_step_arenas(self._h, ctypes.c_void_p(device_ids_ptr),
+                                          ctypes.c_void_p(device_ticks_ptr or None), count, ticks,
+                                          _stream_ptr(stream)))
+
+    def step_graph(self, ticks_per_launch: int, launches: int = 1, stream=None):
+        """`launches` x step(ticks_per_launch), captured once into a hipGraph and replayed."""
+        _check(lib().rg_batch_step_graph(self._h, ticks_per_launch, launches, _stream_ptr(stream)))
+
+    def set_launch_block(self, block: int):
+        _check(lib().rg_batch_set_launch_block(self._h, block))
+
+    def serve(self, clients=256, min_arenas=16, max_arenas=256, horizons=(1, 4, 8), max_arenas_per_launch=65536,
+              max_in_flight=2, request_rate_hz=2000.0, duration_s=2.0, seed=0, ticks_per_arena=False):
+        """Run the serving loop (continuous batching over this batch's arenas); returns the report as a dict and,
+        with ticks_per_arena, the ticks each arena was stepped."""
+        h = (ctypes.c_int32 * 4)(*(list(horizons) + [0] * (4 - len(horizons))))
+        cfg = ServeConfig(clients, min_arenas, max_arenas, len(horizons), h, max_arenas_per_launch, max_in_flight,
+                          request_rate_hz, duration_s, seed)
+        rep = ServeReport()
+        per_arena = np.zeros(self.env_count, np.int32) if ticks_per_arena else None
+        _check(lib().rg_serve_run(self._h, ctypes.byref(cfg), ctypes.byref(rep),
+                                  per_arena.ctypes.data if per_arena is not None else None))
+        return (rep.as_dict(), per_arena) if ticks_per_arena else rep.as_dict()
 
     def synchronize(self, stream=None):
         _check(lib().rg_batch_synchronize(self._h, _stream_ptr(stream)))

@@ -42,7 +42,8 @@ def build_product():
           "-fhip-fp32-correctly-rounded-divide-sqrt", "-fno-gpu-flush-denormals-to-zero",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-shared", "-o", tmp,
-          os.path.join(CSRC, "rg_batch.hip"), os.path.join(CSRC, "mesh_build.cpp")])
+          os.path.join(CSRC, "rg_batch.hip"), os.path.join(CSRC, "mesh_build.cpp"),
+          os.path.join(CSRC, "rg_serve.cpp")])
     os.replace(tmp, PRODUCT)
 
 

@@ -70,17 +70,27 @@ struct StepArgs {
     core::ArenaMeshView mesh;
     core::StepParams params;
     int32_t env_count;
+    // Work list: lane i steps arena arena_ids[i] (or arena i when null) for arena_ticks[i] ticks (or `ticks` when
+    // null).  The host guarantees distinct ids within one launch.
+    const int32_t* arena_ids;
+    const int32_t* arena_ticks;
+    int32_t count;
 };
 
-// One lane per arena; `ticks` ticks back to back with the arena's state in HBM (L2-resident between ticks).
-__global__ void __launch_bounds__(64) StepBallArenasKernel(StepArgs a, int32_t ticks) {
-    const int e = static_cast<int>(blockIdx.x) * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x);
-    if (e >= a.env_count) return;
+constexpr int kMaxStepBlock = 256;
+
+// One lane per arena; its ticks back to back with the arena's state in HBM (L2-resident between ticks).
+__global__ void __launch_bounds__(kMaxStepBlock) StepBallArenasKernel(StepArgs a, int32_t ticks) {
+    const int i = static_cast<int>(blockIdx.x) * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x);
+    if (i >= a.count) return;
+    const int e = a.arena_ids ? a.arena_ids[i] : i;
+    if (e < 0 || e >= a.env_count) return;
+    const int n = a.arena_ticks ? a.arena_ticks[i] : ticks;
     core::BallTickScratch scratch;
     BallState& ball = a.balls[e];
     EnvState& env = a.envs[e];
     core::ArenaInternal& st = a.internal[e];
-    for (int t = 0; t < ticks; ++t) core::StepBallArena(ball, env, st, a.mesh, a.params, scratch);
+    for (int t = 0; t < n; ++t) core::StepBallArena(ball, env, st, a.mesh, a.params, scratch);
 }
 
 __device__ inline uint64_t Fnv(uint64_t h, const void* data, int n) {
@@ -227,23 +237,58 @@ public:
     void Step(int32_t ticks, hipStream_t s) {
         if (ticks < 0) throw InvalidArgument("ticks must be non-negative");
         if (ticks == 0) return;
-        StepArgs a{};
-        a.balls = balls_.data();
-        a.envs = envs_.data();
-        a.internal = internal_.data();
-        a.env_car_count = car_count_.data();
-        a.mesh.tris = tris_.data();
-        a.mesh.nodes = nodes_.data();
-        a.mesh.meshes = infos_.data();
-        a.mesh.mesh_count = static_cast<int32_t>(mesh_host_.meshes.size());
-        a.mesh.tri_count = static_cast<int32_t>(mesh_host_.tris.size());
-        a.params = params_;
-        a.env_count = cfg_.env_count;
-        const int block = 64;
-        const int grid = (cfg_.env_count + block - 1) / block;
-        hipLaunchKernelGGL(StepBallArenasKernel, dim3(grid), dim3(block), 0, s, a, ticks);
-        Check(hipGetLastError(), "StepBallArenasKernel launch");
+        Launch(nullptr, nullptr, cfg_.env_count, ticks, s);
     }
+
+    // Steps a list of distinct arenas (device ids), each for its own tick count (device array, or `ticks` for all
+    // when null).  The serving loop's launch.
+    void StepArenas(const int32_t* device_ids, const int32_t* device_ticks, int32_t count, int32_t ticks,
+                    hipStream_t s) {
+        if (count < 0 || count > cfg_.env_count) throw InvalidArgument("arena count outside 0..env_count");
+        if (count > 0 && device_ids == nullptr) throw InvalidArgument("arena id list is null");
+        if (device_ticks == nullptr && ticks < 0) throw InvalidArgument("ticks must be non-negative");
+        if (count == 0 || (device_ticks == nullptr && ticks == 0)) return;
+        Launch(device_ids, device_ticks, count, ticks, s);
+    }
+
+    // `launches` step launches of `ticks_per_launch` ticks each, captured once into a hipGraph (on a private stream)
+    // and replayed on `s`; the graph is re-captured when the shape or the launch block changes.  The captured kernels
+    // hold the batch's own device pointers, which live as long as the batch.
+    void StepGraph(int32_t ticks_per_launch, int32_t launches, hipStream_t s) {
+        if (ticks_per_launch <= 0 || launches <= 0 || launches > 1024) {
+            throw InvalidArgument("graph step needs positive ticks and 1..1024 launches");
+        }
+        if (!graph_exec_ || graph_ticks_ != ticks_per_launch || graph_launches_ != launches ||
+            graph_block_ != block_) {
+            DestroyGraph();
+            if (!capture_stream_) {
+                Check(hipStreamCreateWithFlags(&capture_stream_, hipStreamNonBlocking), "hipStreamCreate");
+            }
+            Check(hipStreamBeginCapture(capture_stream_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+            for (int32_t l = 0; l < launches; ++l) Launch(nullptr, nullptr, cfg_.env_count, ticks_per_launch, capture_stream_);
+            hipGraph_t graph = nullptr;
+            Check(hipStreamEndCapture(capture_stream_, &graph), "hipStreamEndCapture");
+            const hipError_t e = hipGraphInstantiate(&graph_exec_, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            Check(e, "hipGraphInstantiate");
+            graph_ticks_ = ticks_per_launch;
+            graph_launches_ = launches;
+            graph_block_ = block_;
+        }
+        Check(hipGraphLaunch(graph_exec_, s), "hipGraphLaunch");
+    }
+
+    ~BatchSim() {
+        DestroyGraph();
+        if (capture_stream_) (void)hipStreamDestroy(capture_stream_);
+    }
+
+    // Lanes per workgroup of the step kernel (64, 128 or 256); chosen by the launch-shape tuner.
+    void SetBlock(int block) {
+        if (block != 64 && block != 128 && block != 256) throw InvalidArgument("block must be 64, 128 or 256");
+        block_ = block;
+    }
+    int block() const { return block_; }
 
     void StepWithDeviceControls(const void* controls, int32_t ticks, int32_t stride, hipStream_t s) {
         if (stride != 0 && stride < car_capacity()) {
@@ -271,6 +316,32 @@ public:
     }
 
 private:
+    void DestroyGraph() {
+        if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+        graph_exec_ = nullptr;
+    }
+
+    void Launch(const int32_t* ids, const int32_t* ticks_list, int32_t count, int32_t ticks, hipStream_t s) {
+        StepArgs a{};
+        a.balls = balls_.data();
+        a.envs = envs_.data();
+        a.internal = internal_.data();
+        a.env_car_count = car_count_.data();
+        a.mesh.tris = tris_.data();
+        a.mesh.nodes = nodes_.data();
+        a.mesh.meshes = infos_.data();
+        a.mesh.mesh_count = static_cast<int32_t>(mesh_host_.meshes.size());
+        a.mesh.tri_count = static_cast<int32_t>(mesh_host_.tris.size());
+        a.params = params_;
+        a.env_count = cfg_.env_count;
+        a.arena_ids = ids;
+        a.arena_ticks = ticks_list;
+        a.count = count;
+        const int grid = (count + block_ - 1) / block_;
+        hipLaunchKernelGGL(StepBallArenasKernel, dim3(grid), dim3(block_), 0, s, a, ticks);
+        Check(hipGetLastError(), "StepBallArenasKernel launch");
+    }
+
     static void Expect(size_t n, size_t want, const char* what) {
         if (n != want) throw InvalidArgument(std::string(what) + ": count must equal the array size");
     }
@@ -288,6 +359,10 @@ private:
     DeviceArray<CarState> cars_;
     DeviceArray<CarControls> controls_;
     DeviceArray<uint64_t> digests_;
+    int block_ = 64;
+    hipStream_t capture_stream_ = nullptr;
+    hipGraphExec_t graph_exec_ = nullptr;
+    int32_t graph_ticks_ = 0, graph_launches_ = 0, graph_block_ = 0;
 };
 
 }  // namespace rg
@@ -339,6 +414,7 @@ rgw::ArenaRulesSpec RulesFromLayout(const rg_arena_layout& l) {
 extern "C" {
 
 const char* rg_last_error(void) { return rg::g_last_error.c_str(); }
+void rg_set_last_error(const char* message) { rg::g_last_error = message ? message : ""; }
 const char* rg_version(void) { return "rocketgpu_amd 0.1 (gfx950; ball-only arenas)"; }
 
 int rg_batch_create(const rg_sim_config* config, const rg_arena_layout* layout, rg_batch** out) {
@@ -405,6 +481,19 @@ int rg_batch_step(rg_batch* b, int32_t ticks, void* stream) {
 int rg_batch_step_with_device_controls(rg_batch* b, const void* controls, int32_t ticks, int32_t stride,
                                        void* stream) {
     return Guard([&] { b->sim->StepWithDeviceControls(controls, ticks, stride, S(stream)); });
+}
+
+int rg_batch_step_arenas(rg_batch* b, const int32_t* device_arena_ids, const int32_t* device_ticks, int32_t count,
+                         int32_t ticks, void* stream) {
+    return Guard([&] { b->sim->StepArenas(device_arena_ids, device_ticks, count, ticks, S(stream)); });
+}
+
+int rg_batch_step_graph(rg_batch* b, int32_t ticks_per_launch, int32_t launches, void* stream) {
+    return Guard([&] { b->sim->StepGraph(ticks_per_launch, launches, S(stream)); });
+}
+
+int rg_batch_set_launch_block(rg_batch* b, int32_t block) {
+    return Guard([&] { b->sim->SetBlock(block); });
 }
 
 int rg_batch_synchronize(rg_batch* b, void* stream) {
