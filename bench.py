@@ -178,19 +178,26 @@ def main():
     ticks_done = envs[:, 0:4].copy().view(np.uint32)[:, 0]
     assert np.all(ticks_done == (args.warmup + args.steps) * T), "tick counters do not match the steps run"
 
+    sim.close()
+
     forms = None
     if not args.no_forms:
-        # The same batch, continuing its trajectory: T single-tick launches, one T-tick launch, one graph replay.
-        per_tick = timed(torch, stream, lambda: [sim.step(1, stream=stream) for _ in range(T)], 10)
-        multi = timed(torch, stream, lambda: sim.step(T, stream=stream), 10)
-        sim.step_graph(1, T, stream=stream)
-        graph_1xT = timed(torch, stream, lambda: sim.step_graph(1, T, stream=stream), 10)
-        sim.step_graph(T, 1, stream=stream)
-        graph_T = timed(torch, stream, lambda: sim.step_graph(T, 1, stream=stream), 10)
-        forms = {"ms_per_T_ticks": {"plain_T_launches_of_1_tick": per_tick, "one_launch_of_T_ticks": multi,
-                                    "hipgraph_T_launches_of_1_tick": graph_1xT, "hipgraph_one_launch_of_T": graph_T},
-                 "T": T}
-    sim.close()
+        # Each form on a fresh batch from the same initial states over the same tick window as the headline
+        # (warm-up, then the timed steps): T single-tick launches, one T-tick launch, and hipGraph replays of both.
+        def form(run):
+            fsim, _, _ = make_batch(rga, np, n, args.max_cars, args.seed)
+            fsim.set_launch_block(block)
+            for _ in range(args.warmup):
+                run(fsim)
+            ms = timed(torch, stream, lambda: run(fsim), args.steps)
+            fsim.close()
+            return ms
+
+        forms = {"T": T, "window_ticks": [args.warmup * T, (args.warmup + args.steps) * T], "ms_per_T_ticks": {
+            "plain_T_launches_of_1_tick": form(lambda s: [s.step(1, stream=stream) for _ in range(T)]),
+            "one_launch_of_T_ticks": form(lambda s: s.step(T, stream=stream)),
+            "hipgraph_T_launches_of_1_tick": form(lambda s: s.step_graph(1, T, stream=stream)),
+            "hipgraph_one_launch_of_T_ticks": form(lambda s: s.step_graph(T, 1, stream=stream))}}
 
     serving = None
     if not args.no_serve:
