@@ -11,11 +11,15 @@ peak: 2 x (BallState 288 + EnvState 44 + 34 pads x 16 + world solver state 11,71
 cpu_baseline times the reference CPU build (oracle/_ref) on this machine's host cores over the same tick window of
 the same workload.
 
-Beside the headline: the launch block chosen by a short sweep (64/128/256 lanes), the other step forms on the same
-batch (T launches of one tick, one hipGraph replay), and the serving loop (simulated clients, Poisson arrivals,
-continuous batching over 262,144 resident arenas) with p50/p99 request latency.
+Beside the headline:
+- the launch shape, which the tuner picks over lanes per workgroup, state in place or staged in lane-private
+  memory, and arenas per wavefront (64 down to 4: more waves per CU, less divergence);
+- the other step forms, each on a fresh batch over the same window: T launches of one tick, and hipGraph replays;
+- the serving loop (simulated clients, Poisson arrivals, continuous batching over 262,144 resident arenas), with
+  p50/p99 request latency.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--arenas N] [--ticks T] [--no-serve] [--cpu-ticks C]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--arenas N] [--ticks T] [--block B --stage S --per-wave A]
+                    [--no-forms] [--no-serve] [--cpu-ticks C] [--calibrate-copy-mib M]
 """
 from __future__ import annotations
 
@@ -45,7 +49,9 @@ def parse():
     p.add_argument("--ticks", type=int, default=8, help="ticks per step (T of step_with_device_controls)")
     p.add_argument("--max-cars", type=int, default=6)
     p.add_argument("--seed", type=int, default=2024)
-    p.add_argument("--block", type=int, default=0, help="launch block (0: choose by a short sweep)")
+    p.add_argument("--block", type=int, default=0, help="lanes per workgroup (0: the tuner chooses the shape)")
+    p.add_argument("--stage", type=int, default=0, help="state staging with --block: 0 in place, 1 private copy")
+    p.add_argument("--per-wave", type=int, default=64, help="arenas per 64-lane wavefront with --block")
     p.add_argument("--no-forms", action="store_true")
     p.add_argument("--no-serve", action="store_true")
     p.add_argument("--serve-arenas", type=int, default=262144)
@@ -129,20 +135,25 @@ def main():
     g = torch.Generator(device="cuda")
     g.manual_seed(args.seed)
 
-    # Launch block: a short sweep on a scratch copy of the workload (same trajectory window as the warm-up).
-    block = args.block
+    # Launch shape (lanes per workgroup, state staging): the tuner times every shape on a fresh copy of the workload
+    # over the warm-up window plus ten steps and keeps the fastest.
+    shape = (args.block, args.stage, args.per_wave) if args.block else None
     sweep = {}
-    if block == 0:
-        for b in (64, 128, 256):
+    if shape is None:
+        key = lambda c: f"block{c[0]}_stage{c[1]}_perwave{c[2]}"
+        cands = [(64, s, w) for s in (0, 1) for w in (64, 32, 16, 8, 4)] + [(128, 0, 64), (256, 0, 64)]
+        for cand in cands:
             sim_b, _, _ = make_batch(rga, np, n, args.max_cars, args.seed)
-            sim_b.set_launch_block(b)
-            sim_b.step(T, stream=stream)
-            sweep[b] = timed(torch, stream, lambda: sim_b.step(T, stream=stream), 4)
+            sim_b.set_launch_shape(*cand)
+            for _ in range(args.warmup):
+                sim_b.step(T, stream=stream)
+            sweep[key(cand)] = timed(torch, stream, lambda: sim_b.step(T, stream=stream), 10)
             sim_b.close()
-        block = min(sweep, key=sweep.get)
+            if shape is None or sweep[key(cand)] < sweep[key(shape)]:
+                shape = cand
 
     sim, init_ball, init_env = make_batch(rga, np, n, args.max_cars, args.seed)
-    sim.set_launch_block(block)
+    sim.set_launch_shape(*shape)
     ctrl = torch.randint(0, 256, (T, sim.car_capacity * 24), dtype=torch.uint8, device="cuda", generator=g)
     torch.cuda.synchronize()
     for _ in range(args.warmup):
@@ -186,7 +197,7 @@ def main():
         # (warm-up, then the timed steps): T single-tick launches, one T-tick launch, and hipGraph replays of both.
         def form(run):
             fsim, _, _ = make_batch(rga, np, n, args.max_cars, args.seed)
-            fsim.set_launch_block(block)
+            fsim.set_launch_shape(*shape)
             for _ in range(args.warmup):
                 run(fsim)
             ms = timed(torch, stream, lambda: run(fsim), args.steps)
@@ -202,7 +213,7 @@ def main():
     serving = None
     if not args.no_serve:
         ssim, _, _ = make_batch(rga, np, args.serve_arenas, 2, args.seed + 1)
-        ssim.set_launch_block(block)
+        ssim.set_launch_shape(*shape)
         serving = ssim.serve(clients=256, min_arenas=16, max_arenas=256, horizons=(1, 4, 8),
                              max_arenas_per_launch=65536, max_in_flight=2, request_rate_hz=2000.0, duration_s=2.0,
                              seed=args.seed)
@@ -220,7 +231,7 @@ def main():
         "config": {"workload": f"{n} ball arenas (no cars), procedural arena PA(1, 256) 20,332 triangles, "
                                f"step_with_device_controls T={T}",
                    "arenas": n, "ticks_per_step": T, "cars_per_arena": 0, "parallelism": "single GPU",
-                   "launch_block": block, "launch_block_sweep_ms": sweep or None},
+                   "launch_shape": {"block": shape[0], "stage": shape[1], "arenas_per_wave": shape[2]}, "launch_shape_sweep_ms": sweep or None},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "StepBallArenasKernel", "kernel_avg_ms": avg_kernel_s * 1e3,

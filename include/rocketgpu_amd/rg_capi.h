@@ -113,8 +113,10 @@ int rg_batch_step_arenas(rg_batch* batch, const int32_t* device_arena_ids, const
 /* launches × step(ticks_per_launch) captured once into a hipGraph and replayed on `stream` (the reference captures
  * its multi-kernel step into CUDA graphs keyed on pointers, src/rocketgpu_cuda.cu:3816-3819). */
 int rg_batch_step_graph(rg_batch* batch, int32_t ticks_per_launch, int32_t launches, void* stream);
-/* Lanes per workgroup of the step kernel: 64, 128 or 256 (launch-shape tuning; default 64). */
-int rg_batch_set_launch_block(rg_batch* batch, int32_t block);
+/* Launch shape of the step kernel: lanes per workgroup (64, 128 or 256), state staging (0: the arena records are
+ * stepped in place in HBM; 1: copied into lane-private memory for the launch) and arenas per 64-lane wavefront
+ * (64, 32, 16, 8 or 4).  Results are identical for every shape; the benchmark tunes it.  Defaults 64, 0, 64. */
+int rg_batch_set_launch_shape(rg_batch* batch, int32_t block, int32_t stage, int32_t arenas_per_wave);
 
 /* synchronize(stream)  (rocketgpu.h:1084). */
 int rg_batch_synchronize(rg_batch* batch, void* stream);

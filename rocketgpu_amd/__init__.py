@@ -99,7 +99,7 @@ def lib():
         L.rg_workload_ball_states.argtypes = [ctypes.c_uint64, ctypes.c_int64, ctypes.c_int64, ctypes.c_double, vp]
         L.rg_batch_step_arenas.argtypes = [vp, vp, vp, ctypes.c_int32, ctypes.c_int32, vp]
         L.rg_batch_step_graph.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, vp]
-        L.rg_batch_set_launch_block.argtypes = [vp, ctypes.c_int32]
+        L.rg_batch_set_launch_shape.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]
         L.rg_serve_run.argtypes = [vp, ctypes.POINTER(ServeConfig), ctypes.POINTER(ServeReport), vp]
         L.rg_fold_digests.argtypes = [vp, ctypes.c_int64]
         L.rg_fold_digests.restype = ctypes.c_uint64
@@ -200,8 +200,9 @@ class BatchSim:
         """`launches` x step(ticks_per_launch), captured once into a hipGraph and replayed."""
         _check(lib().rg_batch_step_graph(self._h, ticks_per_launch, launches, _stream_ptr(stream)))
 
-    def set_launch_block(self, block: int):
-        _check(lib().rg_batch_set_launch_block(self._h, block))
+    def set_launch_shape(self, block: int = 64, stage: int = 0, arenas_per_wave: int = 64):
+        """Lanes per workgroup, state staging (0 in place, 1 lane-private copy), arenas per 64-lane wavefront."""
+        _check(lib().rg_batch_set_launch_shape(self._h, block, stage, arenas_per_wave))
 
     def serve(self, clients=256, min_arenas=16, max_arenas=256, horizons=(1, 4, 8), max_arenas_per_launch=65536,
               max_in_flight=2, request_rate_hz=2000.0, duration_s=2.0, seed=0, ticks_per_arena=False):

@@ -75,22 +75,42 @@ struct StepArgs {
     const int32_t* arena_ids;
     const int32_t* arena_ticks;
     int32_t count;
+    // Arenas per 64-lane wavefront (64, 32, 16, 8 or 4): fewer arenas per wave means more waves per CU to hide the
+    // latency of the dependent BVH and state loads, and less divergence within a wave.
+    int32_t arenas_per_wave;
 };
 
 constexpr int kMaxStepBlock = 256;
 
-// One lane per arena; its ticks back to back with the arena's state in HBM (L2-resident between ticks).
+// One lane per arena; its ticks back to back.  kStage = false steps the arena's records in place in HBM (every
+// access of a wave touches 64 records 4 KB apart); kStage = true copies them into the lane's private memory first,
+// which the hardware interleaves across the wave's lanes, so the per-tick accesses of a wave coalesce, and writes
+// them back after the last tick.
+template <bool kStage>
 __global__ void __launch_bounds__(kMaxStepBlock) StepBallArenasKernel(StepArgs a, int32_t ticks) {
-    const int i = static_cast<int>(blockIdx.x) * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x);
+    const int g = static_cast<int>(blockIdx.x) * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x);
+    const int lane = g & 63;
+    if (lane >= a.arenas_per_wave) return;
+    const int i = (g >> 6) * a.arenas_per_wave + lane;
     if (i >= a.count) return;
     const int e = a.arena_ids ? a.arena_ids[i] : i;
     if (e < 0 || e >= a.env_count) return;
     const int n = a.arena_ticks ? a.arena_ticks[i] : ticks;
     core::BallTickScratch scratch;
-    BallState& ball = a.balls[e];
-    EnvState& env = a.envs[e];
-    core::ArenaInternal& st = a.internal[e];
-    for (int t = 0; t < n; ++t) core::StepBallArena(ball, env, st, a.mesh, a.params, scratch);
+    if constexpr (kStage) {
+        BallState ball = a.balls[e];
+        EnvState env = a.envs[e];
+        core::ArenaInternal st = a.internal[e];
+        for (int t = 0; t < n; ++t) core::StepBallArena(ball, env, st, a.mesh, a.params, scratch);
+        a.internal[e] = st;
+        a.envs[e] = env;
+        a.balls[e] = ball;
+    } else {
+        BallState& ball = a.balls[e];
+        EnvState& env = a.envs[e];
+        core::ArenaInternal& st = a.internal[e];
+        for (int t = 0; t < n; ++t) core::StepBallArena(ball, env, st, a.mesh, a.params, scratch);
+    }
 }
 
 __device__ inline uint64_t Fnv(uint64_t h, const void* data, int n) {
@@ -259,7 +279,7 @@ public:
             throw InvalidArgument("graph step needs positive ticks and 1..1024 launches");
         }
         if (!graph_exec_ || graph_ticks_ != ticks_per_launch || graph_launches_ != launches ||
-            graph_block_ != block_) {
+            graph_shape_ != LaunchShape()) {
             DestroyGraph();
             if (!capture_stream_) {
                 Check(hipStreamCreateWithFlags(&capture_stream_, hipStreamNonBlocking), "hipStreamCreate");
@@ -273,7 +293,7 @@ public:
             Check(e, "hipGraphInstantiate");
             graph_ticks_ = ticks_per_launch;
             graph_launches_ = launches;
-            graph_block_ = block_;
+            graph_shape_ = LaunchShape();
         }
         Check(hipGraphLaunch(graph_exec_, s), "hipGraphLaunch");
     }
@@ -283,11 +303,21 @@ public:
         if (capture_stream_) (void)hipStreamDestroy(capture_stream_);
     }
 
-    // Lanes per workgroup of the step kernel (64, 128 or 256); chosen by the launch-shape tuner.
+    // Lanes per workgroup of the step kernel (64, 128 or 256) and state staging (0 in place, 1 private copy);
+    // chosen by the launch-shape tuner.  Results are identical for every shape.
     void SetBlock(int block) {
         if (block != 64 && block != 128 && block != 256) throw InvalidArgument("block must be 64, 128 or 256");
         block_ = block;
     }
+    void SetStage(int stage) {
+        if (stage != 0 && stage != 1) throw InvalidArgument("stage must be 0 or 1");
+        stage_ = stage != 0;
+    }
+    void SetArenasPerWave(int n) {
+        if (n != 64 && n != 32 && n != 16 && n != 8 && n != 4) throw InvalidArgument("arenas per wave must be 64, 32, 16, 8 or 4");
+        per_wave_ = n;
+    }
+    int LaunchShape() const { return (block_ * 2 + (stage_ ? 1 : 0)) * 128 + per_wave_; }
     int block() const { return block_; }
 
     void StepWithDeviceControls(const void* controls, int32_t ticks, int32_t stride, hipStream_t s) {
@@ -337,8 +367,14 @@ private:
         a.arena_ids = ids;
         a.arena_ticks = ticks_list;
         a.count = count;
-        const int grid = (count + block_ - 1) / block_;
-        hipLaunchKernelGGL(StepBallArenasKernel, dim3(grid), dim3(block_), 0, s, a, ticks);
+        a.arenas_per_wave = per_wave_;
+        const int64_t threads = (static_cast<int64_t>(count) + per_wave_ - 1) / per_wave_ * 64;
+        const int grid = static_cast<int>((threads + block_ - 1) / block_);
+        if (stage_) {
+            hipLaunchKernelGGL(StepBallArenasKernel<true>, dim3(grid), dim3(block_), 0, s, a, ticks);
+        } else {
+            hipLaunchKernelGGL(StepBallArenasKernel<false>, dim3(grid), dim3(block_), 0, s, a, ticks);
+        }
         Check(hipGetLastError(), "StepBallArenasKernel launch");
     }
 
@@ -360,9 +396,11 @@ private:
     DeviceArray<CarControls> controls_;
     DeviceArray<uint64_t> digests_;
     int block_ = 64;
+    bool stage_ = false;
+    int per_wave_ = 64;
     hipStream_t capture_stream_ = nullptr;
     hipGraphExec_t graph_exec_ = nullptr;
-    int32_t graph_ticks_ = 0, graph_launches_ = 0, graph_block_ = 0;
+    int32_t graph_ticks_ = 0, graph_launches_ = 0, graph_shape_ = 0;
 };
 
 }  // namespace rg
@@ -492,8 +530,12 @@ int rg_batch_step_graph(rg_batch* b, int32_t ticks_per_launch, int32_t launches,
     return Guard([&] { b->sim->StepGraph(ticks_per_launch, launches, S(stream)); });
 }
 
-int rg_batch_set_launch_block(rg_batch* b, int32_t block) {
-    return Guard([&] { b->sim->SetBlock(block); });
+int rg_batch_set_launch_shape(rg_batch* b, int32_t block, int32_t stage, int32_t arenas_per_wave) {
+    return Guard([&] {
+        b->sim->SetBlock(block);
+        b->sim->SetStage(stage);
+        b->sim->SetArenasPerWave(arenas_per_wave);
+    });
 }
 
 int rg_batch_synchronize(rg_batch* b, void* stream) {

@@ -81,18 +81,21 @@ def test_graph_replays_match_reference(ticks_per_launch, launches):
     sim.close()
 
 
-def test_launch_block_shapes_agree():
+@pytest.mark.parametrize("block,stage,per_wave", [(64, 0, 64), (128, 0, 64), (256, 0, 32), (64, 1, 64),
+                                                  (128, 1, 16), (256, 1, 8), (64, 0, 4), (64, 1, 8)])
+def test_launch_shapes_match_reference(block, stage, per_wave):
+    """Every launch shape (lanes per workgroup, state staging) gives the reference's result, every byte."""
     _torch()
-    f = load_fixture("ball_arena_96_t1000")
-    outs = []
-    for block in (64, 128, 256):
-        sim = make_sim(f)
-        sim.set_launch_block(block)
-        sim.step(200)
-        outs.append(sim.download_state_array("ball_states"))
-        sim.close()
-    assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
-    assert fold(ball_env_digests(outs[0])) == f["batch_digest"][199]
+    f = load_fixture("ball_arena_1024_t3000")
+    sim = make_sim(f)
+    sim.set_launch_shape(block, stage, per_wave)
+    for _ in range(6):
+        sim.step(500)
+    balls = sim.download_state_array("ball_states")
+    envs = sim.download_state_array("env_states")
+    sim.close()
+    assert np.array_equal(balls, f["final_ball"])
+    assert np.array_equal(envs, f["final_env"])
 
 
 def test_serving_results_same_alone_and_in_the_mix():
