@@ -12,8 +12,9 @@ cpu_baseline times the reference CPU build (oracle/_ref) on this machine's host 
 the same workload.
 
 Beside the headline:
-- the launch shape, which the tuner picks over lanes per workgroup, state in place or staged in lane-private
-  memory, and arenas per wavefront (64 down to 4: more waves per CU, less divergence);
+- the launch shape, which the tuner picks over lanes per workgroup, where the arena state and the per-tick contact
+  scratch live (HBM in place, lane-private memory, LDS), and arenas per wavefront (64 down to 4: more waves per CU,
+  less divergence);
 - the other step forms, each on a fresh batch over the same window: T launches of one tick, and hipGraph replays;
 - the serving loop (simulated clients, Poisson arrivals, continuous batching over 262,144 resident arenas), with
   p50/p99 request latency.
@@ -50,7 +51,8 @@ def parse():
     p.add_argument("--max-cars", type=int, default=6)
     p.add_argument("--seed", type=int, default=2024)
     p.add_argument("--block", type=int, default=0, help="lanes per workgroup (0: the tuner chooses the shape)")
-    p.add_argument("--stage", type=int, default=0, help="state staging with --block: 0 in place, 1 private copy")
+    p.add_argument("--stage", type=int, default=0,
+                   help="state staging with --block: 0 in place, 1 private copy, 2 LDS scratch, 3 LDS state+scratch")
     p.add_argument("--per-wave", type=int, default=64, help="arenas per 64-lane wavefront with --block")
     p.add_argument("--no-forms", action="store_true")
     p.add_argument("--no-serve", action="store_true")
@@ -141,15 +143,21 @@ def main():
     sweep = {}
     if shape is None:
         key = lambda c: f"block{c[0]}_stage{c[1]}_perwave{c[2]}"
-        cands = [(64, s, w) for s in (0, 1) for w in (64, 32, 16, 8, 4)] + [(128, 0, 64), (256, 0, 64)]
+        cands = ([(64, s, w) for s in (0, 1) for w in (64, 32, 16, 8, 4)] + [(128, 0, 64), (256, 0, 64)] +
+                 [(64, 2, w) for w in (64, 32, 16, 8)] + [(64, 3, w) for w in (16, 8, 4)] + [(128, 3, 4), (256, 3, 4)])
         for cand in cands:
             sim_b, _, _ = make_batch(rga, np, n, args.max_cars, args.seed)
-            sim_b.set_launch_shape(*cand)
+            try:
+                sim_b.set_launch_shape(*cand)
+            except rga.RocketGpuError as err:  # the shape's LDS does not fit this device
+                sweep[key(cand)] = str(err)
+                sim_b.close()
+                continue
             for _ in range(args.warmup):
                 sim_b.step(T, stream=stream)
             sweep[key(cand)] = timed(torch, stream, lambda: sim_b.step(T, stream=stream), 10)
             sim_b.close()
-            if shape is None or sweep[key(cand)] < sweep[key(shape)]:
+            if shape is None or sweep[key(cand)] < sweep[key(shape)]:  # shape is always a timed candidate
                 shape = cand
 
     sim, init_ball, init_env = make_batch(rga, np, n, args.max_cars, args.seed)

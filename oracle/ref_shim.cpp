@@ -146,13 +146,19 @@ long rgref_offsetof(int which, const char* field) {
     return -1;
 }
 
-void* rgref_create(int env_count, int max_cars, double scale, double fillet_radius, uint64_t respawn_seed) {
+// flags: bit 0 boost pads, bit 1 car-car collisions, bit 2 demos, bit 3 scoring (rgref_create sets all four).
+void* rgref_create_flags(int env_count, int max_cars, double scale, double fillet_radius, uint64_t respawn_seed,
+                         int flags) {
     try {
         auto h = std::make_unique<RefSim>();
         SimConfig cfg;
         cfg.env_count = env_count;
         cfg.max_cars_per_env = max_cars;
         cfg.respawn_rng_seed = respawn_seed;
+        cfg.enable_boost_pads = (flags & 1) ? 1 : 0;
+        cfg.enable_car_car_collisions = (flags & 2) ? 1 : 0;
+        cfg.enable_demos = (flags & 4) ? 1 : 0;
+        cfg.enable_scoring = (flags & 8) ? 1 : 0;
         h->sim = std::make_unique<BatchSim>(cfg, ProceduralLayout(scale, fillet_radius));
         h->env_count = env_count;
         h->max_cars = max_cars;
@@ -162,6 +168,10 @@ void* rgref_create(int env_count, int max_cars, double scale, double fillet_radi
         Fail(e);
         return nullptr;
     }
+}
+
+void* rgref_create(int env_count, int max_cars, double scale, double fillet_radius, uint64_t respawn_seed) {
+    return rgref_create_flags(env_count, max_cars, scale, fillet_radius, respawn_seed, 15);
 }
 
 void rgref_destroy(void* p) { delete static_cast<RefSim*>(p); }

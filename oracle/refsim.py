@@ -37,6 +37,9 @@ def lib():
         L.rgref_offsetof.argtypes = [ctypes.c_int, ctypes.c_char_p]
         L.rgref_create.restype = ctypes.c_void_p
         L.rgref_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_uint64]
+        L.rgref_create_flags.restype = ctypes.c_void_p
+        L.rgref_create_flags.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_uint64,
+                                         ctypes.c_int]
         L.rgref_destroy.argtypes = [ctypes.c_void_p]
         vp, cp, lg = ctypes.c_void_p, ctypes.c_void_p, ctypes.c_long
         for name, args in [
@@ -101,9 +104,10 @@ class RefSim:
     """One reference BatchSim(cfg, procedural layout)."""
 
     def __init__(self, env_count: int, max_cars: int, scale: float = 1.0, fillet_radius: float = 256.0,
-                 respawn_seed: int = 0):
+                 respawn_seed: int = 0, flags: int = 15):
+        """flags: bit 0 boost pads, bit 1 car-car collisions, bit 2 demos, bit 3 scoring."""
         self._L = lib()
-        self.h = self._L.rgref_create(env_count, max_cars, scale, fillet_radius, respawn_seed)
+        self.h = self._L.rgref_create_flags(env_count, max_cars, scale, fillet_radius, respawn_seed, flags)
         if not self.h:
             raise RuntimeError(self._L.rgref_last_error().decode())
         self.env_count = env_count

@@ -82,12 +82,31 @@ struct StepArgs {
 
 constexpr int kMaxStepBlock = 256;
 
-// One lane per arena; its ticks back to back.  kStage = false steps the arena's records in place in HBM (every
-// access of a wave touches 64 records 4 KB apart); kStage = true copies them into the lane's private memory first,
-// which the hardware interleaves across the wave's lanes, so the per-tick accesses of a wave coalesce, and writes
-// them back after the last tick.
-template <bool kStage>
+// Where a launch keeps an arena's records while it steps them (the launch shape's `stage`):
+//   0  in place in HBM, per-tick contact scratch in lane-private memory;
+//   1  records and scratch in lane-private memory (the hardware interleaves private memory across a wave's lanes,
+//      so a wave's accesses to one field coalesce);
+//   2  records in place, per-tick contact scratch in LDS (the hottest data: contact rows, manifold candidates);
+//   3  records and scratch in LDS.
+// Records are copied in before the first tick and back after the last.  LDS holds one slice per active lane of
+// the workgroup: blockDim / 64 waves x arenas_per_wave lanes.
+enum StageMode : int { kStageInPlace = 0, kStagePrivate = 1, kStageLdsScratch = 2, kStageLds = 3 };
+
+struct LdsSlice {
+    core::BallTickScratch scratch;
+    core::ArenaInternal st;
+    BallState ball;
+    EnvState env;
+};
+
+__host__ __device__ constexpr size_t LdsBytesPerLane(int stage) {
+    return stage == kStageLds ? sizeof(LdsSlice) : (stage == kStageLdsScratch ? sizeof(core::BallTickScratch) : 0);
+}
+
+// One lane per arena; its ticks back to back.
+template <int kStage>
 __global__ void __launch_bounds__(kMaxStepBlock) StepBallArenasKernel(StepArgs a, int32_t ticks) {
+    extern __shared__ __align__(16) unsigned char lds[];
     const int g = static_cast<int>(blockIdx.x) * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x);
     const int lane = g & 63;
     if (lane >= a.arenas_per_wave) return;
@@ -96,20 +115,34 @@ __global__ void __launch_bounds__(kMaxStepBlock) StepBallArenasKernel(StepArgs a
     const int e = a.arena_ids ? a.arena_ids[i] : i;
     if (e < 0 || e >= a.env_count) return;
     const int n = a.arena_ticks ? a.arena_ticks[i] : ticks;
-    core::BallTickScratch scratch;
-    if constexpr (kStage) {
-        BallState ball = a.balls[e];
-        EnvState env = a.envs[e];
-        core::ArenaInternal st = a.internal[e];
-        for (int t = 0; t < n; ++t) core::StepBallArena(ball, env, st, a.mesh, a.params, scratch);
-        a.internal[e] = st;
-        a.envs[e] = env;
-        a.balls[e] = ball;
+    const int slot = (static_cast<int>(threadIdx.x) >> 6) * a.arenas_per_wave + lane;
+    if constexpr (kStage == kStageInPlace || kStage == kStagePrivate) {
+        core::BallTickScratch scratch;
+        if constexpr (kStage == kStagePrivate) {
+            BallState ball = a.balls[e];
+            EnvState env = a.envs[e];
+            core::ArenaInternal st = a.internal[e];
+            for (int t = 0; t < n; ++t) core::StepBallArena(ball, env, st, a.mesh, a.params, scratch);
+            a.internal[e] = st;
+            a.envs[e] = env;
+            a.balls[e] = ball;
+        } else {
+            for (int t = 0; t < n; ++t) {
+                core::StepBallArena(a.balls[e], a.envs[e], a.internal[e], a.mesh, a.params, scratch);
+            }
+        }
+    } else if constexpr (kStage == kStageLdsScratch) {
+        core::BallTickScratch& scratch = reinterpret_cast<core::BallTickScratch*>(lds)[slot];
+        for (int t = 0; t < n; ++t) core::StepBallArena(a.balls[e], a.envs[e], a.internal[e], a.mesh, a.params, scratch);
     } else {
-        BallState& ball = a.balls[e];
-        EnvState& env = a.envs[e];
-        core::ArenaInternal& st = a.internal[e];
-        for (int t = 0; t < n; ++t) core::StepBallArena(ball, env, st, a.mesh, a.params, scratch);
+        LdsSlice& sl = reinterpret_cast<LdsSlice*>(lds)[slot];
+        sl.ball = a.balls[e];
+        sl.env = a.envs[e];
+        sl.st = a.internal[e];
+        for (int t = 0; t < n; ++t) core::StepBallArena(sl.ball, sl.env, sl.st, a.mesh, a.params, sl.scratch);
+        a.internal[e] = sl.st;
+        a.envs[e] = sl.env;
+        a.balls[e] = sl.ball;
     }
 }
 
@@ -310,15 +343,35 @@ public:
         block_ = block;
     }
     void SetStage(int stage) {
-        if (stage != 0 && stage != 1) throw InvalidArgument("stage must be 0 or 1");
-        stage_ = stage != 0;
+        if (stage < kStageInPlace || stage > kStageLds) throw InvalidArgument("stage must be 0, 1, 2 or 3");
+        stage_ = stage;
     }
     void SetArenasPerWave(int n) {
         if (n != 64 && n != 32 && n != 16 && n != 8 && n != 4) throw InvalidArgument("arenas per wave must be 64, 32, 16, 8 or 4");
         per_wave_ = n;
     }
-    int LaunchShape() const { return (block_ * 2 + (stage_ ? 1 : 0)) * 128 + per_wave_; }
+    int LaunchShape() const { return (block_ * 4 + stage_) * 128 + per_wave_; }
+
+    // Rejects a shape whose LDS slices do not fit one workgroup's LDS.
+    void CheckLaunchShape() const {
+        const size_t lds = LdsBytesPerLane(stage_) * static_cast<size_t>(block_ / 64) * per_wave_;
+        int device = 0, limit = 0;
+        Check(hipGetDevice(&device), "hipGetDevice");
+        Check(hipDeviceGetAttribute(&limit, hipDeviceAttributeMaxSharedMemoryPerBlock, device), "hipDeviceGetAttribute");
+        if (lds > static_cast<size_t>(limit)) {
+            throw InvalidArgument("launch shape needs " + std::to_string(lds) + " B of LDS per workgroup; the device allows " +
+                                  std::to_string(limit));
+        }
+        if (lds > 65536) {
+            Check(hipFuncSetAttribute(reinterpret_cast<const void*>(&StepBallArenasKernel<kStageLdsScratch>),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, limit), "hipFuncSetAttribute");
+            Check(hipFuncSetAttribute(reinterpret_cast<const void*>(&StepBallArenasKernel<kStageLds>),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, limit), "hipFuncSetAttribute");
+        }
+    }
     int block() const { return block_; }
+    int stage() const { return stage_; }
+    int arenas_per_wave() const { return per_wave_; }
 
     void StepWithDeviceControls(const void* controls, int32_t ticks, int32_t stride, hipStream_t s) {
         if (stride != 0 && stride < car_capacity()) {
@@ -370,10 +423,20 @@ private:
         a.arenas_per_wave = per_wave_;
         const int64_t threads = (static_cast<int64_t>(count) + per_wave_ - 1) / per_wave_ * 64;
         const int grid = static_cast<int>((threads + block_ - 1) / block_);
-        if (stage_) {
-            hipLaunchKernelGGL(StepBallArenasKernel<true>, dim3(grid), dim3(block_), 0, s, a, ticks);
-        } else {
-            hipLaunchKernelGGL(StepBallArenasKernel<false>, dim3(grid), dim3(block_), 0, s, a, ticks);
+        const size_t lds = LdsBytesPerLane(stage_) * static_cast<size_t>(block_ / 64) * per_wave_;
+        switch (stage_) {
+        case kStageInPlace:
+            hipLaunchKernelGGL(StepBallArenasKernel<kStageInPlace>, dim3(grid), dim3(block_), 0, s, a, ticks);
+            break;
+        case kStagePrivate:
+            hipLaunchKernelGGL(StepBallArenasKernel<kStagePrivate>, dim3(grid), dim3(block_), 0, s, a, ticks);
+            break;
+        case kStageLdsScratch:
+            hipLaunchKernelGGL(StepBallArenasKernel<kStageLdsScratch>, dim3(grid), dim3(block_), lds, s, a, ticks);
+            break;
+        default:
+            hipLaunchKernelGGL(StepBallArenasKernel<kStageLds>, dim3(grid), dim3(block_), lds, s, a, ticks);
+            break;
         }
         Check(hipGetLastError(), "StepBallArenasKernel launch");
     }
@@ -396,7 +459,7 @@ private:
     DeviceArray<CarControls> controls_;
     DeviceArray<uint64_t> digests_;
     int block_ = 64;
-    bool stage_ = false;
+    int stage_ = kStageInPlace;
     int per_wave_ = 64;
     hipStream_t capture_stream_ = nullptr;
     hipGraphExec_t graph_exec_ = nullptr;
@@ -532,9 +595,18 @@ int rg_batch_step_graph(rg_batch* b, int32_t ticks_per_launch, int32_t launches,
 
 int rg_batch_set_launch_shape(rg_batch* b, int32_t block, int32_t stage, int32_t arenas_per_wave) {
     return Guard([&] {
-        b->sim->SetBlock(block);
-        b->sim->SetStage(stage);
-        b->sim->SetArenasPerWave(arenas_per_wave);
+        const int old_block = b->sim->block(), old_stage = b->sim->stage(), old_per_wave = b->sim->arenas_per_wave();
+        try {
+            b->sim->SetBlock(block);
+            b->sim->SetStage(stage);
+            b->sim->SetArenasPerWave(arenas_per_wave);
+            b->sim->CheckLaunchShape();
+        } catch (...) {
+            b->sim->SetBlock(old_block);
+            b->sim->SetStage(old_stage);
+            b->sim->SetArenasPerWave(old_per_wave);
+            throw;
+        }
     });
 }
 

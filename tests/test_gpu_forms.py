@@ -82,9 +82,10 @@ def test_graph_replays_match_reference(ticks_per_launch, launches):
 
 
 @pytest.mark.parametrize("block,stage,per_wave", [(64, 0, 64), (128, 0, 64), (256, 0, 32), (64, 1, 64),
-                                                  (128, 1, 16), (256, 1, 8), (64, 0, 4), (64, 1, 8)])
+                                                  (128, 1, 16), (256, 1, 8), (64, 0, 4), (64, 1, 8),
+                                                  (64, 2, 16), (128, 2, 8), (64, 3, 8), (128, 3, 4), (256, 3, 4)])
 def test_launch_shapes_match_reference(block, stage, per_wave):
-    """Every launch shape (lanes per workgroup, state staging) gives the reference's result, every byte."""
+    """Every launch shape (lanes per workgroup, state staging, arenas per wave) gives the reference's result."""
     _torch()
     f = load_fixture("ball_arena_1024_t3000")
     sim = make_sim(f)
@@ -127,6 +128,19 @@ def test_serving_results_same_alone_and_in_the_mix():
     alone.synchronize()
     assert np.array_equal(alone.download_state_array("ball_states"), served)
     alone.close()
+
+
+def test_launch_shape_rejects_oversized_lds():
+    _torch()
+    import rocketgpu_amd as rga
+
+    sim = rga.BatchSim(64, 2)
+    with pytest.raises(rga.RocketGpuError, match="LDS"):
+        sim.set_launch_shape(256, 3, 64)
+    with pytest.raises(rga.RocketGpuError):
+        sim.set_launch_shape(64, 5, 64)
+    sim.step(2)  # the previous shape still works
+    sim.close()
 
 
 def test_serving_rejects_bad_configs():
