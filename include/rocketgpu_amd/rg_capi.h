@@ -120,6 +120,13 @@ int rg_batch_step_graph(rg_batch* batch, int32_t ticks_per_launch, int32_t launc
  * identical for every shape; the benchmark tunes it.  Defaults 64, 0, 64. */
 int rg_batch_set_launch_shape(rg_batch* batch, int32_t block, int32_t stage, int32_t arenas_per_wave);
 
+/* ball_path(samples, ticks, goal_y, out, result, stream)  (rocketgpu.h:1069-1073): every arena's ball stepped in
+ * place for samples x ticks ticks; pos and vel after every `ticks` ticks into device_out[env][sample][6] (f32), and
+ * into device_result[env][2] the goal it enters (0 none, 1 the +y goal, 2 the -y goal) and the tick it crosses
+ * |y| = goal_y on, after which it is not stepped.  The arenas must hold no cars. */
+int rg_batch_ball_path(rg_batch* batch, int32_t samples, int32_t ticks, float goal_y, float* device_out,
+                       int32_t* device_result, void* stream);
+
 /* synchronize(stream)  (rocketgpu.h:1084). */
 int rg_batch_synchronize(rg_batch* batch, void* stream);
 

@@ -242,6 +242,12 @@ int rgref_step(void* p, int ticks) {
     catch (const std::exception& e) { return Fail(e); }
 }
 
+// BatchSim::ball_path (host pointers in the CPU build): out[env][sample][6], result[env][2].
+int rgref_ball_path(void* p, int samples, int ticks, float goal_y, float* out, int32_t* result) {
+    try { static_cast<RefSim*>(p)->sim->ball_path(samples, ticks, goal_y, out, result); return 0; }
+    catch (const std::exception& e) { return Fail(e); }
+}
+
 int rgref_step_with_controls(void* p, const void* controls_by_tick, int ticks, int stride) {
     try {
         static_cast<RefSim*>(p)->sim->step_with_device_controls(static_cast<const CarControls*>(controls_by_tick),

@@ -57,6 +57,8 @@ def lib():
             f = getattr(L, name)
             f.argtypes = args
             f.restype = ctypes.c_int
+        L.rgref_ball_path.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_float, cp, cp]
+        L.rgref_ball_path.restype = ctypes.c_int
         L.rgref_default_state.argtypes = [ctypes.c_int, cp]
         L.rgref_workload_ball_states.argtypes = [ctypes.c_uint64, ctypes.c_long, ctypes.c_long, ctypes.c_double, cp]
         L.rgref_math.argtypes = [ctypes.c_int, cp, cp, cp, ctypes.c_long]
@@ -166,6 +168,14 @@ class RefSim:
         out = np.zeros(self.env_count, dtype=np.uint64)
         self._check(self._L.rgref_env_digests(self.h, _ptr(a), _ptr(out)))
         return out
+
+    def ball_path(self, samples: int, ticks: int, goal_y: float):
+        """BatchSim::ball_path: (out [env, samples, 6] f32 pos+vel, result [env, 2] i32 goal side and tick)."""
+        out = np.zeros((self.env_count, samples, 6), np.float32)
+        res = np.zeros((self.env_count, 2), np.int32)
+        if self._L.rgref_ball_path(self.h, samples, ticks, goal_y, _ptr(out), _ptr(res)) != 0:
+            raise RuntimeError(self._L.rgref_last_error().decode())
+        return out, res
 
     def run(self, car_counts, ctrl_seed: int, ticks: int, checkpoint_every: int = 0, p_boost: float = 0.3):
         a = np.ascontiguousarray(car_counts, dtype=np.int32)

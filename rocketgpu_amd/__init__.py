@@ -99,6 +99,7 @@ def lib():
         L.rg_workload_ball_states.argtypes = [ctypes.c_uint64, ctypes.c_int64, ctypes.c_int64, ctypes.c_double, vp]
         L.rg_batch_step_arenas.argtypes = [vp, vp, vp, ctypes.c_int32, ctypes.c_int32, vp]
         L.rg_batch_step_graph.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, vp]
+        L.rg_batch_ball_path.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_float, vp, vp, vp]
         L.rg_batch_set_launch_shape.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]
         L.rg_serve_run.argtypes = [vp, ctypes.POINTER(ServeConfig), ctypes.POINTER(ServeReport), vp]
         L.rg_fold_digests.argtypes = [vp, ctypes.c_int64]
@@ -199,6 +200,20 @@ class BatchSim:
     def step_graph(self, ticks_per_launch: int, launches: int = 1, stream=None):
         """`launches` x step(ticks_per_launch), captured once into a hipGraph and replayed."""
         _check(lib().rg_batch_step_graph(self._h, ticks_per_launch, launches, _stream_ptr(stream)))
+
+    def ball_path(self, samples: int, ticks: int, goal_y: float, out_ptr: int = 0, result_ptr: int = 0, stream=None):
+        """BatchSim::ball_path: every arena's ball stepped in place for samples x ticks ticks.  With device pointers,
+        writes out [env, samples, 6] f32 and result [env, 2] i32 there; without, returns them as torch tensors."""
+        ret = None
+        if not out_ptr or not result_ptr:
+            import torch
+
+            out = torch.zeros((self.env_count, samples, 6), dtype=torch.float32, device="cuda")
+            res = torch.zeros((self.env_count, 2), dtype=torch.int32, device="cuda")
+            out_ptr, result_ptr, ret = out.data_ptr(), res.data_ptr(), (out, res)
+        _check(lib().rg_batch_ball_path(self._h, samples, ticks, goal_y, ctypes.c_void_p(out_ptr),
+                                        ctypes.c_void_p(result_ptr), _stream_ptr(stream)))
+        return ret
 
     def set_launch_shape(self, block: int = 64, stage: int = 0, arenas_per_wave: int = 64):
         """Lanes per workgroup, state staging (0 in place, 1 lane-private copy), arenas per 64-lane wavefront."""

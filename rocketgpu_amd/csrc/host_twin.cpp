@@ -2,6 +2,7 @@
 // compiled for the host, so that the CPU test suite can compare the core against the oracle (the reference CPU
 // build) without a GPU, and so a GPU mismatch can be localized to the compiler or the device path.  It is not part
 // of the product: BatchSim never calls it and the Python package never loads it.
+#include <cmath>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -154,6 +155,32 @@ int rgt_env_digests(void* p, uint64_t* out, uint64_t* batch) {
         out[e] = rgw::BodyDigest(rgw::kFnvOffset, &b.pos.x, &b.vel.x, &b.ang_vel.x, rot);
     }
     if (batch) *batch = rgw::Fnv1a(rgw::kFnvOffset, out, t.balls.size() * sizeof(uint64_t));
+    return 0;
+}
+
+// ball_path on the host: the same loop as the device kernel (BallPathEnv, env_step.hpp:7217-7259).
+int rgt_ball_path(void* p, int samples, int ticks, float goal_y, float* out, int32_t* result) {
+    Twin& t = *static_cast<Twin*>(p);
+    auto scratch = std::make_unique<core::BallTickScratch>();
+    for (size_t e = 0; e < t.balls.size(); ++e) {
+        BallState& ball = t.balls[e];
+        bool scored = std::fabs(ball.pos.y) >= goal_y;
+        int goal_tick = 0;
+        float* o = out + e * static_cast<size_t>(samples) * 6;
+        for (int s = 0; s < samples; ++s) {
+            for (int k = 0; k < ticks && !scored; ++k) {
+                core::StepBallArena(ball, t.envs[e], t.internal[e], t.view, t.params, *scratch);
+                if (std::fabs(ball.pos.y) >= goal_y) {
+                    scored = true;
+                    goal_tick = s * ticks + k + 1;
+                }
+            }
+            const float v[6] = {ball.pos.x, ball.pos.y, ball.pos.z, ball.vel.x, ball.vel.y, ball.vel.z};
+            std::memcpy(o + 6 * s, v, sizeof(v));
+        }
+        result[2 * e] = scored ? (ball.pos.y > 0.0f ? 1 : 2) : 0;
+        result[2 * e + 1] = goal_tick;
+    }
     return 0;
 }
 

@@ -155,6 +155,38 @@ __device__ inline uint64_t Fnv(uint64_t h, const void* data, int n) {
     return h;
 }
 
+// Ball path (BallPathEnv, env_step.hpp:7217-7259): every arena's ball stepped in place for samples x ticks ticks;
+// after each `ticks` ticks its pos and vel go to out[arena][sample][6].  Once |pos.y| reaches goal_y the arena is
+// not stepped any more (the remaining samples repeat that state); result[arena] = {goal side (1 +y, 2 -y, 0 none),
+// the tick it crossed on (0 if it started there)}.
+__global__ void __launch_bounds__(64) BallPathKernel(StepArgs a, int32_t samples, int32_t ticks, float goal_y,
+                                                      float* out, int32_t* result) {
+    const int e = static_cast<int>(blockIdx.x) * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x);
+    if (e >= a.env_count) return;
+    core::BallTickScratch scratch;
+    BallState& ball = a.balls[e];
+    bool scored = fabsf(ball.pos.y) >= goal_y;
+    int goal_tick = 0;
+    float* o = out + static_cast<size_t>(e) * samples * 6;
+    for (int s = 0; s < samples; ++s) {
+        for (int k = 0; k < ticks && !scored; ++k) {
+            core::StepBallArena(ball, a.envs[e], a.internal[e], a.mesh, a.params, scratch);
+            if (fabsf(ball.pos.y) >= goal_y) {
+                scored = true;
+                goal_tick = s * ticks + k + 1;
+            }
+        }
+        o[6 * s + 0] = ball.pos.x;
+        o[6 * s + 1] = ball.pos.y;
+        o[6 * s + 2] = ball.pos.z;
+        o[6 * s + 3] = ball.vel.x;
+        o[6 * s + 4] = ball.vel.y;
+        o[6 * s + 5] = ball.vel.z;
+    }
+    result[2 * e] = scored ? (ball.pos.y > 0.0f ? 1 : 2) : 0;
+    result[2 * e + 1] = goal_tick;
+}
+
 __global__ void EnvDigestKernel(const BallState* balls, const CarState* cars, const int32_t* env_car_count,
                                 int32_t env_count, int32_t max_cars, uint64_t* out) {
     const int e = static_cast<int>(blockIdx.x) * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x);
@@ -373,11 +405,13 @@ public:
     int stage() const { return stage_; }
     int arenas_per_wave() const { return per_wave_; }
 
+    // Same argument checks and messages as the reference (src/rocketgpu.cpp:2412-2421).
     void StepWithDeviceControls(const void* controls, int32_t ticks, int32_t stride, hipStream_t s) {
+        if (ticks <= 0) return;
+        if (controls == nullptr) throw InvalidArgument("device_controls_by_tick must not be null");
         if (stride != 0 && stride < car_capacity()) {
-            throw InvalidArgument("control_stride must be 0 or at least car_capacity");
+            throw InvalidArgument("control_stride must be zero or at least the total car capacity");
         }
-        if (ticks > 0 && car_capacity() > 0 && controls == nullptr) throw InvalidArgument("controls pointer is null");
         Step(ticks, s);
     }
 
@@ -388,6 +422,28 @@ public:
         v->ball_states = balls_.data();
         v->env_states = envs_.data();
         v->env_car_count = car_count_.data();
+    }
+
+    // ball_path (rocketgpu.h:1069-1073): device out [env][samples][6] and result [env][2].  The arenas must hold
+    // no cars, as in the reference; this build's arenas never do.
+    void BallPath(int32_t samples, int32_t ticks, float goal_y, float* out, int32_t* result, hipStream_t s) {
+        if (samples <= 0 || ticks <= 0) return;
+        if (!out || !result) throw InvalidArgument("ball_path output pointers must not be null");
+        StepArgs a{};
+        a.balls = balls_.data();
+        a.envs = envs_.data();
+        a.internal = internal_.data();
+        a.env_car_count = car_count_.data();
+        a.mesh.tris = tris_.data();
+        a.mesh.nodes = nodes_.data();
+        a.mesh.meshes = infos_.data();
+        a.mesh.mesh_count = static_cast<int32_t>(mesh_host_.meshes.size());
+        a.mesh.tri_count = static_cast<int32_t>(mesh_host_.tris.size());
+        a.params = params_;
+        a.env_count = cfg_.env_count;
+        const int grid = (cfg_.env_count + 63) / 64;
+        hipLaunchKernelGGL(BallPathKernel, dim3(grid), dim3(64), 0, s, a, samples, ticks, goal_y, out, result);
+        Check(hipGetLastError(), "BallPathKernel launch");
     }
 
     void EnvDigests(uint64_t* out, hipStream_t s) {
@@ -617,6 +673,11 @@ int rg_batch_synchronize(rg_batch* b, void* stream) {
 
 int rg_batch_device_view(rg_batch* b, rg_device_view* out) {
     return Guard([&] { b->sim->DeviceView(out); });
+}
+
+int rg_batch_ball_path(rg_batch* b, int32_t samples, int32_t ticks, float goal_y, float* device_out,
+                       int32_t* device_result, void* stream) {
+    return Guard([&] { b->sim->BallPath(samples, ticks, goal_y, device_out, device_result, S(stream)); });
 }
 
 int rg_batch_env_digests(rg_batch* b, uint64_t* device_out, void* stream) {

@@ -22,8 +22,8 @@ def load_fixture(name):
     return d
 
 
-def fixture_names():
-    return sorted(f[:-4] for f in os.listdir(GOLDEN) if f.endswith(".npz"))
+def fixture_names(prefix="ball_arena_"):
+    return sorted(f[:-4] for f in os.listdir(GOLDEN) if f.endswith(".npz") and f.startswith(prefix))
 
 
 TWIN_PATH = os.path.join(REPO, "rocketgpu_amd", "_build", "librg_twin.so")
@@ -43,6 +43,7 @@ def twin():
     T.rgt_step.argtypes = [vp, ctypes.c_int]
     T.rgt_env_digests.argtypes = [vp, vp, vp]
     T.rgt_math.argtypes = [ctypes.c_int, vp, vp, vp, ctypes.c_long]
+    T.rgt_ball_path.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_float, vp, vp]
     T.rgt_mesh_tables.argtypes = [vp, vp, vp, vp, vp, vp]
     return T
 

@@ -57,9 +57,38 @@ def make(name, n, ticks, every, seed, scale, fillet):
     print(f"{name}: {n} arenas x {ticks} ticks in {dt:.1f}s -> {os.path.getsize(path)} bytes")
 
 
+BALL_PATH_FIXTURES = {
+    # name: (arenas, samples, ticks per sample, goal_y, seed, scale, fillet)
+    "ball_path_256_s40_t6": (256, 40, 6, 5124.2373046875, 21, 1.0, 256.0),
+    "ball_path_small_scale_64_s25_t9": (64, 25, 9, 4000.0, 23, 0.8, 192.0),
+}
+
+
+def make_ball_path(name, n, samples, ticks, goal_y, seed, scale, fillet):
+    """BatchSim::ball_path on seeded ball launches: out [n, samples, 6] f32, result [n, 2] i32, final states."""
+    s = R.RefSim(n, 2, scale, fillet)
+    s.set_env_car_count(np.zeros(n, np.int32))
+    init_env = np.tile(R.default_state(R.ENV), (n, 1))
+    init_ball = R.workload_ball_states(seed, 0, n, scale)
+    s.upload(R.ENV, init_env)
+    s.upload(R.BALL, init_ball)
+    out, res = s.ball_path(samples, ticks, goal_y)
+    meta = {"workload": "ball_path over seeded ball launches", "arenas": n, "samples": samples, "ticks": ticks,
+            "goal_y": goal_y, "seed": seed, "scale": scale, "fillet_radius": fillet,
+            "generator": "oracle/_ref reference CPU build"}
+    path = os.path.join(HERE, name + ".npz")
+    np.savez_compressed(path, init_ball=init_ball, init_env=init_env, out=out, result=res,
+                        final_ball=s.download(R.BALL, n), final_env=s.download(R.ENV, n),
+                        meta=np.frombuffer(json.dumps(meta).encode(), np.uint8))
+    print(f"{name}: {int((res[:, 0] > 0).sum())} of {n} reach a goal -> {os.path.getsize(path)} bytes")
+
+
 def main(names):
-    for name in names or FIXTURES:
-        make(name, *FIXTURES[name])
+    for name in names or list(FIXTURES) + list(BALL_PATH_FIXTURES):
+        if name in FIXTURES:
+            make(name, *FIXTURES[name])
+        else:
+            make_ball_path(name, *BALL_PATH_FIXTURES[name])
 
 
 if __name__ == "__main__":
