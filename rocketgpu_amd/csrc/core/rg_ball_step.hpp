@@ -674,8 +674,8 @@ RG_HD void BallMeshContacts(BallState& ball, const StepParams& p, ArenaInternal&
             QuantizePoint(info.q, center + Vec3(r_th, r_th, r_th), 1, qmax);
             Vec3 bx, by, bz;
             BallBasis(ball, bx, by, bz);
-            QueryMeshLeaves(
-                mesh, info, qmin, qmax,
+            WalkMeshBvh(
+                mesh.nodes, info.node_start, info.node_start + info.node_count, qmin, qmax,
                 [&](int32_t local) {
                     const int32_t g = info.tri_start + local;
                     const MeshTriangle& tri = mesh.tris[g];

@@ -64,28 +64,8 @@ struct MeshInfo {
     int32_t node_start;
     int32_t node_count;
     int32_t ray_order;  // meshes[k].ray_order: the mesh a ray visits k-th (by quantized max x, stable)
-    int32_t grid_offset;  // this mesh's kGridCellsPerMesh + 1 cell starts in ArenaMeshView::cell_start
-    int32_t pad;
+    int32_t pad[2];
 };
-
-// Leaf grid of one mesh tree, in the tree's own quantized coordinates (16 bits per axis): 16 x 16 x 16 cells of
-// 4096 quantized units.  Cell c lists, in increasing node order, every leaf whose quantized box overlaps the cell
-// (its box and triangle).  A parent's quantized box is the union of its children's, so the stackless walk reaches
-// exactly the leaves whose box overlaps the query, in increasing node order; merging the lists of the cells the
-// query covers by node index and testing each leaf's box gives the same leaves in the same order, with independent
-// loads instead of a chain of dependent node reads.
-constexpr int kGridCells = 16;
-constexpr int kGridShift = 12;
-constexpr int kGridCellsPerMesh = kGridCells * kGridCells * kGridCells;
-constexpr int kGridMaxMergeCells = 8;  // larger queries walk the tree
-
-struct LeafEntry {
-    uint16_t qmin[3];
-    uint16_t qmax[3];
-    int32_t node;  // mesh-local node index (the merge key)
-    int32_t tri;   // mesh-local triangle index
-};
-static_assert(sizeof(LeafEntry) == 20, "LeafEntry is 20 bytes");
 
 // Everything one arena geometry needs on the device.  Pointers are device addresses (host addresses in the
 // host-compiled core).
@@ -95,8 +75,6 @@ struct ArenaMeshView {
     const MeshInfo* meshes;
     int32_t mesh_count;
     int32_t tri_count;
-    const int32_t* cell_start;  // per mesh kGridCellsPerMesh + 1 offsets into leaves, or null (then queries walk)
-    const LeafEntry* leaves;
 };
 
 // ------------------------------------------------------------------------------------------------------------
@@ -139,54 +117,6 @@ RG_HD void WalkMeshBvh(const MeshBvhNode* nodes, int begin, int end, const uint1
         const bool is_leaf = node.escape_or_tri >= 0;
         if (is_leaf && overlap && accept(node.qmin, node.qmax)) leaf(node.escape_or_tri);
         i += (overlap || is_leaf) ? 1 : -node.escape_or_tri;
-    }
-}
-
-// Every leaf of the mesh whose quantized box overlaps [qmin, qmax] and passes `accept`, in increasing node order:
-// the same calls, in the same order, as WalkMeshBvh over the mesh's tree (see kGridCells).
-template <typename Leaf, typename Accept>
-RG_HD void QueryMeshLeaves(const ArenaMeshView& m, const MeshInfo& info, const uint16_t* qmin, const uint16_t* qmax,
-                           Leaf&& leaf, Accept&& accept) {
-    const int x0 = qmin[0] >> kGridShift, x1 = qmax[0] >> kGridShift;
-    const int y0 = qmin[1] >> kGridShift, y1 = qmax[1] >> kGridShift;
-    const int z0 = qmin[2] >> kGridShift, z1 = qmax[2] >> kGridShift;
-    const int cells = (x1 - x0 + 1) * (y1 - y0 + 1) * (z1 - z0 + 1);
-    if (m.cell_start == nullptr || cells > kGridMaxMergeCells) {
-        WalkMeshBvh(m.nodes, info.node_start, info.node_start + info.node_count, qmin, qmax, leaf, accept);
-        return;
-    }
-    const int32_t* starts = m.cell_start + info.grid_offset;
-    int32_t cur[kGridMaxMergeCells], end[kGridMaxMergeCells];
-    int n = 0;
-    for (int z = z0; z <= z1; ++z) {
-        for (int y = y0; y <= y1; ++y) {
-            for (int x = x0; x <= x1; ++x) {
-                const int c = (z * kGridCells + y) * kGridCells + x;
-                cur[n] = starts[c];
-                end[n] = starts[c + 1];
-                ++n;
-            }
-        }
-    }
-    int32_t last = -1;
-    while (true) {
-        int best = -1;
-        int32_t best_node = 0;
-        for (int k = 0; k < n; ++k) {
-            if (cur[k] < end[k]) {
-                const int32_t node = m.leaves[cur[k]].node;
-                if (best < 0 || node < best_node) {
-                    best = k;
-                    best_node = node;
-                }
-            }
-        }
-        if (best < 0) break;
-        const LeafEntry e = m.leaves[cur[best]];
-        ++cur[best];
-        if (e.node == last) continue;
-        last = e.node;
-        if (QuantizedOverlap(qmin, qmax, e.qmin, e.qmax) && accept(e.qmin, e.qmax)) leaf(e.tri);
     }
 }
 
@@ -284,8 +214,8 @@ RG_HD RayHit CastArenaRay(const ArenaMeshView& m, const Vec3& from, const Vec3& 
         QuantizePoint(info.q, lo, 0, qmin);
         QuantizePoint(info.q, hi, 1, qmax);
         const RayLeafTest test(info.q, from, to);
-        QueryMeshLeaves(
-            m, info, qmin, qmax,
+        WalkMeshBvh(
+            m.nodes, info.node_start, info.node_start + info.node_count, qmin, qmax,
             [&](int32_t local) {
                 const int32_t g = info.tri_start + local;
                 RayTriangle(m.tris[g], g, from, to, best);

@@ -42,8 +42,6 @@ void* rgt_create(int env_count, double scale, double fillet_radius) {
         t->view.nodes = t->mesh.nodes.data();
         t->view.meshes = t->mesh.meshes.data();
         t->view.mesh_count = static_cast<int32_t>(t->mesh.meshes.size());
-        t->view.cell_start = t->mesh.cell_start.data();
-        t->view.leaves = t->mesh.leaves.data();
         t->view.tri_count = static_cast<int32_t>(t->mesh.tris.size());
         t->params = DefaultStepParams(rgw::ProceduralArenaRules(ap));
         t->balls.assign(static_cast<size_t>(env_count), BallState{});
@@ -186,13 +184,11 @@ int rgt_ball_path(void* p, int samples, int ticks, float goal_y, float* out, int
     return 0;
 }
 
-// Invariants of the mesh tables.  out[0]: internal nodes whose quantized box does not contain a node of their
-// subtree; out[1]: random query boxes on which the leaf grid and the tree walk report different leaf sequences;
-// out[2]: leaves reported in total (so a test can see the queries were not all empty); out[3]: queries run.
-int rgt_mesh_invariants(void* p, int queries, uint64_t seed, long* out) {
-    using namespace core;
+// Invariant of the mesh trees the stackless walk relies on: a parent's quantized box contains every node of its
+// subtree.  Returns the number of violations.
+long rgt_bvh_containment_violations(void* p) {
     const Twin& t = *static_cast<Twin*>(p);
-    long bad_contain = 0, bad_query = 0, leaves = 0, runs = 0;
+    long bad = 0;
     for (const core::MeshInfo& info : t.mesh.meshes) {
         const core::MeshBvhNode* nodes = t.mesh.nodes.data() + info.node_start;
         for (int i = 0; i < info.node_count; ++i) {
@@ -200,36 +196,12 @@ int rgt_mesh_invariants(void* p, int queries, uint64_t seed, long* out) {
             const int end = i - nodes[i].escape_or_tri;
             for (int j = i + 1; j < end && j < info.node_count; ++j) {
                 for (int a = 0; a < 3; ++a) {
-                    if (nodes[j].qmin[a] < nodes[i].qmin[a] || nodes[j].qmax[a] > nodes[i].qmax[a]) ++bad_contain;
+                    if (nodes[j].qmin[a] < nodes[i].qmin[a] || nodes[j].qmax[a] > nodes[i].qmax[a]) ++bad;
                 }
             }
         }
     }
-    uint64_t s = seed;
-    for (int q = 0; q < queries; ++q) {
-        const core::MeshInfo& info = t.mesh.meshes[rgw::SplitMix64(s) % t.mesh.meshes.size()];
-        const Vec3 span = info.q.bvh_max - info.q.bvh_min;
-        const Vec3 c(info.q.bvh_min.x + span.x * rgw::UnitFloat(rgw::SplitMix64(s)),
-                     info.q.bvh_min.y + span.y * rgw::UnitFloat(rgw::SplitMix64(s)),
-                     info.q.bvh_min.z + span.z * rgw::UnitFloat(rgw::SplitMix64(s)));
-        const float h = 0.2f + 12.0f * rgw::UnitFloat(rgw::SplitMix64(s)) * rgw::UnitFloat(rgw::SplitMix64(s));
-        uint16_t qmin[3], qmax[3];
-        core::QuantizePoint(info.q, c - Vec3(h, h, h), 0, qmin);
-        core::QuantizePoint(info.q, c + Vec3(h, h, h), 1, qmax);
-        std::vector<int32_t> walk, grid;
-        core::WalkMeshBvh(t.mesh.nodes.data(), info.node_start, info.node_start + info.node_count, qmin, qmax,
-                          [&](int32_t tri) { walk.push_back(tri); }, core::AcceptAll{});
-        core::QueryMeshLeaves(t.view, info, qmin, qmax, [&](int32_t tri) { grid.push_back(tri); },
-                              core::AcceptAll{});
-        if (walk != grid) ++bad_query;
-        leaves += static_cast<long>(walk.size());
-        ++runs;
-    }
-    out[0] = bad_contain;
-    out[1] = bad_query;
-    out[2] = leaves;
-    out[3] = runs;
-    return 0;
+    return bad;
 }
 
 int rgt_math(int fn, const float* x, const float* y, float* out, long n) {

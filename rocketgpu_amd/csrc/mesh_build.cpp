@@ -349,43 +349,6 @@ ArenaMeshHost BuildArenaMesh(const std::vector<MeshSource>& sources) {
         info.tri_count = n;
         info.node_start = static_cast<int32_t>(out.nodes.size());
         info.node_count = static_cast<int32_t>(nodes.size());
-        // Leaf grid: count, prefix-sum, fill; nodes visited in increasing order keep each cell's list sorted.
-        info.grid_offset = static_cast<int32_t>(out.cell_start.size());
-        std::vector<int32_t> counts(kGridCellsPerMesh, 0);
-        const auto for_cells = [](const MeshBvhNode& node, auto&& fn) {
-            for (int z = node.qmin[2] >> kGridShift; z <= (node.qmax[2] >> kGridShift); ++z) {
-                for (int y = node.qmin[1] >> kGridShift; y <= (node.qmax[1] >> kGridShift); ++y) {
-                    for (int x = node.qmin[0] >> kGridShift; x <= (node.qmax[0] >> kGridShift); ++x) {
-                        fn((z * kGridCells + y) * kGridCells + x);
-                    }
-                }
-            }
-        };
-        for (const MeshBvhNode& node : nodes) {
-            if (node.escape_or_tri >= 0) for_cells(node, [&](int c) { ++counts[static_cast<size_t>(c)]; });
-        }
-        const int32_t first = static_cast<int32_t>(out.leaves.size());
-        std::vector<int32_t> fill(kGridCellsPerMesh);
-        int32_t at = first;
-        for (int c = 0; c < kGridCellsPerMesh; ++c) {
-            out.cell_start.push_back(at);
-            fill[static_cast<size_t>(c)] = at;
-            at += counts[static_cast<size_t>(c)];
-        }
-        out.cell_start.push_back(at);
-        out.leaves.resize(static_cast<size_t>(at));
-        for (size_t k = 0; k < nodes.size(); ++k) {
-            const MeshBvhNode& node = nodes[k];
-            if (node.escape_or_tri < 0) continue;
-            LeafEntry e{};
-            for (int a = 0; a < 3; ++a) {
-                e.qmin[a] = node.qmin[a];
-                e.qmax[a] = node.qmax[a];
-            }
-            e.node = static_cast<int32_t>(k);
-            e.tri = node.escape_or_tri;
-            for_cells(node, [&](int c) { out.leaves[static_cast<size_t>(fill[static_cast<size_t>(c)]++)] = e; });
-        }
         out.meshes.push_back(info);
         out.nodes.insert(out.nodes.end(), nodes.begin(), nodes.end());
     }

@@ -19,8 +19,6 @@ struct ArenaMeshHost {
     std::vector<core::MeshTriangle> tris;
     std::vector<core::MeshBvhNode> nodes;
     std::vector<core::MeshInfo> meshes;
-    std::vector<int32_t> cell_start;       // core::kGridCellsPerMesh + 1 per mesh, mesh order
-    std::vector<core::LeafEntry> leaves;   // the cells' leaf lists
 };
 
 // Groups of at most 16 meshes, triangles in source order; throws std::invalid_argument on malformed input.

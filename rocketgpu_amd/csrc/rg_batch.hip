@@ -230,10 +230,6 @@ public:
         tris_.Upload(mesh_host_.tris.data(), mesh_host_.tris.size(), nullptr);
         nodes_.Upload(mesh_host_.nodes.data(), mesh_host_.nodes.size(), nullptr);
         infos_.Upload(mesh_host_.meshes.data(), mesh_host_.meshes.size(), nullptr);
-        cell_start_.Resize(mesh_host_.cell_start.size());
-        cell_start_.Upload(mesh_host_.cell_start.data(), mesh_host_.cell_start.size(), nullptr);
-        leaves_.Resize(mesh_host_.leaves.size());
-        leaves_.Upload(mesh_host_.leaves.data(), mesh_host_.leaves.size(), nullptr);
         params_ = DefaultStepParams(rules);
         params_.tick_dt = cfg.tick_dt;
         params_.ball_damping_scale =
@@ -442,8 +438,6 @@ public:
         a.mesh.nodes = nodes_.data();
         a.mesh.meshes = infos_.data();
         a.mesh.mesh_count = static_cast<int32_t>(mesh_host_.meshes.size());
-        a.mesh.cell_start = cell_start_.data();
-        a.mesh.leaves = leaves_.data();
         a.mesh.tri_count = static_cast<int32_t>(mesh_host_.tris.size());
         a.params = params_;
         a.env_count = cfg_.env_count;
@@ -476,8 +470,6 @@ private:
         a.mesh.nodes = nodes_.data();
         a.mesh.meshes = infos_.data();
         a.mesh.mesh_count = static_cast<int32_t>(mesh_host_.meshes.size());
-        a.mesh.cell_start = cell_start_.data();
-        a.mesh.leaves = leaves_.data();
         a.mesh.tri_count = static_cast<int32_t>(mesh_host_.tris.size());
         a.params = params_;
         a.env_count = cfg_.env_count;
@@ -515,8 +507,6 @@ private:
     DeviceArray<core::MeshTriangle> tris_;
     DeviceArray<core::MeshBvhNode> nodes_;
     DeviceArray<core::MeshInfo> infos_;
-    DeviceArray<int32_t> cell_start_;
-    DeviceArray<core::LeafEntry> leaves_;
     DeviceArray<BallState> balls_;
     DeviceArray<EnvState> envs_;
     DeviceArray<core::ArenaInternal> internal_;

@@ -43,7 +43,8 @@ def twin():
     T.rgt_step.argtypes = [vp, ctypes.c_int]
     T.rgt_env_digests.argtypes = [vp, vp, vp]
     T.rgt_math.argtypes = [ctypes.c_int, vp, vp, vp, ctypes.c_long]
-    T.rgt_mesh_invariants.argtypes = [vp, ctypes.c_int, ctypes.c_uint64, vp]
+    T.rgt_bvh_containment_violations.argtypes = [vp]
+    T.rgt_bvh_containment_violations.restype = ctypes.c_long
     T.rgt_ball_path.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_float, vp, vp]
     T.rgt_mesh_tables.argtypes = [vp, vp, vp, vp, vp, vp]
     return T

@@ -1,8 +1,5 @@
-"""Arena mesh tables: the host build matches the reference build's tables bit for bit, and the invariants the queries
-rely on hold (a parent's quantized box contains its subtree; the leaf grid reports the same leaves, in the same
-order, as the stackless tree walk)."""
-import ctypes
-
+"""Arena mesh tables: the host build matches the reference build's tables bit for bit, and the invariant the stackless
+walk relies on holds (a parent's quantized box contains its subtree)."""
 import numpy as np
 import pytest
 
@@ -39,16 +36,12 @@ def test_mesh_tables_match_reference(scale, fillet):
 
 
 @pytest.mark.parametrize("scale,fillet", [(1.0, 256.0), (0.8, 192.0)])
-def test_bvh_containment_and_leaf_grid_equivalence(scale, fillet):
+def test_bvh_parent_boxes_contain_subtrees(scale, fillet):
+    """The walk skips a subtree when its root's box misses the query; that only equals testing every leaf when each
+    parent box contains its subtree."""
     T = twin()
     h = T.rgt_create(1, scale, fillet)
     try:
-        out = np.zeros(4, np.int64)
-        T.rgt_mesh_invariants(h, 20000, 12345, out.ctypes.data)
-        bad_contain, bad_query, leaves, runs = out
-        assert runs == 20000
-        assert bad_contain == 0, "a parent box does not contain its subtree"
-        assert bad_query == 0, "leaf grid and tree walk disagree"
-        assert leaves > 20000, "the random queries should reach many leaves"
+        assert T.rgt_bvh_containment_violations(h) == 0
     finally:
         T.rgt_destroy(h)
