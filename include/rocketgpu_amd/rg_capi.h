@@ -100,6 +100,31 @@ int rg_batch_upload(rg_batch* batch, int array_id, const void* host_src, size_t 
 /* download_*_states (rocketgpu.h:1036-1053); synchronous like the reference's non-async downloads. */
 int rg_batch_download(rg_batch* batch, int array_id, void* host_dst, size_t count, void* stream);
 
+/* boost_pad_capacity()  (rocketgpu.h:1019): env_count x the layout's pad count. */
+int64_t rg_batch_boost_pad_capacity(const rg_batch* batch);
+/* set_teams(host_teams, count, stream)  (rocketgpu.h:1024): Team per car slot (0 Blue, 1 Orange), one byte each. */
+int rg_batch_set_teams(rg_batch* batch, const uint8_t* host_teams, size_t count, void* stream);
+/* set_car_configs(host_configs, count, stream)  (rocketgpu.h:1025): CarConfig records (120 B) per car slot. */
+int rg_batch_set_car_configs(rg_batch* batch, const void* host_configs, size_t count, void* stream);
+/* MutatorConfig::kickoff_settled (rocketgpu.h:827-832): nonzero (the default) makes a kickoff reset without a
+ * countdown put the ball where the game's countdown leaves it (computed once per batch on the GPU); zero leaves it
+ * asleep at the spawn, as RocketSim does. */
+int rg_batch_set_kickoff_settled(rg_batch* batch, int32_t on);
+
+/* reset_to_kickoff(seed, stream)  (rocketgpu.h:1065): every env to a new match's kickoff. */
+int rg_batch_reset_to_kickoff(rg_batch* batch, uint64_t seed, void* stream);
+/* apply_env_commands(host_commands, count, stream)  (rocketgpu.h:1066): EnvCommand i (16 B records,
+ * rg_types.h) to env i for i < count. */
+int rg_batch_apply_env_commands(rg_batch* batch, const void* host_commands, size_t count, void* stream);
+/* apply_env_commands_device(device_commands, count, stream)  (rocketgpu.h:1067). */
+int rg_batch_apply_env_commands_device(rg_batch* batch, const void* device_commands, size_t count, void* stream);
+/* step_with_device_commands(controls, control_stride, env_commands, env_command_stride, ticks, stream)
+ * (rocketgpu.h:1077-1083): each tick t starts with env e's command device_env_commands_by_tick[t * stride + e];
+ * afterwards the last tick's controls are the batch's controls. */
+int rg_batch_step_with_device_commands(rg_batch* batch, const void* device_controls_by_tick, int32_t control_stride,
+                                       const void* device_env_commands_by_tick, int32_t env_command_stride,
+                                       int32_t ticks, void* stream);
+
 /* step(ticks, stream)  (rocketgpu.h:1068). */
 int rg_batch_step(rg_batch* batch, int32_t ticks, void* stream);
 /* step_with_device_controls(device_controls_by_tick, ticks, control_stride, stream)  (rocketgpu.h:1075). */

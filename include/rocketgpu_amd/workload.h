@@ -42,14 +42,16 @@ struct GoalSpec {
     V3 point, normal, min, max;
 };
 
+constexpr int kMaxPads = 48;  // ArenaLayout::boost_pads holds at most kMaxBoostPads (rocketgpu.h:526)
+
 struct ArenaRulesSpec {
     GoalSpec goals[2];               // [0]: Blue's goal (-y), [1]: Orange's goal (+y)
     SpawnSpot kickoff[2][5];
     SpawnSpot respawn[2][4];
     V3 ball_spawn;
-    int pad_count;
-    V3 pad_pos[34];
-    uint8_t pad_big[34];
+    int pad_count;  // at most kMaxPads
+    V3 pad_pos[48];
+    uint8_t pad_big[48];
 };
 
 namespace detail {

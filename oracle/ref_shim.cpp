@@ -118,10 +118,43 @@ long rgref_sizeof(int which) {
     }
 }
 
-// Field offsets used by the layout-equality test: which 0 CarState, 1 BallState, 2 EnvState, 3 BoostPadState.
+// Field offsets used by the layout-equality test: which 0 CarState, 1 BallState, 2 EnvState, 3 BoostPadState,
+// 4 CarControls, 5 CarConfig.  Dotted names address a member of a nested record.
 long rgref_offsetof(int which, const char* field) {
     const std::string f(field);
 #define RGREF_FIELD(T, name) if (f == #name) return static_cast<long>(offsetof(T, name));
+#define RGREF_SUB(T, m, S, name) if (f == #m "." #name) return static_cast<long>(offsetof(T, m) + offsetof(T::S, name));
+    if (which == 1) {
+        RGREF_SUB(BallState, hs_info, HeatseekerInfo, y_target_dir)
+        RGREF_SUB(BallState, hs_info, HeatseekerInfo, cur_target_speed)
+        RGREF_SUB(BallState, hs_info, HeatseekerInfo, time_since_hit)
+        RGREF_SUB(BallState, ds_info, DropshotInfo, charge_level)
+        RGREF_SUB(BallState, ds_info, DropshotInfo, accumulated_hit_force)
+        RGREF_SUB(BallState, ds_info, DropshotInfo, y_target_dir)
+        RGREF_SUB(BallState, ds_info, DropshotInfo, has_damaged)
+        RGREF_SUB(BallState, ds_info, DropshotInfo, last_damage_tick)
+        RGREF_FIELD(BallState, vel_impulse_cache) RGREF_FIELD(BallState, tick_count_when_extra_impulse_applied)
+        RGREF_FIELD(BallState, basis_right) RGREF_FIELD(BallState, basis_up) RGREF_FIELD(BallState, basis_authoritative)
+        RGREF_FIELD(BallState, vel_impulse_cache_bt)
+    } else if (which == 2) {
+        RGREF_FIELD(EnvState, orange_score) RGREF_FIELD(EnvState, last_scored_team)
+        RGREF_FIELD(EnvState, car_visit_present_mask) RGREF_FIELD(EnvState, kickoff_resets)
+        RGREF_FIELD(EnvState, countdown_phase) RGREF_FIELD(EnvState, countdown_fires) RGREF_FIELD(EnvState, frame_ticks)
+        RGREF_FIELD(EnvState, countdown_count) RGREF_FIELD(EnvState, kickoff_match_order)
+        RGREF_FIELD(EnvState, kickoff_match_index) RGREF_FIELD(EnvState, goal_latched)
+    } else if (which == 3) {
+        RGREF_FIELD(BoostPadState, is_big) RGREF_FIELD(BoostPadState, prev_locked_car_id)
+    } else if (which == 4) {
+        RGREF_FIELD(CarControls, throttle) RGREF_FIELD(CarControls, steer) RGREF_FIELD(CarControls, pitch)
+        RGREF_FIELD(CarControls, yaw) RGREF_FIELD(CarControls, roll) RGREF_FIELD(CarControls, jump)
+        RGREF_FIELD(CarControls, boost) RGREF_FIELD(CarControls, handbrake)
+    } else if (which == 5) {
+        RGREF_FIELD(CarConfig, hitbox_size) RGREF_FIELD(CarConfig, hitbox_offset)
+        RGREF_FIELD(CarConfig, front_wheels_offset) RGREF_FIELD(CarConfig, back_wheels_offset)
+        RGREF_FIELD(CarConfig, front_wheel_radius) RGREF_FIELD(CarConfig, back_wheel_radius)
+        RGREF_FIELD(CarConfig, front_suspension_rest) RGREF_FIELD(CarConfig, back_suspension_rest)
+        RGREF_FIELD(CarConfig, dodge_deadzone) RGREF_FIELD(CarConfig, three_wheels)
+    }
     if (which == 0) {
         RGREF_FIELD(CarState, pos) RGREF_FIELD(CarState, rot) RGREF_FIELD(CarState, vel) RGREF_FIELD(CarState, ang_vel)
         RGREF_FIELD(CarState, boost) RGREF_FIELD(CarState, is_on_ground) RGREF_FIELD(CarState, is_demoed)
@@ -142,6 +175,7 @@ long rgref_offsetof(int which, const char* field) {
         RGREF_FIELD(BoostPadState, cooldown) RGREF_FIELD(BoostPadState, is_active)
         RGREF_FIELD(BoostPadState, pickup_frac_q) RGREF_FIELD(BoostPadState, cur_locked_car_id)
     }
+#undef RGREF_SUB
 #undef RGREF_FIELD
     return -1;
 }
@@ -252,6 +286,27 @@ int rgref_step_with_controls(void* p, const void* controls_by_tick, int ticks, i
     try {
         static_cast<RefSim*>(p)->sim->step_with_device_controls(static_cast<const CarControls*>(controls_by_tick),
                                                                  ticks, stride);
+        return 0;
+    } catch (const std::exception& e) { return Fail(e); }
+}
+
+// BatchSim::step_with_device_commands (host pointers in the CPU build): controls [tick][control_stride] and
+// commands [tick][command_stride].
+int rgref_step_with_commands(void* p, const void* controls_by_tick, int control_stride, const void* commands_by_tick,
+                             int command_stride, int ticks) {
+    try {
+        static_cast<RefSim*>(p)->sim->step_with_device_commands(static_cast<const CarControls*>(controls_by_tick),
+                                                                 control_stride,
+                                                                 static_cast<const EnvCommand*>(commands_by_tick),
+                                                                 command_stride, ticks);
+        return 0;
+    } catch (const std::exception& e) { return Fail(e); }
+}
+
+int rgref_apply_env_commands_device(void* p, const void* cmds, long n) {
+    try {
+        static_cast<RefSim*>(p)->sim->apply_env_commands_device(static_cast<const EnvCommand*>(cmds),
+                                                                 static_cast<size_t>(n));
         return 0;
     } catch (const std::exception& e) { return Fail(e); }
 }

@@ -52,6 +52,8 @@ def lib():
             ("rgref_apply_env_commands", [vp, cp, lg]),
             ("rgref_step", [vp, ctypes.c_int]),
             ("rgref_step_with_controls", [vp, cp, ctypes.c_int, ctypes.c_int]),
+            ("rgref_step_with_commands", [vp, cp, ctypes.c_int, cp, ctypes.c_int, ctypes.c_int]),
+            ("rgref_apply_env_commands_device", [vp, cp, lg]),
             ("rgref_env_digests", [vp, cp, cp]),
         ]:
             f = getattr(L, name)
@@ -156,8 +158,20 @@ class RefSim:
         cmds = np.ascontiguousarray(cmds, dtype=np.uint8)
         self._check(self._L.rgref_apply_env_commands(self.h, _ptr(cmds), cmds.shape[0]))
 
+    def apply_env_commands_device(self, cmds: np.ndarray):
+        cmds = np.ascontiguousarray(cmds, dtype=np.uint8)
+        self._check(self._L.rgref_apply_env_commands_device(self.h, _ptr(cmds), cmds.shape[0]))
+
     def step(self, ticks: int = 1):
         self._check(self._L.rgref_step(self.h, ticks))
+
+    def step_with_commands(self, controls: np.ndarray, control_stride: int, commands: np.ndarray,
+                           command_stride: int, ticks: int):
+        """BatchSim::step_with_device_commands: commands [ticks * command_stride, 16] u8 EnvCommand records."""
+        controls = np.ascontiguousarray(controls, dtype=np.uint8)
+        commands = np.ascontiguousarray(commands, dtype=np.uint8)
+        self._check(self._L.rgref_step_with_commands(self.h, _ptr(controls), control_stride, _ptr(commands),
+                                                     command_stride, ticks))
 
     def step_with_controls(self, controls: np.ndarray, ticks: int, stride: int):
         controls = np.ascontiguousarray(controls, dtype=np.uint8)

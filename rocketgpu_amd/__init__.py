@@ -26,7 +26,14 @@ LIB_PATH = os.path.join(HERE, "_build", "librocketgpu_amd.so")
 RG_ARRAY = {"car_states": 0, "ball_states": 1, "env_states": 2, "boost_pad_states": 3, "controls": 4,
             "env_car_count": 5}
 ITEMSIZE = {"car_states": 880, "ball_states": 288, "env_states": 44, "boost_pad_states": 16, "controls": 24,
-            "env_car_count": 4}
+            "env_car_count": 4, "car_configs": 120, "teams": 1, "env_commands": 16}
+
+# EnvCommand (include/rocketgpu/rocketgpu.h:683-707 of the reference), 16 bytes.
+ENV_COMMAND_DTYPE = np.dtype([("reset_to_kickoff", "u1"), ("reset_scoreboard", "u1"), ("reset_tick_count", "u1"),
+                              ("kickoff_seed_mode", "u1"), ("new_round_bodies", "u1"), ("keep_boost_pads", "u1"),
+                              ("kickoff_countdown_ticks", "<u2"), ("kickoff_seed", "<u8")])
+KICKOFF_COUNTDOWN_GAME_TIMERS = 0xFFFF  # kKickoffCountdownGameTimers
+KICKOFF_COUNTDOWN_TICKS = 480           # kKickoffCountdownTicks
 
 _lib = None
 
@@ -76,6 +83,13 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RocketGpuError(f"librocketgpu_amd.so is not built ({LIB_PATH}); run __graft_entry__.build()")
+        # One HIP runtime per process: torch ships its own libamdhip64.so.7.  Loaded first, it is the one this
+        # library binds to (same soname).  Loaded after this library, the process would hold two HIP/HSA runtimes
+        # and the second to open the GPU sees no device.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(LIB_PATH)
         vp = ctypes.c_void_p
         L.rg_last_error.restype = ctypes.c_char_p
@@ -104,6 +118,16 @@ def lib():
         L.rg_serve_run.argtypes = [vp, ctypes.POINTER(ServeConfig), ctypes.POINTER(ServeReport), vp]
         L.rg_fold_digests.argtypes = [vp, ctypes.c_int64]
         L.rg_fold_digests.restype = ctypes.c_uint64
+        L.rg_batch_boost_pad_capacity.argtypes = [vp]
+        L.rg_batch_boost_pad_capacity.restype = ctypes.c_int64
+        L.rg_batch_set_teams.argtypes = [vp, vp, ctypes.c_size_t, vp]
+        L.rg_batch_set_car_configs.argtypes = [vp, vp, ctypes.c_size_t, vp]
+        L.rg_batch_set_kickoff_settled.argtypes = [vp, ctypes.c_int32]
+        L.rg_batch_reset_to_kickoff.argtypes = [vp, ctypes.c_uint64, vp]
+        L.rg_batch_apply_env_commands.argtypes = [vp, vp, ctypes.c_size_t, vp]
+        L.rg_batch_apply_env_commands_device.argtypes = [vp, vp, ctypes.c_size_t, vp]
+        L.rg_batch_step_with_device_commands.argtypes = [vp, vp, ctypes.c_int32, vp, ctypes.c_int32, ctypes.c_int32,
+                                                          vp]
         L.rg_state_layouts_json.argtypes = [ctypes.c_char_p, ctypes.c_size_t]
         L.rg_state_layouts_json.restype = ctypes.c_int64
         _lib = L
@@ -151,6 +175,7 @@ class BatchSim:
         self._h = h
         self.env_count = int(L.rg_batch_env_count(h))
         self.car_capacity = int(L.rg_batch_car_capacity(h))
+        self.boost_pad_capacity = int(L.rg_batch_boost_pad_capacity(h))
         self.max_cars_per_env = max_cars_per_env
 
     def close(self):
@@ -165,7 +190,8 @@ class BatchSim:
             pass
 
     def _count(self, name):
-        return {"car_states": self.car_capacity, "controls": self.car_capacity}.get(name, self.env_count)
+        return {"car_states": self.car_capacity, "controls": self.car_capacity, "car_configs": self.car_capacity,
+                "teams": self.car_capacity, "boost_pad_states": self.boost_pad_capacity}.get(name, self.env_count)
 
     def set_env_car_count(self, counts, stream=None):
         a = np.ascontiguousarray(counts, dtype=np.int32)
@@ -178,11 +204,54 @@ class BatchSim:
             raise ValueError(f"{name}: byte size is not a multiple of {ITEMSIZE[name]}")
         _check(lib().rg_batch_upload(self._h, RG_ARRAY[name], a.ctypes.data, n, _stream_ptr(stream)))
 
-    def download_state_array(self, name: str, stream=None) -> np.ndarray:
-        n = self._count(name)
+    def download_state_array(self, name: str, host_ptr: int = 0, count: int | None = None, stream=None):
+        """download_state_array (python/rocketgpu_bindings.cpp:984-1000): `count` records (all by default) of
+        `name` into host memory at host_ptr; without a pointer, returns them as a [count, itemsize] uint8 array."""
+        n = self._count(name) if count is None else int(count)
+        if host_ptr:
+            _check(lib().rg_batch_download(self._h, RG_ARRAY[name], ctypes.c_void_p(host_ptr), n,
+                                           _stream_ptr(stream)))
+            return None
         out = np.zeros((n, ITEMSIZE[name]), dtype=np.uint8)
         _check(lib().rg_batch_download(self._h, RG_ARRAY[name], out.ctypes.data, n, _stream_ptr(stream)))
         return out
+
+    def upload_state_array(self, name: str, host_ptr: int, count: int, stream=None):
+        """upload_state_array (python/rocketgpu_bindings.cpp:955-983), public-state mode: `count` raw records."""
+        _check(lib().rg_batch_upload(self._h, RG_ARRAY[name], ctypes.c_void_p(host_ptr), count, _stream_ptr(stream)))
+
+    def set_teams(self, teams, stream=None):
+        a = np.ascontiguousarray(teams, dtype=np.uint8)
+        _check(lib().rg_batch_set_teams(self._h, a.ctypes.data, a.size, _stream_ptr(stream)))
+
+    def set_car_configs(self, configs, stream=None):
+        a = np.ascontiguousarray(configs).view(np.uint8).reshape(-1, ITEMSIZE["car_configs"])
+        _check(lib().rg_batch_set_car_configs(self._h, a.ctypes.data, a.shape[0], _stream_ptr(stream)))
+
+    def set_kickoff_settled(self, on: bool = True):
+        """MutatorConfig::kickoff_settled: a kickoff reset without a countdown puts the ball where the countdown
+        leaves it (True, the default) or asleep at its spawn (False)."""
+        _check(lib().rg_batch_set_kickoff_settled(self._h, 1 if on else 0))
+
+    def reset_to_kickoff(self, seed: int = 0, stream=None):
+        _check(lib().rg_batch_reset_to_kickoff(self._h, seed, _stream_ptr(stream)))
+
+    def apply_env_commands(self, commands, stream=None):
+        """Command i (ENV_COMMAND_DTYPE records, or [n, 16] uint8) to env i."""
+        a = np.ascontiguousarray(commands).view(np.uint8).reshape(-1, ITEMSIZE["env_commands"])
+        _check(lib().rg_batch_apply_env_commands(self._h, a.ctypes.data, a.shape[0], _stream_ptr(stream)))
+
+    def apply_env_commands_device(self, device_commands_ptr: int, count: int, stream=None):
+        _check(lib().rg_batch_apply_env_commands_device(self._h, ctypes.c_void_p(device_commands_ptr), count,
+                                                        _stream_ptr(stream)))
+
+    def step_with_device_commands(self, device_controls_ptr: int, control_stride: int, device_commands_ptr: int,
+                                  command_stride: int, ticks: int, stream=None):
+        """Per tick t, env e's command device_commands[t * command_stride + e] (EnvCommand records), then the tick;
+        the last tick's controls become the batch's controls."""
+        _check(lib().rg_batch_step_with_device_commands(self._h, ctypes.c_void_p(device_controls_ptr), control_stride,
+                                                        ctypes.c_void_p(device_commands_ptr), command_stride, ticks,
+                                                        _stream_ptr(stream)))
 
     def step(self, ticks: int = 1, stream=None):
         _check(lib().rg_batch_step(self._h, ticks, _stream_ptr(stream)))
@@ -244,19 +313,32 @@ class BatchSim:
         """Per-env body digests (uint64) into a device buffer of env_count entries."""
         _check(lib().rg_batch_env_digests(self._h, ctypes.c_void_p(device_ptr), _stream_ptr(stream)))
 
-    def device_arrays(self):
-        """Zero-copy torch views of the state arrays as raw bytes ([n, itemsize] uint8 tensors on cuda:0)."""
+    def device_arrays(self) -> dict:
+        """device_arrays() (python/rocketgpu_bindings.cpp:846-898): per state array, a dict with a zero-copy
+        `__cuda_array_interface__` (v3, one opaque record per element), itemsize, count, per_env and what one
+        element is.  Records are the byte-compatible ones of state_layouts()."""
+        view = self.device_view()
+        cars = self.max_cars_per_env
+        pads = self.boost_pad_capacity // self.env_count if self.env_count else 0
+        spec = [("car_states", cars, "car slot"), ("controls", cars, "car slot"), ("car_configs", cars, "car slot"),
+                ("teams", cars, "car slot"), ("ball_states", 1, "env"), ("env_states", 1, "env"),
+                ("boost_pad_states", pads, "boost pad"), ("env_car_count", 1, "env")]
+        out = {}
+        for name, per_env, element in spec:
+            count = self._count(name)
+            out[name] = {
+                "__cuda_array_interface__": {"data": (int(view[name] or 0), False), "shape": (count,),
+                                             "typestr": f"|V{ITEMSIZE[name]}", "strides": None, "version": 3},
+                "itemsize": ITEMSIZE[name], "count": count, "per_env": per_env, "element": element}
+        return out
+
+    def torch_arrays(self) -> dict:
+        """Zero-copy torch views of the state arrays as raw bytes ([count, itemsize] uint8 tensors on cuda:0)."""
         import torch
 
-        view = self.device_view()
-        out = {}
-        for name in ("ball_states", "env_states", "car_states", "controls"):
-            ptr = view[name]
-            if not ptr:
-                continue
-            n = self._count(name)
-            out[name] = _torch_from_ptr(ptr, (n, ITEMSIZE[name]), torch.uint8)
-        return out
+        return {name: _torch_from_ptr(e["__cuda_array_interface__"]["data"][0], (e["count"], e["itemsize"]),
+                                      torch.uint8)
+                for name, e in self.device_arrays().items() if e["count"] and e["__cuda_array_interface__"]["data"][0]}
 
 
 def _torch_from_ptr(ptr: int, shape, dtype):
@@ -268,3 +350,7 @@ def _torch_from_ptr(ptr: int, shape, dtype):
                                              "version": 3, "strides": None}
 
     return torch.as_tensor(_Holder(), device="cuda")
+
+
+from .views import (DeviceArray, DeviceFieldArray, device_arrays, field_arrays, host_struct_array,  # noqa: E402
+                    state_dtype, state_layout)

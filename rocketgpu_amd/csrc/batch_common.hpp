@@ -35,6 +35,10 @@ inline core::StepParams DefaultStepParams(const rgw::ArenaRulesSpec& rules) {
         p.goal_min[t] = Vec3(g.min.x, g.min.y, g.min.z);
         p.goal_max[t] = Vec3(g.max.x, g.max.y, g.max.z);
     }
+    p.ball_spawn = Vec3(rules.ball_spawn.x, rules.ball_spawn.y, rules.ball_spawn.z);
+    p.pad_count = rules.pad_count;
+    p.pad_big_mask = 0;
+    for (int k = 0; k < rules.pad_count; ++k) p.pad_big_mask |= static_cast<uint64_t>(rules.pad_big[k] != 0) << k;
     return p;
 }
 

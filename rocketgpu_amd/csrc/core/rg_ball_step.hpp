@@ -39,6 +39,9 @@ struct StepParams {
     uint8_t enable_scoring;
     uint8_t pad[3];
     Vec3 goal_point[2], goal_normal[2], goal_min[2], goal_max[2];
+    Vec3 ball_spawn;        // kickoff ball position (uu), from the arena's rules
+    int32_t pad_count;      // boost pads per env
+    uint64_t pad_big_mask;  // bit k: pad k is a big pad
 };
 
 struct BallManifoldPoint {
@@ -954,6 +957,26 @@ RG_HD void SolveBallSplit(BallState& ball, BallTickScratch& t, float dt) {
 }
 
 // One tick of an arena holding only the ball.
+// The round countdown on render-frame time (CountdownUpdateTimers, env_step.hpp:6516-6538): phase 1 waits one
+// second, phase 2 fires three times a second apart, then the countdown ends and the ball's 2D constraint lifts.
+RG_HD void CountdownUpdateTimers(EnvState& env, BallState& ball, float frame_dt) {
+    if (env.countdown_phase == 0) return;
+    env.countdown_count += frame_dt;
+    if (env.countdown_count > 1.0f) {
+        if (env.countdown_phase == 1) {
+            env.countdown_phase = 2;
+            env.countdown_count = 0.0f;
+        } else {
+            env.countdown_count -= 1.0f;
+            if (++env.countdown_fires == 3) {
+                env.countdown_phase = 0;
+                env.countdown_fires = 0;
+                ball.constrained_2d = 0;
+            }
+        }
+    }
+}
+
 RG_HD void StepBallArena(BallState& ball, EnvState& env, ArenaInternal& st, const ArenaMeshView& mesh,
                          const StepParams& p, BallTickScratch& t) {
     const float dt = p.tick_dt;
@@ -1056,8 +1079,11 @@ RG_HD void StepBallArena(BallState& ball, EnvState& env, ArenaInternal& st, cons
     }
     env.frame_ticks += 1;
     if (p.frame_frac >= 0.0f) {
+        // The render frame ends on this tick: the countdown timers advance by its DeltaSeconds.
+        const float frame_dt = (static_cast<float>(env.frame_ticks) + p.frame_frac - env.frame_frac_prev) * p.tick_dt;
         env.frame_ticks = 0;
         env.frame_frac_prev = p.frame_frac;
+        CountdownUpdateTimers(env, ball, frame_dt);
     }
     env.tick += 1;
 }

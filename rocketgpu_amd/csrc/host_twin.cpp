@@ -5,10 +5,13 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <new>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
 #include "batch_common.hpp"
+#include "core/rg_env_commands.hpp"
 
 using namespace rg;
 
@@ -21,7 +24,48 @@ struct Twin {
     std::vector<BallState> balls;
     std::vector<EnvState> envs;
     std::vector<core::ArenaInternal> internal;
+    std::vector<BoostPadState> pads;
+    bool kickoff_settled = true;
+    bool settled_ready = false;
+    BallState settled{};
 };
+
+BoostPadState* EnvPads(Twin& t, size_t e) {
+    return t.pads.empty() ? nullptr : t.pads.data() + e * static_cast<size_t>(t.params.pad_count);
+}
+
+// The settled kickoff ball, by the same procedure as BatchSim::EnsureSettledBall, on the host core.
+const BallState* Settled(Twin& t) {
+    if (!t.kickoff_settled) return nullptr;
+    if (!t.settled_ready) {
+        BallState ball;
+        EnvState env;
+        std::memset(static_cast<void*>(&ball), 0, sizeof(ball));
+        std::memset(static_cast<void*>(&env), 0, sizeof(env));
+        new (&ball) BallState;
+        new (&env) EnvState;
+        auto st = std::make_unique<core::ArenaInternal>();
+        std::memset(static_cast<void*>(st.get()), 0, sizeof(core::ArenaInternal));
+        core::ResetToKickoffNoCars(ball, env, *st, nullptr, 0, 0, nullptr, t.params);
+        EnvCommand c{};
+        c.reset_to_kickoff = 1;
+        c.reset_scoreboard = 1;
+        c.kickoff_seed_mode = 3;
+        c.kickoff_countdown_ticks = core::kKickoffCountdownGameTimers;
+        core::ApplyEnvCommandNoCars(ball, env, *st, nullptr, 0, c, nullptr, t.params);
+        auto scratch = std::make_unique<core::BallTickScratch>();
+        int ticks = core::kKickoffCountdownTicks;
+        for (int k = 0;; ++k) {
+            for (int i = 0; i < ticks; ++i) core::StepBallArena(ball, env, *st, t.view, t.params, *scratch);
+            if (env.countdown_phase == 0) break;
+            if (k == 16) throw std::runtime_error("kickoff_settled: the countdown did not end");
+            ticks = 1;
+        }
+        t.settled = ball;
+        t.settled_ready = true;
+    }
+    return &t.settled;
+}
 
 thread_local std::string g_err;
 
@@ -48,6 +92,8 @@ void* rgt_create(int env_count, double scale, double fillet_radius) {
         t->envs.assign(static_cast<size_t>(env_count), EnvState{});
         t->internal.assign(static_cast<size_t>(env_count), core::ArenaInternal{});
         std::memset(t->internal.data(), 0, t->internal.size() * sizeof(core::ArenaInternal));
+        t->pads.assign(static_cast<size_t>(env_count) * static_cast<size_t>(t->params.pad_count), BoostPadState{});
+        for (size_t e = 0; e < static_cast<size_t>(env_count); ++e) core::ResetBoostPads(EnvPads(*t, e), t->params);
         return t.release();
     } catch (const std::exception& e) {
         g_err = e.what();
@@ -119,6 +165,10 @@ int rgt_upload(void* p, int which, const void* src, long n) {
         std::memcpy(t.envs.data(), src, static_cast<size_t>(n) * sizeof(EnvState));
         return 0;
     }
+    if (which == 3) {
+        std::memcpy(t.pads.data(), src, static_cast<size_t>(n) * sizeof(BoostPadState));
+        return 0;
+    }
     return -1;
 }
 
@@ -132,7 +182,64 @@ int rgt_download(void* p, int which, void* dst, long n) {
         std::memcpy(dst, t.envs.data(), static_cast<size_t>(n) * sizeof(EnvState));
         return 0;
     }
+    if (which == 3) {
+        std::memcpy(dst, t.pads.data(), static_cast<size_t>(n) * sizeof(BoostPadState));
+        return 0;
+    }
     return -1;
+}
+
+void rgt_set_kickoff_settled(void* p, int on) { static_cast<Twin*>(p)->kickoff_settled = on != 0; }
+
+int rgt_settled_ball(void* p, void* out) {
+    try {
+        const BallState* b = Settled(*static_cast<Twin*>(p));
+        if (!b) return -1;
+        std::memcpy(out, b, sizeof(BallState));
+        return 0;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return -1;
+    }
+}
+
+int rgt_reset_to_kickoff(void* p, uint64_t seed) {
+    Twin& t = *static_cast<Twin*>(p);
+    const BallState* settled = Settled(t);
+    for (size_t e = 0; e < t.balls.size(); ++e) {
+        core::ResetToKickoffNoCars(t.balls[e], t.envs[e], t.internal[e], EnvPads(t, e), static_cast<int>(e), seed,
+                                   settled, t.params);
+    }
+    return 0;
+}
+
+int rgt_apply_env_commands(void* p, const void* commands, long n) {
+    Twin& t = *static_cast<Twin*>(p);
+    const EnvCommand* c = static_cast<const EnvCommand*>(commands);
+    const BallState* settled = Settled(t);
+    for (long e = 0; e < n; ++e) {
+        const size_t i = static_cast<size_t>(e);
+        core::ApplyEnvCommandNoCars(t.balls[i], t.envs[i], t.internal[i], EnvPads(t, i), static_cast<int>(e), c[e],
+                                    settled, t.params);
+    }
+    return 0;
+}
+
+// step_with_device_commands on the host: commands[tick * stride + env] before each env's tick.
+int rgt_step_with_commands(void* p, const void* commands, int stride, int ticks) {
+    Twin& t = *static_cast<Twin*>(p);
+    const EnvCommand* c = static_cast<const EnvCommand*>(commands);
+    const BallState* settled = Settled(t);
+    auto scratch = std::make_unique<core::BallTickScratch>();
+    for (int k = 0; k < ticks; ++k) {
+        for (size_t e = 0; e < t.balls.size(); ++e) {
+            core::ApplyTickCommandNoCars(t.balls[e], t.envs[e], t.internal[e], EnvPads(t, e), static_cast<int>(e),
+                                         c[static_cast<size_t>(k) * static_cast<size_t>(stride) + e], settled,
+                                         t.params);
+            core::StepBallArena(t.balls[e], t.envs[e], t.internal[e], t.view, t.params, *scratch);
+        }
+    }
+    return 0;
 }
 
 int rgt_step(void* p, int ticks) {

@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "batch_common.hpp"
+#include "core/rg_env_commands.hpp"
 #include "rocketgpu_amd/rg_capi.h"
 
 namespace rg {
@@ -78,6 +79,15 @@ struct StepArgs {
     // Arenas per 64-lane wavefront (64, 32, 16, 8 or 4): fewer arenas per wave means more waves per CU to hide the
     // latency of the dependent BVH and state loads, and less divergence within a wave.
     int32_t arenas_per_wave;
+    // Env commands and kickoff resets: pads (pad_count per env), controls (max_cars per env, cleared by a reset),
+    // the settled kickoff ball (null: resets leave the ball at its spawn), and per-tick commands
+    // (commands[t * command_stride + env]) for step_with_device_commands.
+    BoostPadState* pads;
+    CarControls* controls;
+    int32_t max_cars;
+    const BallState* settled;
+    const EnvCommand* commands;
+    int32_t command_stride;
 };
 
 constexpr int kMaxStepBlock = 256;
@@ -144,6 +154,50 @@ __global__ void __launch_bounds__(kMaxStepBlock) StepBallArenasKernel(StepArgs a
         a.envs[e] = sl.env;
         a.balls[e] = sl.ball;
     }
+}
+
+__device__ inline BoostPadState* EnvPads(const StepArgs& a, int e) {
+    return a.pads ? a.pads + static_cast<size_t>(e) * a.params.pad_count : nullptr;
+}
+
+__device__ inline void ClearEnvControls(const StepArgs& a, int e) {
+    for (int k = 0; k < a.max_cars; ++k) a.controls[static_cast<size_t>(e) * a.max_cars + k] = CarControls{};
+}
+
+// step_with_device_commands: each tick starts with the env's command for that tick (StepEnvPreTickEnv), then the
+// tick.  Records stay in place; the work mapping is the step kernel's.
+__global__ void __launch_bounds__(kMaxStepBlock) StepBallArenasCommandsKernel(StepArgs a, int32_t ticks) {
+    const int g = static_cast<int>(blockIdx.x) * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x);
+    const int lane = g & 63;
+    if (lane >= a.arenas_per_wave) return;
+    const int e = (g >> 6) * a.arenas_per_wave + lane;
+    if (e >= a.env_count) return;
+    core::BallTickScratch scratch;
+    BoostPadState* pads = EnvPads(a, e);
+    for (int t = 0; t < ticks; ++t) {
+        const EnvCommand c = a.commands[static_cast<size_t>(t) * a.command_stride + e];
+        core::ApplyTickCommandNoCars(a.balls[e], a.envs[e], a.internal[e], pads, e, c, a.settled, a.params);
+        if (c.reset_to_kickoff) ClearEnvControls(a, e);
+        core::StepBallArena(a.balls[e], a.envs[e], a.internal[e], a.mesh, a.params, scratch);
+    }
+}
+
+// apply_env_commands: command e to env e, for e < count.
+__global__ void __launch_bounds__(64) ApplyEnvCommandsKernel(StepArgs a, const EnvCommand* commands, int32_t count) {
+    const int e = static_cast<int>(blockIdx.x) * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x);
+    if (e >= count) return;
+    const EnvCommand c = commands[e];
+    if (!core::CommandActive(c)) return;
+    core::ApplyEnvCommandNoCars(a.balls[e], a.envs[e], a.internal[e], EnvPads(a, e), e, c, a.settled, a.params);
+    if (c.reset_to_kickoff) ClearEnvControls(a, e);
+}
+
+// reset_to_kickoff: every env to a new match's kickoff.
+__global__ void __launch_bounds__(64) ResetToKickoffKernel(StepArgs a, uint64_t seed) {
+    const int e = static_cast<int>(blockIdx.x) * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x);
+    if (e >= a.env_count) return;
+    core::ResetToKickoffNoCars(a.balls[e], a.envs[e], a.internal[e], EnvPads(a, e), e, seed, a.settled, a.params);
+    ClearEnvControls(a, e);
 }
 
 __device__ inline uint64_t Fnv(uint64_t h, const void* data, int n) {
@@ -244,23 +298,130 @@ public:
         car_count_.Resize(n);
         cars_.Resize(cap);
         controls_.Resize(cap);
+        teams_.Resize(cap);
+        configs_.Resize(cap);
+        pads_.Resize(n * static_cast<size_t>(params_.pad_count));
+        settled_.Resize(1);
         digests_.Resize(n);
-        const std::vector<BallState> balls(n);
-        const std::vector<EnvState> envs(n);
+        // Records start as the reference's constructor leaves them (src/rocketgpu.cpp:1111-1130): default-valued
+        // records with zeroed padding, Blue teams, default car configs; then a kickoff reset with seed 0 (:1281).
+        std::vector<BallState> balls(n);
+        std::vector<EnvState> envs(n);
+        ZeroedDefaults(balls);
+        ZeroedDefaults(envs);
         const std::vector<int32_t> counts(n, 0);
-        const std::vector<CarState> cars(cap);
-        const std::vector<CarControls> controls(cap);
+        std::vector<CarState> cars(cap);
+        std::vector<CarControls> controls(cap);
+        std::vector<CarConfig> configs(cap);
+        ZeroedDefaults(cars);
+        ZeroedDefaults(controls);
+        ZeroedDefaults(configs);
+        const std::vector<uint8_t> teams(cap, 0);
         balls_.Upload(balls.data(), n, nullptr);
         envs_.Upload(envs.data(), n, nullptr);
         car_count_.Upload(counts.data(), n, nullptr);
         cars_.Upload(cars.data(), cap, nullptr);
         controls_.Upload(controls.data(), cap, nullptr);
+        configs_.Upload(configs.data(), cap, nullptr);
+        teams_.Upload(teams.data(), cap, nullptr);
         Check(hipMemset(internal_.data(), 0, n * sizeof(core::ArenaInternal)), "hipMemset");
+        if (pads_.size()) Check(hipMemset(pads_.data(), 0, pads_.size() * sizeof(BoostPadState)), "hipMemset");
+        Check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+        ResetToKickoff(0, nullptr);
         Check(hipDeviceSynchronize(), "hipDeviceSynchronize");
     }
 
     int64_t env_count() const { return cfg_.env_count; }
     int64_t car_capacity() const { return static_cast<int64_t>(cfg_.env_count) * cfg_.max_cars_per_env; }
+    int64_t boost_pad_capacity() const { return static_cast<int64_t>(cfg_.env_count) * params_.pad_count; }
+
+    // set_teams / set_car_configs (src/rocketgpu.cpp:2934-2950): stored for the slots; arenas here carry no cars.
+    void SetTeams(const uint8_t* teams, size_t n, hipStream_t s) {
+        if (n > static_cast<size_t>(car_capacity())) throw std::out_of_range("set_teams count exceeds car_capacity");
+        if (n && !teams) throw InvalidArgument("host_teams must not be null");
+        teams_.Upload(teams, n, s);
+        Check(hipStreamSynchronize(s), "hipStreamSynchronize");
+    }
+    void SetCarConfigs(const CarConfig* configs, size_t n, hipStream_t s) {
+        if (n > static_cast<size_t>(car_capacity())) {
+            throw std::out_of_range("set_car_configs count exceeds car_capacity");
+        }
+        if (n && !configs) throw InvalidArgument("host_configs must not be null");
+        configs_.Upload(configs, n, s);
+        Check(hipStreamSynchronize(s), "hipStreamSynchronize");
+    }
+
+    // MutatorConfig::kickoff_settled (rocketgpu.h:827-832): resets without a countdown put the ball where the
+    // game's countdown leaves it.  On by default, as in the reference.
+    void SetKickoffSettled(bool on) { kickoff_settled_ = on; }
+
+    // reset_to_kickoff (src/rocketgpu.cpp:2085-2133): every env to a new match's kickoff.
+    void ResetToKickoff(uint64_t seed, hipStream_t s) {
+        EnsureSettledBall();
+        StepArgs a = BaseArgs();
+        const int grid = (cfg_.env_count + 63) / 64;
+        hipLaunchKernelGGL(ResetToKickoffKernel, dim3(grid), dim3(64), 0, s, a, seed);
+        Check(hipGetLastError(), "ResetToKickoffKernel launch");
+    }
+
+    // apply_env_commands (src/rocketgpu.cpp:2136-2218): command i to env i for i < count.  Same checks and messages.
+    void ApplyEnvCommands(const EnvCommand* host_commands, size_t count, hipStream_t s) {
+        if (host_commands == nullptr) throw InvalidArgument("host_commands must not be null");
+        if (count > static_cast<size_t>(cfg_.env_count)) {
+            throw std::out_of_range("apply_env_commands_host count exceeds env_count");
+        }
+        size_t active = 0;
+        bool settled_reset = false;
+        for (size_t i = 0; i < count; ++i) {
+            active += core::CommandActive(host_commands[i]) ? 1 : 0;
+            settled_reset = settled_reset || (host_commands[i].reset_to_kickoff && host_commands[i].kickoff_countdown_ticks == 0);
+        }
+        if (active == 0) return;
+        if (settled_reset) EnsureSettledBall();
+        if (commands_.size() < count) commands_.Resize(static_cast<size_t>(cfg_.env_count));
+        commands_.Upload(host_commands, count, s);
+        LaunchApplyCommands(commands_.data(), static_cast<int32_t>(count), s);
+        Check(hipStreamSynchronize(s), "hipStreamSynchronize");
+    }
+
+    // apply_env_commands_device (src/rocketgpu.cpp:2221-2248): the commands are read on the device.
+    void ApplyEnvCommandsDevice(const EnvCommand* device_commands, size_t count, hipStream_t s) {
+        if (device_commands == nullptr) throw InvalidArgument("device_commands must not be null");
+        if (count > static_cast<size_t>(cfg_.env_count)) {
+            throw std::out_of_range("apply_env_commands_device count exceeds env_count");
+        }
+        EnsureSettledBall();
+        if (count == 0) return;
+        LaunchApplyCommands(device_commands, static_cast<int32_t>(count), s);
+    }
+
+    // step_with_device_commands (src/rocketgpu.cpp:2554-2627): per tick, the env's command, then the tick; the
+    // last tick's controls become the batch's controls.  Same checks and messages as the reference.
+    void StepWithDeviceCommands(const CarControls* controls, int32_t control_stride, const EnvCommand* commands,
+                                int32_t command_stride, int32_t ticks, hipStream_t s) {
+        if (ticks <= 0) return;
+        if (controls == nullptr) throw InvalidArgument("device_controls_by_tick must not be null");
+        if (commands == nullptr) throw InvalidArgument("device_env_commands_by_tick must not be null");
+        EnsureSettledBall();
+        if (control_stride != 0 && control_stride < car_capacity()) {
+            throw InvalidArgument("control_stride must be zero or at least the total car capacity");
+        }
+        if (command_stride < cfg_.env_count) throw InvalidArgument("env_command_stride must be at least env_count");
+        StepArgs a = BaseArgs();
+        a.commands = commands;
+        a.command_stride = command_stride;
+        a.count = cfg_.env_count;
+        a.arenas_per_wave = per_wave_;
+        const int64_t threads = (static_cast<int64_t>(cfg_.env_count) + per_wave_ - 1) / per_wave_ * 64;
+        const int grid = static_cast<int>((threads + block_ - 1) / block_);
+        hipLaunchKernelGGL(StepBallArenasCommandsKernel, dim3(grid), dim3(block_), 0, s, a, ticks);
+        Check(hipGetLastError(), "StepBallArenasCommandsKernel launch");
+        if (car_capacity() > 0) {
+            Check(hipMemcpyAsync(controls_.data(), controls + static_cast<size_t>(ticks - 1) * control_stride,
+                                 static_cast<size_t>(car_capacity()) * sizeof(CarControls), hipMemcpyDeviceToDevice, s),
+                  "hipMemcpyAsync controls");
+        }
+    }
 
     void SetEnvCarCount(const int32_t* counts, size_t n, hipStream_t s) {
         if (n != static_cast<size_t>(cfg_.env_count)) throw InvalidArgument("set_env_car_count needs env_count values");
@@ -302,6 +463,11 @@ public:
             cars_.Upload(static_cast<const CarState*>(src), n, s);
             Check(hipStreamSynchronize(s), "hipStreamSynchronize");
             break;
+        case RG_ARRAY_BOOST_PAD_STATES:
+            Expect(n, pads_.size(), "boost pad states");
+            pads_.Upload(static_cast<const BoostPadState*>(src), n, s);
+            Check(hipStreamSynchronize(s), "hipStreamSynchronize");
+            break;
         default:
             throw InvalidArgument("unknown or read-only state array id");
         }
@@ -313,6 +479,7 @@ public:
         case RG_ARRAY_ENV_STATES: envs_.Download(static_cast<EnvState*>(dst), n, s); break;
         case RG_ARRAY_CAR_STATES: cars_.Download(static_cast<CarState*>(dst), n, s); break;
         case RG_ARRAY_CONTROLS: controls_.Download(static_cast<CarControls*>(dst), n, s); break;
+        case RG_ARRAY_BOOST_PAD_STATES: pads_.Download(static_cast<BoostPadState*>(dst), n, s); break;
         case RG_ARRAY_ENV_CAR_COUNT: car_count_.Download(static_cast<int32_t*>(dst), n, s); break;
         default: throw InvalidArgument("unknown state array id");
         }
@@ -422,6 +589,9 @@ public:
         v->ball_states = balls_.data();
         v->env_states = envs_.data();
         v->env_car_count = car_count_.data();
+        v->boost_pad_states = pads_.data();
+        v->teams = teams_.data();
+        v->car_configs = configs_.data();
     }
 
     // ball_path (rocketgpu.h:1069-1073): device out [env][samples][6] and result [env][2].  The arenas must hold
@@ -429,18 +599,7 @@ public:
     void BallPath(int32_t samples, int32_t ticks, float goal_y, float* out, int32_t* result, hipStream_t s) {
         if (samples <= 0 || ticks <= 0) return;
         if (!out || !result) throw InvalidArgument("ball_path output pointers must not be null");
-        StepArgs a{};
-        a.balls = balls_.data();
-        a.envs = envs_.data();
-        a.internal = internal_.data();
-        a.env_car_count = car_count_.data();
-        a.mesh.tris = tris_.data();
-        a.mesh.nodes = nodes_.data();
-        a.mesh.meshes = infos_.data();
-        a.mesh.mesh_count = static_cast<int32_t>(mesh_host_.meshes.size());
-        a.mesh.tri_count = static_cast<int32_t>(mesh_host_.tris.size());
-        a.params = params_;
-        a.env_count = cfg_.env_count;
+        const StepArgs a = BaseArgs();
         const int grid = (cfg_.env_count + 63) / 64;
         hipLaunchKernelGGL(BallPathKernel, dim3(grid), dim3(64), 0, s, a, samples, ticks, goal_y, out, result);
         Check(hipGetLastError(), "BallPathKernel launch");
@@ -460,7 +619,14 @@ private:
         graph_exec_ = nullptr;
     }
 
-    void Launch(const int32_t* ids, const int32_t* ticks_list, int32_t count, int32_t ticks, hipStream_t s) {
+    template <typename T>
+    static void ZeroedDefaults(std::vector<T>& v) {
+        std::memset(static_cast<void*>(v.data()), 0, v.size() * sizeof(T));
+        for (T& x : v) new (&x) T;
+    }
+
+    // The batch's own arrays and geometry; work-list fields are set by the caller.
+    StepArgs BaseArgs() const {
         StepArgs a{};
         a.balls = balls_.data();
         a.envs = envs_.data();
@@ -473,6 +639,68 @@ private:
         a.mesh.tri_count = static_cast<int32_t>(mesh_host_.tris.size());
         a.params = params_;
         a.env_count = cfg_.env_count;
+        a.pads = pads_.data();
+        a.controls = controls_.data();
+        a.max_cars = cfg_.max_cars_per_env;
+        a.settled = kickoff_settled_ && settled_ready_ ? settled_.data() : nullptr;
+        return a;
+    }
+
+    void LaunchApplyCommands(const EnvCommand* commands, int32_t count, hipStream_t s) {
+        const StepArgs a = BaseArgs();
+        const int grid = (count + 63) / 64;
+        hipLaunchKernelGGL(ApplyEnvCommandsKernel, dim3(grid), dim3(64), 0, s, a, commands, count);
+        Check(hipGetLastError(), "ApplyEnvCommandsKernel launch");
+    }
+
+    // The settled kickoff ball (the reference's record_kickoff_settled, src/rocketgpu.cpp:1538-1612): a fresh
+    // arena gets a new-match reset with the game's countdown (the ball re-created 7.737 uu above its spawn, under
+    // the 2D constraint), then kKickoffCountdownTicks ticks and one more at a time until the countdown has ended.
+    // Its ball is what a settled kickoff reset copies.  Computed once per batch, on this GPU, by the step kernel.
+    void EnsureSettledBall() {
+        if (!kickoff_settled_ || settled_ready_) return;
+        std::vector<BallState> ball(1);
+        std::vector<EnvState> env(1);
+        ZeroedDefaults(ball);
+        ZeroedDefaults(env);
+        std::vector<core::ArenaInternal> st(1);
+        std::memset(static_cast<void*>(st.data()), 0, sizeof(core::ArenaInternal));
+        core::ResetToKickoffNoCars(ball[0], env[0], st[0], nullptr, 0, 0, nullptr, params_);
+        EnvCommand c{};
+        c.reset_to_kickoff = 1;
+        c.reset_scoreboard = 1;
+        c.kickoff_seed_mode = 3;
+        c.kickoff_countdown_ticks = core::kKickoffCountdownGameTimers;
+        core::ApplyEnvCommandNoCars(ball[0], env[0], st[0], nullptr, 0, c, nullptr, params_);
+        DeviceArray<EnvState> d_env;
+        DeviceArray<core::ArenaInternal> d_st;
+        d_env.Resize(1);
+        d_st.Resize(1);
+        settled_.Upload(ball.data(), 1, nullptr);
+        d_env.Upload(env.data(), 1, nullptr);
+        d_st.Upload(st.data(), 1, nullptr);
+        StepArgs a = BaseArgs();
+        a.balls = settled_.data();
+        a.envs = d_env.data();
+        a.internal = d_st.data();
+        a.env_count = 1;
+        a.count = 1;
+        a.arenas_per_wave = 64;
+        int32_t ticks = core::kKickoffCountdownTicks;
+        for (int k = 0;; ++k) {
+            hipLaunchKernelGGL(StepBallArenasKernel<kStageInPlace>, dim3(1), dim3(64), 0, nullptr, a, ticks);
+            Check(hipGetLastError(), "StepBallArenasKernel launch");
+            d_env.Download(env.data(), 1, nullptr);
+            Check(hipStreamSynchronize(nullptr), "hipStreamSynchronize");
+            if (env[0].countdown_phase == 0) break;
+            if (k == 16) throw std::runtime_error("kickoff_settled: the countdown did not end");
+            ticks = 1;
+        }
+        settled_ready_ = true;
+    }
+
+    void Launch(const int32_t* ids, const int32_t* ticks_list, int32_t count, int32_t ticks, hipStream_t s) {
+        StepArgs a = BaseArgs();
         a.arena_ids = ids;
         a.arena_ticks = ticks_list;
         a.count = count;
@@ -513,6 +741,13 @@ private:
     DeviceArray<int32_t> car_count_;
     DeviceArray<CarState> cars_;
     DeviceArray<CarControls> controls_;
+    DeviceArray<uint8_t> teams_;
+    DeviceArray<CarConfig> configs_;
+    DeviceArray<BoostPadState> pads_;
+    DeviceArray<EnvCommand> commands_;
+    DeviceArray<BallState> settled_;
+    bool kickoff_settled_ = true;
+    bool settled_ready_ = false;
     DeviceArray<uint64_t> digests_;
     int block_ = 64;
     int stage_ = kStageInPlace;
@@ -562,6 +797,26 @@ rgw::ArenaRulesSpec RulesFromLayout(const rg_arena_layout& l) {
         r.goals[t].max = rgw::V3{g[9], g[10], g[11]};
         const float len = std::sqrt(g[3] * g[3] + g[4] * g[4] + g[5] * g[5]);
         if (!(len > 0.0f) || !std::isfinite(len)) throw rg::InvalidArgument("ArenaGoal normal must be nonzero");
+    }
+    for (int t = 0; t < 2; ++t) {
+        for (int k = 0; k < 5; ++k) {
+            const float* q = l.kickoff_spawns[t][k];
+            r.kickoff[t][k] = rgw::SpawnSpot{q[0], q[1], q[2], q[3]};
+        }
+        for (int k = 0; k < 4; ++k) {
+            const float* q = l.respawn_spawns[t][k];
+            r.respawn[t][k] = rgw::SpawnSpot{q[0], q[1], q[2], q[3]};
+        }
+    }
+    r.ball_spawn = rgw::V3{l.ball_spawn[0], l.ball_spawn[1], l.ball_spawn[2]};
+    if (l.pad_count < 0 || l.pad_count > rgw::kMaxPads) {
+        throw rg::InvalidArgument("ArenaLayout::boost_pads holds at most kMaxBoostPads pads");
+    }
+    if (l.pad_count > 0 && (!l.pad_positions || !l.pad_is_big)) throw rg::InvalidArgument("null pad arrays");
+    r.pad_count = l.pad_count;
+    for (int k = 0; k < l.pad_count; ++k) {
+        r.pad_pos[k] = rgw::V3{l.pad_positions[3 * k], l.pad_positions[3 * k + 1], l.pad_positions[3 * k + 2]};
+        r.pad_big[k] = l.pad_is_big[k] ? 1 : 0;
     }
     return r;
 }
@@ -638,6 +893,46 @@ int rg_batch_step(rg_batch* b, int32_t ticks, void* stream) {
 int rg_batch_step_with_device_controls(rg_batch* b, const void* controls, int32_t ticks, int32_t stride,
                                        void* stream) {
     return Guard([&] { b->sim->StepWithDeviceControls(controls, ticks, stride, S(stream)); });
+}
+
+int64_t rg_batch_boost_pad_capacity(const rg_batch* b) { return b ? b->sim->boost_pad_capacity() : -1; }
+
+int rg_batch_set_teams(rg_batch* b, const uint8_t* host_teams, size_t count, void* stream) {
+    return Guard([&] { b->sim->SetTeams(host_teams, count, S(stream)); });
+}
+
+int rg_batch_set_car_configs(rg_batch* b, const void* host_configs, size_t count, void* stream) {
+    return Guard([&] { b->sim->SetCarConfigs(static_cast<const rg::CarConfig*>(host_configs), count, S(stream)); });
+}
+
+int rg_batch_set_kickoff_settled(rg_batch* b, int32_t on) {
+    return Guard([&] { b->sim->SetKickoffSettled(on != 0); });
+}
+
+int rg_batch_reset_to_kickoff(rg_batch* b, uint64_t seed, void* stream) {
+    return Guard([&] { b->sim->ResetToKickoff(seed, S(stream)); });
+}
+
+int rg_batch_apply_env_commands(rg_batch* b, const void* host_commands, size_t count, void* stream) {
+    return Guard([&] {
+        b->sim->ApplyEnvCommands(static_cast<const rg::EnvCommand*>(host_commands), count, S(stream));
+    });
+}
+
+int rg_batch_apply_env_commands_device(rg_batch* b, const void* device_commands, size_t count, void* stream) {
+    return Guard([&] {
+        b->sim->ApplyEnvCommandsDevice(static_cast<const rg::EnvCommand*>(device_commands), count, S(stream));
+    });
+}
+
+int rg_batch_step_with_device_commands(rg_batch* b, const void* device_controls_by_tick, int32_t control_stride,
+                                       const void* device_env_commands_by_tick, int32_t env_command_stride,
+                                       int32_t ticks, void* stream) {
+    return Guard([&] {
+        b->sim->StepWithDeviceCommands(static_cast<const rg::CarControls*>(device_controls_by_tick), control_stride,
+                                       static_cast<const rg::EnvCommand*>(device_env_commands_by_tick),
+                                       env_command_stride, ticks, S(stream));
+    });
 }
 
 int rg_batch_step_arenas(rg_batch* b, const int32_t* device_arena_ids, const int32_t* device_ticks, int32_t count,
@@ -721,7 +1016,16 @@ int64_t rg_state_layouts_json(char* buf, size_t capacity) {
         first_field = false;
     };
 #define RG_F(T, f, type, n) field(#f, type, offsetof(T, f), n)
+#define RG_SUB(T, m, S, f, type) field(#m "." #f, type, offsetof(T, m) + offsetof(T::S, f), 0)
     begin("ball_states", sizeof(rg::BallState), true);
+    RG_SUB(rg::BallState, hs_info, HeatseekerInfo, y_target_dir, "<f4");
+    RG_SUB(rg::BallState, hs_info, HeatseekerInfo, cur_target_speed, "<f4");
+    RG_SUB(rg::BallState, hs_info, HeatseekerInfo, time_since_hit, "<f4");
+    RG_SUB(rg::BallState, ds_info, DropshotInfo, charge_level, "<i4");
+    RG_SUB(rg::BallState, ds_info, DropshotInfo, accumulated_hit_force, "<f4");
+    RG_SUB(rg::BallState, ds_info, DropshotInfo, y_target_dir, "<f4");
+    RG_SUB(rg::BallState, ds_info, DropshotInfo, has_damaged, "|u1");
+    RG_SUB(rg::BallState, ds_info, DropshotInfo, last_damage_tick, "<u8");
     RG_F(rg::BallState, pos, "<f4", 3);
     RG_F(rg::BallState, rot, "<f4", 4);
     RG_F(rg::BallState, vel, "<f4", 3);
@@ -734,6 +1038,8 @@ int64_t rg_state_layouts_json(char* buf, size_t capacity) {
     RG_F(rg::BallState, basis_forward, "<f4", 3);
     RG_F(rg::BallState, basis_right, "<f4", 3);
     RG_F(rg::BallState, basis_up, "<f4", 3);
+    RG_F(rg::BallState, basis_authoritative, "<u4", 0);
+    RG_F(rg::BallState, vel_impulse_cache_bt, "<f4", 3);
     RG_F(rg::BallState, sleeping, "<u4", 0);
     RG_F(rg::BallState, deactivation_time, "<f4", 0);
     RG_F(rg::BallState, constrained_2d, "<u2", 0);
@@ -776,6 +1082,29 @@ int64_t rg_state_layouts_json(char* buf, size_t capacity) {
     RG_F(rg::CarControls, jump, "|u1", 0);
     RG_F(rg::CarControls, boost, "|u1", 0);
     RG_F(rg::CarControls, handbrake, "|u1", 0);
+    begin("car_configs", sizeof(rg::CarConfig), false);
+    RG_F(rg::CarConfig, hitbox_size, "<f4", 3);
+    RG_F(rg::CarConfig, hitbox_offset, "<f4", 3);
+    RG_F(rg::CarConfig, front_wheels_offset, "<f4", 3);
+    RG_F(rg::CarConfig, back_wheels_offset, "<f4", 3);
+    RG_F(rg::CarConfig, front_wheel_radius, "<f4", 0);
+    RG_F(rg::CarConfig, back_wheel_radius, "<f4", 0);
+    RG_F(rg::CarConfig, front_suspension_rest, "<f4", 0);
+    RG_F(rg::CarConfig, back_suspension_rest, "<f4", 0);
+    RG_F(rg::CarConfig, dodge_deadzone, "<f4", 0);
+    RG_F(rg::CarConfig, three_wheels, "|u1", 0);
+    begin("teams", 1, false);
+    field("team", "|u1", 0, 0);
+    begin("env_car_count", sizeof(int32_t), false);
+    field("count", "<i4", 0, 0);
+    begin("boost_pad_states", sizeof(rg::BoostPadState), false);
+    RG_F(rg::BoostPadState, cooldown, "<f4", 0);
+    RG_F(rg::BoostPadState, is_active, "|u1", 0);
+    RG_F(rg::BoostPadState, is_big, "|u1", 0);
+    RG_F(rg::BoostPadState, pickup_frac_q, "<u2", 0);
+    RG_F(rg::BoostPadState, prev_locked_car_id, "<u4", 0);
+    RG_F(rg::BoostPadState, cur_locked_car_id, "<u4", 0);
+#undef RG_SUB
 #undef RG_F
     j += "]}}";
     if (buf && capacity > 0) {

@@ -47,6 +47,12 @@ def twin():
     T.rgt_bvh_containment_violations.restype = ctypes.c_long
     T.rgt_ball_path.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_float, vp, vp]
     T.rgt_mesh_tables.argtypes = [vp, vp, vp, vp, vp, vp]
+    T.rgt_set_kickoff_settled.argtypes = [vp, ctypes.c_int]
+    T.rgt_set_kickoff_settled.restype = None
+    T.rgt_settled_ball.argtypes = [vp, vp]
+    T.rgt_reset_to_kickoff.argtypes = [vp, ctypes.c_uint64]
+    T.rgt_apply_env_commands.argtypes = [vp, vp, ctypes.c_long]
+    T.rgt_step_with_commands.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int]
     return T
 
 

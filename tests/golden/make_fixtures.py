@@ -83,10 +83,103 @@ def make_ball_path(name, n, samples, ticks, goal_y, seed, scale, fillet):
     print(f"{name}: {int((res[:, 0] > 0).sum())} of {n} reach a goal -> {os.path.getsize(path)} bytes")
 
 
+# EnvCommand (rocketgpu.h:683-707), 16 bytes.
+COMMAND = np.dtype([("reset_to_kickoff", "u1"), ("reset_scoreboard", "u1"), ("reset_tick_count", "u1"),
+                    ("kickoff_seed_mode", "u1"), ("new_round_bodies", "u1"), ("keep_boost_pads", "u1"),
+                    ("kickoff_countdown_ticks", "<u2"), ("kickoff_seed", "<u8")])
+
+# One command of each kind the env-command path distinguishes.
+COMMAND_VARIANTS = [
+    {}, {"reset_to_kickoff": 1}, {"reset_to_kickoff": 1, "reset_scoreboard": 1},
+    {"reset_to_kickoff": 1, "kickoff_seed": 12345}, {"reset_to_kickoff": 1, "kickoff_seed_mode": 1, "kickoff_seed": 99},
+    {"reset_to_kickoff": 1, "kickoff_seed_mode": 2, "kickoff_seed": 5}, {"reset_to_kickoff": 1, "kickoff_seed_mode": 3},
+    {"reset_to_kickoff": 1, "keep_boost_pads": 1}, {"reset_to_kickoff": 1, "kickoff_countdown_ticks": 0xFFFF},
+    {"reset_to_kickoff": 1, "kickoff_countdown_ticks": 30, "reset_tick_count": 1}, {"new_round_bodies": 1},
+    {"reset_scoreboard": 1, "reset_tick_count": 1},
+]
+
+ENV_COMMAND_FIXTURES = {
+    # name: (arenas, seed, scale, fillet, stream ticks)
+    "env_commands_48": (48, 77, 1.0, 256.0, 240),
+}
+
+# The phases of an env-command fixture, in order; states are recorded after each.
+ENV_COMMAND_PHASES = ["step 300", "apply_env_commands", "step 700", "apply_env_commands_device", "step 100",
+                      "step_with_device_commands", "reset_to_kickoff 777", "step 50"]
+
+
+def env_command_inputs(n, seed, stream_ticks, pads0):
+    """Seeded inputs of an env-command fixture: pads with random cooldowns, one command per env (cycling the
+    variants), a sparse per-tick command stream and a per-tick controls stream (2 car slots per env)."""
+    rng = np.random.default_rng(seed)
+    pads = pads0.copy()
+    pads[:, 4] = rng.integers(0, 2, pads.shape[0])
+    pads[:, 0:4] = np.frombuffer(rng.uniform(0, 5, pads.shape[0]).astype(np.float32).tobytes(), np.uint8).reshape(-1, 4)
+    cmds = np.zeros(n, COMMAND)
+    for i in range(n):
+        for k, v in COMMAND_VARIANTS[i % len(COMMAND_VARIANTS)].items():
+            cmds[i][k] = v
+    stream = np.zeros(stream_ticks * n, COMMAND)
+    for t in range(stream_ticks):
+        for e in rng.choice(n, 2, replace=False):
+            for k, v in COMMAND_VARIANTS[rng.integers(len(COMMAND_VARIANTS))].items():
+                stream[t * n + e][k] = v
+            stream[t * n + e]["kickoff_seed"] = int(rng.integers(0, 1 << 40)) if rng.random() < 0.5 else 0
+    # Only the last tick's controls survive a command-stream step (they become the batch's controls).
+    controls = np.zeros((stream_ticks * n * 2, 24), np.uint8)
+    controls[-n * 2:] = rng.integers(0, 256, (n * 2, 24), dtype=np.uint8)
+    return pads, cmds.view(np.uint8).reshape(n, 16), stream.view(np.uint8).reshape(-1, 16), controls
+
+
+def env_command_run(n, seed, scale, fillet, stream_ticks):
+    """The env-command path on ball arenas in the oracle: apply_env_commands (host and device forms),
+    step_with_device_commands and reset_to_kickoff between stepping phases; every byte of the balls, envs and pads
+    after each phase, and the states a fresh batch starts with."""
+    s = R.RefSim(n, 2, scale, fillet)
+    s.set_env_car_count(np.zeros(n, np.int32))
+    fresh = {"ball": s.download(R.BALL, n), "env": s.download(R.ENV, n), "pads": s.download(R.PAD, n * 34)}
+    init_env = np.tile(R.default_state(R.ENV), (n, 1))
+    init_ball = R.workload_ball_states(seed, 0, n, scale)
+    init_pads, cmds, stream, controls = env_command_inputs(n, seed, stream_ticks, fresh["pads"])
+    s.upload(R.ENV, init_env)
+    s.upload(R.BALL, init_ball)
+    s.upload(R.PAD, init_pads)
+    balls, envs, pads = [], [], []
+    for phase in ENV_COMMAND_PHASES:
+        if phase.startswith("step ") and phase[5:].isdigit():
+            s.step(int(phase[5:]))
+        elif phase == "apply_env_commands":
+            s.apply_env_commands(cmds)
+        elif phase == "apply_env_commands_device":
+            s.apply_env_commands_device(cmds)
+        elif phase == "step_with_device_commands":
+            s.step_with_commands(controls, 2 * n, stream, n, stream_ticks)
+        else:
+            s.reset_to_kickoff(int(phase.split()[-1]))
+        balls.append(s.download(R.BALL, n))
+        envs.append(s.download(R.ENV, n))
+        pads.append(s.download(R.PAD, n * 34))
+    return dict(init_ball=init_ball, init_env=init_env, init_pads=init_pads, commands=cmds, command_stream=stream,
+                control_stream=controls, fresh_ball=fresh["ball"], fresh_env=fresh["env"], fresh_pads=fresh["pads"],
+                ball=np.stack(balls), env=np.stack(envs), pads=np.stack(pads))
+
+
+def make_env_commands(name, n, seed, scale, fillet, stream_ticks):
+    run = env_command_run(n, seed, scale, fillet, stream_ticks)
+    meta = {"workload": "env commands on ball arenas (seeded ball launches, no cars)", "arenas": n, "seed": seed,
+            "scale": scale, "fillet_radius": fillet, "stream_ticks": stream_ticks, "phases": ENV_COMMAND_PHASES,
+            "generator": "oracle/_ref reference CPU build"}
+    path = os.path.join(HERE, name + ".npz")
+    np.savez_compressed(path, **run, meta=np.frombuffer(json.dumps(meta).encode(), np.uint8))
+    print(f"{name}: {len(ENV_COMMAND_PHASES)} phases -> {os.path.getsize(path)} bytes")
+
+
 def main(names):
-    for name in names or list(FIXTURES) + list(BALL_PATH_FIXTURES):
+    for name in names or list(FIXTURES) + list(BALL_PATH_FIXTURES) + list(ENV_COMMAND_FIXTURES):
         if name in FIXTURES:
             make(name, *FIXTURES[name])
+        elif name in ENV_COMMAND_FIXTURES:
+            make_env_commands(name, *ENV_COMMAND_FIXTURES[name])
         else:
             make_ball_path(name, *BALL_PATH_FIXTURES[name])
 

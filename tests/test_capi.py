@@ -49,15 +49,20 @@ def test_layouts_match_reference_offsets():
     assert lay["env_states"]["itemsize"] == R.sizeof(R.ENV) == 44
     assert lay["car_states"]["itemsize"] == R.sizeof(R.CAR) == 880
     assert lay["controls"]["itemsize"] == R.sizeof(R.CONTROLS) == 24
-    which = {"ball_states": R.BALL, "env_states": R.ENV, "car_states": R.CAR}
+    assert lay["boost_pad_states"]["itemsize"] == R.sizeof(R.PAD) == 16
+    assert lay["car_configs"]["itemsize"] == R.sizeof(5) == 120
+    which = {"ball_states": R.BALL, "env_states": R.ENV, "car_states": R.CAR, "boost_pad_states": R.PAD,
+             "controls": 4, "car_configs": 5}
     checked = 0
     for arr, w in which.items():
         for name, _typestr, off, _shape in lay[arr]["fields"]:
             ref = R.offsetof(w, name)
+            # Every field of the records this build steps is checked against the reference's offsetof.
+            assert ref >= 0 or arr == "car_states", (arr, name)
             if ref >= 0:
                 assert off == ref, (arr, name, off, ref)
                 checked += 1
-    assert checked >= 12
+    assert checked >= 70
     assert (BALL_POS, BALL_ROT, BALL_VEL, BALL_ANG_VEL) == tuple(
         R.offsetof(R.BALL, f) for f in ("pos", "rot", "vel", "ang_vel"))
 
