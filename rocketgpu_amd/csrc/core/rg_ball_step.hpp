@@ -505,6 +505,7 @@ struct BallTickScratch {
 
 // Predictive contact ray of the ball (Bullet's createPredictiveContacts for the CCD body, as a ray from the center;
 // env_step.hpp:3893-4127, ball branch).
+template <typename Walk = SerialWalk>
 RG_HD bool BallPredictiveRay(const BallState& ball, float dt, const ArenaMeshView& mesh, Vec3& n_out, float& d_out) {
     constexpr float kCcdMotionThresholdBt = 0.02f;
     constexpr float kRayBackoffBt = 0.04f;
@@ -517,7 +518,7 @@ RG_HD bool BallPredictiveRay(const BallState& ball, float dt, const ArenaMeshVie
     const Vec3 from(cur.x - inv_len * m.x * kRayBackoffBt, cur.y - inv_len * m.y * kRayBackoffBt,
                     cur.z - inv_len * m.z * kRayBackoffBt);
     const Vec3 delta(to.x - from.x, to.y - from.y, to.z - from.z);
-    const RayHit hit = CastArenaRay(mesh, from, delta);
+    const RayHit hit = CastArenaRay<Walk>(mesh, from, delta);
     if (!hit.hit || !(hit.fraction < 1.0f)) return false;
     Vec3 n = hit.normal;
     if (hit.tri >= 0 && hit.tri < mesh.tri_count) {
@@ -634,6 +635,7 @@ RG_HD void BallBroadphase(ArenaInternal& st, const ArenaMeshView& mesh, BallTick
 }
 
 // Ball-vs-mesh contacts and their rows (AppendBallWorldSpecialContactRows for meshes only).
+template <typename Walk = SerialWalk>
 RG_HD void BallMeshContacts(BallState& ball, const StepParams& p, ArenaInternal& st, const ArenaMeshView& mesh,
                             BallTickScratch& t, const Vec3& ext_lin_uu, const Vec3& solver_vel_bt, float dt) {
     struct Cp {
@@ -677,29 +679,44 @@ RG_HD void BallMeshContacts(BallState& ball, const StepParams& p, ArenaInternal&
             QuantizePoint(info.q, center + Vec3(r_th, r_th, r_th), 1, qmax);
             Vec3 bx, by, bz;
             BallBasis(ball, bx, by, bz);
-            WalkMeshBvh(
-                mesh.nodes, info.node_start, info.node_start + info.node_count, qmin, qmax,
+            // A leaf's contact point (independent of the other leaves), then its insertion into the manifold (in
+            // node order).
+            const auto contact = [&](int32_t local, BallManifoldPoint& np) {
+                const MeshTriangle& tri = mesh.tris[info.tri_start + local];
+                Vec3 normal, point;
+                float depth, pen_uu;
+                if (!SphereTriangleContact(center, radius_bt, th, tri.a, tri.b, tri.c, normal, point, depth, pen_uu)) {
+                    return false;
+                }
+                const Vec3 on_a = point + normal * depth;
+                const Vec3 rel = on_a - center;
+                np.local_a = Vec3(Dot(rel, bx), Dot(rel, by), Dot(rel, bz));
+                Vec3 on_b = point;
+                Vec3 cn = normal;
+                AdjustInternalEdgeContact(tri, on_a, depth, on_b, cn);
+                np.normal = cn;
+                np.pen_uu = pen_uu;
+                np.local_b = on_b;
+                return true;
+            };
+            struct LeafContact {
+                BallManifoldPoint np;
+                int32_t hit;
+            };
+            Walk::Run(
+                mesh.nodes, info.node_start, info.node_start + info.node_count, qmin, qmax, AcceptAll{},
                 [&](int32_t local) {
-                    const int32_t g = info.tri_start + local;
-                    const MeshTriangle& tri = mesh.tris[g];
-                    Vec3 normal, point;
-                    float depth, pen_uu;
-                    if (!SphereTriangleContact(center, radius_bt, th, tri.a, tri.b, tri.c, normal, point, depth, pen_uu)) {
-                        return;
-                    }
-                    const Vec3 on_a = point + normal * depth;
-                    const Vec3 rel = on_a - center;
                     BallManifoldPoint np;
-                    np.local_a = Vec3(Dot(rel, bx), Dot(rel, by), Dot(rel, bz));
-                    Vec3 on_b = point;
-                    Vec3 cn = normal;
-                    AdjustInternalEdgeContact(tri, on_a, depth, on_b, cn);
-                    np.normal = cn;
-                    np.pen_uu = pen_uu;
-                    np.local_b = on_b;
-                    InsertManifoldPoint(man.pts, count, np, th);
+                    if (contact(local, np)) InsertManifoldPoint(man.pts, count, np, th);
                 },
-                AcceptAll{});
+                [&](int32_t local) {
+                    LeafContact c;
+                    c.hit = contact(local, c.np) ? 1 : 0;
+                    return c;
+                },
+                [&](const LeafContact& c) {
+                    if (c.hit) InsertManifoldPoint(man.pts, count, c.np, th);
+                });
         }
         BallPointFull full[4];
         RefreshManifold(man.pts, count, ball, center, th, full);
@@ -977,6 +994,7 @@ RG_HD void CountdownUpdateTimers(EnvState& env, BallState& ball, float frame_dt)
     }
 }
 
+template <typename Walk = SerialWalk>
 RG_HD void StepBallArena(BallState& ball, EnvState& env, ArenaInternal& st, const ArenaMeshView& mesh,
                          const StepParams& p, BallTickScratch& t) {
     const float dt = p.tick_dt;
@@ -1005,7 +1023,7 @@ RG_HD void StepBallArena(BallState& ball, EnvState& env, ArenaInternal& st, cons
     // 3b. predictive ray.
     t.predictive_hit = 0;
     if (!(sleeping && !vehicle_woke)) {
-        t.predictive_hit = BallPredictiveRay(ball, dt, mesh, t.predictive_normal, t.predictive_distance_bt) ? 1 : 0;
+        t.predictive_hit = BallPredictiveRay<Walk>(ball, dt, mesh, t.predictive_normal, t.predictive_distance_bt) ? 1 : 0;
     }
     // 3c. broadphase.
     {
@@ -1032,7 +1050,7 @@ RG_HD void StepBallArena(BallState& ball, EnvState& env, ArenaInternal& st, cons
         any_retained = any_retained || (((st.ball_mesh_retained >> mi) & 1u) != 0u && st.ball_mesh[mi].count != 0);
     }
     const bool dispatched = !sleeping || vehicle_woke || any_retained;
-    if (awake_rows && dispatched) BallMeshContacts(ball, p, st, mesh, t, ext_lin_uu, solver_vel_bt, dt);
+    if (awake_rows && dispatched) BallMeshContacts<Walk>(ball, p, st, mesh, t, ext_lin_uu, solver_vel_bt, dt);
     // 3e. predictive rows.
     if (t.predictive_hit) BallPredictiveRows(ball, p, st, t, ext_lin_uu, solver_vel_bt, dt);
     // 3f. solve.

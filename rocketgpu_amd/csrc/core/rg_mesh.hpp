@@ -124,6 +124,18 @@ struct AcceptAll {
     RG_HD bool operator()(const uint16_t*, const uint16_t*) const { return true; }
 };
 
+// How a query walks one mesh tree.  A walk policy runs, for every leaf the stackless walk visits (in node order),
+// either serial_leaf(local_tri), or compute(local_tri) then commit(result): compute may run for several leaves at
+// once on other lanes; commit runs in node order.  Both forms give the same result.  SerialWalk is the one-lane
+// walk; a cooperative device walk lives with the kernels.
+struct SerialWalk {
+    template <typename Accept, typename SerialLeaf, typename Compute, typename Commit>
+    RG_HD static void Run(const MeshBvhNode* nodes, int begin, int end, const uint16_t* qmin, const uint16_t* qmax,
+                          Accept&& accept, SerialLeaf&& serial_leaf, Compute&&, Commit&&) {
+        WalkMeshBvh(nodes, begin, end, qmin, qmax, serial_leaf, accept);
+    }
+};
+
 // ------------------------------------------------------------------------------------------------------------
 // Ray cast (Bullet's btTriangleRaycastCallback over btBvhTriangleMeshShape::performRaycast; reference
 // host_validation_and_contact_debug.hpp:1311-1378 and :1683-1944).
@@ -199,7 +211,57 @@ RG_HD void RayTriangle(const MeshTriangle& tri, int32_t tri_idx, const Vec3& fro
     }
 }
 
+// One triangle against the ray, without RayTriangle's early rejection against the best hit so far (valid: the hit
+// RayTriangle would record if the best hit were farther).  CommitRayCandidate then applies RayTriangle's
+// comparisons with the best hit, in its order; the pair records exactly what RayTriangle records.
+struct RayCandidate {
+    float distance;
+    Vec3 normal;
+    int32_t tri;
+    int32_t valid;
+};
+
+RG_HD RayCandidate RayTriangleCandidate(const MeshTriangle& tri, int32_t tri_idx, const Vec3& from, const Vec3& to) {
+    RayCandidate c{0.0f, Vec3(0.0f, 0.0f, 0.0f), tri_idx, 0};
+    const Vec3 v10 = tri.b - tri.a;
+    const Vec3 v20 = tri.c - tri.a;
+    Vec3 n = Cross(v10, v20);
+    const float plane = Dot(tri.a, n);
+    const float dist_a = Dot(n, from) - plane;
+    const float dist_b = Dot(n, to) - plane;
+    if (dist_a * dist_b >= 0.0f) return c;
+    const float proj = dist_a - dist_b;
+    if (proj == 0.0f) return c;
+    const float distance = dist_a / proj;
+    c.distance = distance;
+    const float tol = -0.0001f * LengthSq(n);
+    const Vec3 p = (1.0f - distance) * from + distance * to;
+    const Vec3 v0p = tri.a - p;
+    const Vec3 v1p = tri.b - p;
+    if (Dot(Cross(v0p, v1p), n) < tol) return c;
+    const Vec3 v2p = tri.c - p;
+    if (Dot(Cross(v1p, v2p), n) < tol) return c;
+    if (Dot(Cross(v2p, v0p), n) < tol) return c;
+    n = Normalize(n, tri.normal);
+    if (dist_a <= 0.0f) n = -n;
+    if (distance < 0.0f || distance > 1.0f) return c;
+    c.normal = n;
+    c.valid = 1;
+    return c;
+}
+
+RG_HD void CommitRayCandidate(const RayCandidate& c, RayHit& best) {
+    if (!c.valid || c.distance >= best.fraction) return;
+    if (!best.hit || c.distance < best.fraction) {
+        best.hit = 1;
+        best.fraction = c.distance;
+        best.normal = c.normal;
+        best.tri = c.tri;
+    }
+}
+
 // Nearest hit of the segment from -> from + delta over every mesh (meshes in order, leaves in node order).
+template <typename Walk = SerialWalk>
 RG_HD RayHit CastArenaRay(const ArenaMeshView& m, const Vec3& from, const Vec3& delta) {
     RayHit best;
     const Vec3 to = from + delta;
@@ -214,13 +276,12 @@ RG_HD RayHit CastArenaRay(const ArenaMeshView& m, const Vec3& from, const Vec3& 
         QuantizePoint(info.q, lo, 0, qmin);
         QuantizePoint(info.q, hi, 1, qmax);
         const RayLeafTest test(info.q, from, to);
-        WalkMeshBvh(
-            m.nodes, info.node_start, info.node_start + info.node_count, qmin, qmax,
-            [&](int32_t local) {
-                const int32_t g = info.tri_start + local;
-                RayTriangle(m.tris[g], g, from, to, best);
-            },
-            test);
+        const int32_t tri_start = info.tri_start;
+        Walk::Run(
+            m.nodes, info.node_start, info.node_start + info.node_count, qmin, qmax, test,
+            [&](int32_t local) { RayTriangle(m.tris[tri_start + local], tri_start + local, from, to, best); },
+            [&](int32_t local) { return RayTriangleCandidate(m.tris[tri_start + local], tri_start + local, from, to); },
+            [&](const RayCandidate& c) { CommitRayCandidate(c, best); });
     }
     return best;
 }

@@ -1,0 +1,85 @@
+// Cooperative mesh-tree walk for the gfx950 step kernel: G lanes of one wavefront (aligned, G a power of two up to
+// 32) share one arena and run its tick redundantly, with identical data, so every lane holds the same state.  Only
+// the tree walks split the work.
+//
+// The stackless walk visits nodes in increasing index: from node i it moves to i + 1 when the node is a leaf or
+// its box overlaps the query, else past the node's subtree (i + escape).  The group loads the window i .. i+G-1
+// at once, one node per lane, with the overlap and leaf tests done per lane.  It then follows the walk through the
+// window on ballot masks, so one memory latency covers up to G steps of the chain.  The leaves it visits compute
+// their results in parallel, each on the lane that loaded it.  Those results are broadcast and committed on every
+// lane in node order.  The walk visits the same leaves in the same order as SerialWalk and commits the same
+// results, so the tick's bits are unchanged.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "core/rg_mesh.hpp"
+
+namespace rg {
+
+// Every 32-bit word of a trivially copyable record, read from lane `src`.
+template <typename T>
+__device__ inline T ShuffleRecord(const T& v, int src) {
+    static_assert(sizeof(T) % 4 == 0, "records shuffle as 32-bit words");
+    int in[sizeof(T) / 4];
+    int out[sizeof(T) / 4];
+    __builtin_memcpy(in, &v, sizeof(T));
+#pragma unroll
+    for (int w = 0; w < static_cast<int>(sizeof(T) / 4); ++w) out[w] = __shfl(in[w], src);
+    T r;
+    __builtin_memcpy(&r, out, sizeof(T));
+    return r;
+}
+
+template <int G>
+struct GroupWalk {
+    static_assert(G >= 2 && G <= 32 && (G & (G - 1)) == 0, "group size must be a power of two in 2..32");
+
+    // The group's G bits of a wave-wide ballot.
+    __device__ static uint32_t Bits(unsigned long long ballot, int first) {
+        return static_cast<uint32_t>((ballot >> first) & ((1ull << G) - 1ull));
+    }
+
+    template <typename Accept, typename SerialLeaf, typename Compute, typename Commit>
+    __device__ static void Run(const core::MeshBvhNode* nodes, int begin, int end, const uint16_t* qmin,
+                               const uint16_t* qmax, Accept&& accept, SerialLeaf&&, Compute&& compute,
+                               Commit&& commit) {
+        using Result = decltype(compute(int32_t{0}));
+        const int lane = static_cast<int>(__lane_id());
+        const int k = lane & (G - 1);
+        const int first = lane - k;
+        int base = begin;
+        while (base < end) {
+            const int idx = base + k;
+            const bool valid = idx < end;
+            core::MeshBvhNode node{};
+            if (valid) node = nodes[idx];
+            const bool overlap = valid && core::QuantizedOverlap(qmin, qmax, node.qmin, node.qmax);
+            const bool leaf = valid && node.escape_or_tri >= 0;
+            const bool take = overlap && leaf && accept(node.qmin, node.qmax);
+            const uint32_t valid_m = Bits(__ballot(valid), first);
+            const uint32_t next_m = Bits(__ballot(overlap || leaf), first);
+            const uint32_t take_m = Bits(__ballot(take), first);
+            // Follow the walk through the window (uniform within the group).
+            uint32_t visit = 0;
+            int pos = 0;
+            while (pos < G && ((valid_m >> pos) & 1u) != 0u) {
+                if (((next_m >> pos) & 1u) != 0u) {
+                    visit |= take_m & (1u << pos);
+                    pos += 1;
+                } else {
+                    pos -= __shfl(node.escape_or_tri, first + pos);
+                }
+            }
+            Result mine{};
+            if (((visit >> k) & 1u) != 0u) mine = compute(node.escape_or_tri);
+            for (uint32_t m = visit; m != 0u; m &= m - 1u) {
+                const int j = __builtin_ctz(m);
+                commit(ShuffleRecord(mine, first + j));
+            }
+            base += pos;
+        }
+    }
+};
+
+}  // namespace rg

@@ -17,6 +17,7 @@
 
 #include "batch_common.hpp"
 #include "core/rg_env_commands.hpp"
+#include "group_walk.hpp"
 #include "rocketgpu_amd/rg_capi.h"
 
 namespace rg {
@@ -100,7 +101,9 @@ constexpr int kMaxStepBlock = 256;
 //   3  records and scratch in LDS.
 // Records are copied in before the first tick and back after the last.  LDS holds one slice per active lane of
 // the workgroup: blockDim / 64 waves x arenas_per_wave lanes.
-enum StageMode : int { kStageInPlace = 0, kStagePrivate = 1, kStageLdsScratch = 2, kStageLds = 3 };
+//   4  records in place, and 64 / arenas_per_wave lanes per arena: the lanes of an arena run its tick together
+//      and split its mesh-tree walks (GroupWalk), which shortens the chain of dependent node loads.
+enum StageMode : int { kStageInPlace = 0, kStagePrivate = 1, kStageLdsScratch = 2, kStageLds = 3, kStageGroup = 4 };
 
 struct LdsSlice {
     core::BallTickScratch scratch;
@@ -198,6 +201,22 @@ __global__ void __launch_bounds__(64) ResetToKickoffKernel(StepArgs a, uint64_t 
     if (e >= a.env_count) return;
     core::ResetToKickoffNoCars(a.balls[e], a.envs[e], a.internal[e], EnvPads(a, e), e, seed, a.settled, a.params);
     ClearEnvControls(a, e);
+}
+
+// G lanes per arena (stage 4).  Work item i = thread / G; every lane of the group steps arena e with the same
+// data, so their in-place writes agree.
+template <int G>
+__global__ void __launch_bounds__(kMaxStepBlock) StepBallArenasGroupKernel(StepArgs a, int32_t ticks) {
+    const int g = static_cast<int>(blockIdx.x) * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x);
+    const int i = g / G;
+    if (i >= a.count) return;
+    const int e = a.arena_ids ? a.arena_ids[i] : i;
+    if (e < 0 || e >= a.env_count) return;
+    const int n = a.arena_ticks ? a.arena_ticks[i] : ticks;
+    core::BallTickScratch scratch;
+    for (int t = 0; t < n; ++t) {
+        core::StepBallArena<GroupWalk<G>>(a.balls[e], a.envs[e], a.internal[e], a.mesh, a.params, scratch);
+    }
 }
 
 __device__ inline uint64_t Fnv(uint64_t h, const void* data, int n) {
@@ -542,7 +561,7 @@ public:
         block_ = block;
     }
     void SetStage(int stage) {
-        if (stage < kStageInPlace || stage > kStageLds) throw InvalidArgument("stage must be 0, 1, 2 or 3");
+        if (stage < kStageInPlace || stage > kStageGroup) throw InvalidArgument("stage must be 0, 1, 2, 3 or 4");
         stage_ = stage;
     }
     void SetArenasPerWave(int n) {
@@ -553,6 +572,9 @@ public:
 
     // Rejects a shape whose LDS slices do not fit one workgroup's LDS.
     void CheckLaunchShape() const {
+        if (stage_ == kStageGroup && per_wave_ != 16 && per_wave_ != 8 && per_wave_ != 4) {
+            throw InvalidArgument("stage 4 needs 16, 8 or 4 arenas per wave (4, 8 or 16 lanes per arena)");
+        }
         const size_t lds = LdsBytesPerLane(stage_) * static_cast<size_t>(block_ / 64) * per_wave_;
         int device = 0, limit = 0;
         Check(hipGetDevice(&device), "hipGetDevice");
@@ -705,6 +727,27 @@ private:
         a.arena_ticks = ticks_list;
         a.count = count;
         a.arenas_per_wave = per_wave_;
+        if (stage_ == kStageGroup) {
+            // 64 / per_wave lanes per arena.
+            const int lanes = 64 / per_wave_;
+            const int64_t threads = static_cast<int64_t>(count) * lanes;
+            const int grid = static_cast<int>((threads + block_ - 1) / block_);
+            switch (lanes) {
+            case 4:
+                hipLaunchKernelGGL(StepBallArenasGroupKernel<4>, dim3(grid), dim3(block_), 0, s, a, ticks);
+                break;
+            case 8:
+                hipLaunchKernelGGL(StepBallArenasGroupKernel<8>, dim3(grid), dim3(block_), 0, s, a, ticks);
+                break;
+            case 16:
+                hipLaunchKernelGGL(StepBallArenasGroupKernel<16>, dim3(grid), dim3(block_), 0, s, a, ticks);
+                break;
+            default:
+                throw InvalidArgument("stage 4 needs 16, 8 or 4 arenas per wave (4, 8 or 16 lanes per arena)");
+            }
+            Check(hipGetLastError(), "StepBallArenasGroupKernel launch");
+            return;
+        }
         const int64_t threads = (static_cast<int64_t>(count) + per_wave_ - 1) / per_wave_ * 64;
         const int grid = static_cast<int>((threads + block_ - 1) / block_);
         const size_t lds = LdsBytesPerLane(stage_) * static_cast<size_t>(block_ / 64) * per_wave_;
