@@ -994,7 +994,8 @@ RG_HD void CountdownUpdateTimers(EnvState& env, BallState& ball, float frame_dt)
     }
 }
 
-template <typename Walk = SerialWalk>
+// ContactWalk answers the contact query's mesh walks, RayWalk the predictive ray's.
+template <typename ContactWalk = SerialWalk, typename RayWalk = ContactWalk>
 RG_HD void StepBallArena(BallState& ball, EnvState& env, ArenaInternal& st, const ArenaMeshView& mesh,
                          const StepParams& p, BallTickScratch& t) {
     const float dt = p.tick_dt;
@@ -1023,7 +1024,7 @@ RG_HD void StepBallArena(BallState& ball, EnvState& env, ArenaInternal& st, cons
     // 3b. predictive ray.
     t.predictive_hit = 0;
     if (!(sleeping && !vehicle_woke)) {
-        t.predictive_hit = BallPredictiveRay<Walk>(ball, dt, mesh, t.predictive_normal, t.predictive_distance_bt) ? 1 : 0;
+        t.predictive_hit = BallPredictiveRay<RayWalk>(ball, dt, mesh, t.predictive_normal, t.predictive_distance_bt) ? 1 : 0;
     }
     // 3c. broadphase.
     {
@@ -1050,7 +1051,7 @@ RG_HD void StepBallArena(BallState& ball, EnvState& env, ArenaInternal& st, cons
         any_retained = any_retained || (((st.ball_mesh_retained >> mi) & 1u) != 0u && st.ball_mesh[mi].count != 0);
     }
     const bool dispatched = !sleeping || vehicle_woke || any_retained;
-    if (awake_rows && dispatched) BallMeshContacts<Walk>(ball, p, st, mesh, t, ext_lin_uu, solver_vel_bt, dt);
+    if (awake_rows && dispatched) BallMeshContacts<ContactWalk>(ball, p, st, mesh, t, ext_lin_uu, solver_vel_bt, dt);
     // 3e. predictive rows.
     if (t.predictive_hit) BallPredictiveRows(ball, p, st, t, ext_lin_uu, solver_vel_bt, dt);
     // 3f. solve.

@@ -215,7 +215,9 @@ __global__ void __launch_bounds__(kMaxStepBlock) StepBallArenasGroupKernel(StepA
     const int n = a.arena_ticks ? a.arena_ticks[i] : ticks;
     core::BallTickScratch scratch;
     for (int t = 0; t < n; ++t) {
-        core::StepBallArena<GroupWalk<G>>(a.balls[e], a.envs[e], a.internal[e], a.mesh, a.params, scratch);
+        // The predictive ray's walk is short and stays serial (every lane walks it, with the same loads).
+        core::StepBallArena<GroupWalk<G>, core::SerialWalk>(a.balls[e], a.envs[e], a.internal[e], a.mesh, a.params,
+                                                             scratch);
     }
 }
 
