@@ -34,6 +34,8 @@ __device__ inline T ShuffleRecord(const T& v, int src) {
 template <int G>
 struct GroupWalk {
     static_assert(G >= 2 && G <= 32 && (G & (G - 1)) == 0, "group size must be a power of two in 2..32");
+    // Bits of a landing code in 0..G.
+    static constexpr int kLandBits = G >= 32 ? 6 : G >= 16 ? 5 : G >= 8 ? 4 : G >= 4 ? 3 : 2;
 
     // The group's G bits of a wave-wide ballot.
     __device__ static uint32_t Bits(unsigned long long ballot, int first) {
@@ -57,27 +59,49 @@ struct GroupWalk {
             const bool overlap = valid && core::QuantizedOverlap(qmin, qmax, node.qmin, node.qmax);
             const bool leaf = valid && node.escape_or_tri >= 0;
             const bool take = overlap && leaf && accept(node.qmin, node.qmax);
+            // Window-relative landing of each lane's node when the walk reaches it: the next node, or past its
+            // subtree.  Capped at G + 1 ("beyond the window") and spread over kLandBits ballots, so the group can
+            // follow the walk through the window on bitmasks alone.
+            const int land = (overlap || leaf) ? k + 1 : k - node.escape_or_tri;
+            const int code = (land < G + 1 ? land : G + 1) - 1;
             const uint32_t valid_m = Bits(__ballot(valid), first);
             const uint32_t next_m = Bits(__ballot(overlap || leaf), first);
             const uint32_t take_m = Bits(__ballot(take), first);
-            // Follow the walk through the window (uniform within the group).
+            uint32_t land_m[kLandBits];
+#pragma unroll
+            for (int b = 0; b < kLandBits; ++b) land_m[b] = Bits(__ballot((code >> b) & 1), first);
             uint32_t visit = 0;
-            int pos = 0;
+            int pos = 0, far = -1;
             while (pos < G && ((valid_m >> pos) & 1u) != 0u) {
-                if (((next_m >> pos) & 1u) != 0u) {
-                    visit |= take_m & (1u << pos);
-                    pos += 1;
+                // The run of nodes from pos that the walk steps through one by one, up to the first node it skips
+                // past (or the end of the mesh's nodes).
+                const uint64_t from = ~0ull << pos;
+                const uint64_t stops = ((~static_cast<uint64_t>(next_m) | ~static_cast<uint64_t>(valid_m)) & from) |
+                                       (1ull << G);
+                const int q = __builtin_ctzll(stops);
+                visit |= take_m & static_cast<uint32_t>(((1ull << q) - 1ull) & from);
+                if (q >= G || ((valid_m >> q) & 1u) == 0u) {
+                    pos = q;
+                    break;
+                }
+                int c = 0;
+#pragma unroll
+                for (int b = 0; b < kLandBits; ++b) c |= static_cast<int>((land_m[b] >> q) & 1u) << b;
+                if (c + 1 <= G) {
+                    pos = c + 1;
                 } else {
-                    pos -= __shfl(node.escape_or_tri, first + pos);
+                    far = q;
+                    break;
                 }
             }
+            const int far_land = __shfl(land, first + (far >= 0 ? far : 0));
             Result mine{};
             if (((visit >> k) & 1u) != 0u) mine = compute(node.escape_or_tri);
             for (uint32_t m = visit; m != 0u; m &= m - 1u) {
                 const int j = __builtin_ctz(m);
                 commit(ShuffleRecord(mine, first + j));
             }
-            base += pos;
+            base += far >= 0 ? far_land : pos;
         }
     }
 };
