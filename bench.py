@@ -53,7 +53,7 @@ def parse():
     p.add_argument("--block", type=int, default=0, help="lanes per workgroup (0: the tuner chooses the shape)")
     p.add_argument("--stage", type=int, default=0,
                    help="state staging with --block: 0 in place, 1 private copy, 2 LDS scratch, 3 LDS state+scratch, "
-                        "4 in place with 64/per-wave lanes per arena sharing its tree walks")
+                        "4 in place with 64/per-wave lanes per arena sharing its tree walks, 5 as 4 with the scratch in LDS")
     p.add_argument("--per-wave", type=int, default=64, help="arenas per 64-lane wavefront with --block")
     p.add_argument("--no-forms", action="store_true")
     p.add_argument("--no-serve", action="store_true")
@@ -146,7 +146,7 @@ def main():
         key = lambda c: f"block{c[0]}_stage{c[1]}_perwave{c[2]}"
         cands = ([(64, s, w) for s in (0, 1) for w in (64, 32, 16, 8, 4)] + [(128, 0, 64), (256, 0, 64)] +
                  [(64, 2, w) for w in (64, 32, 16, 8)] + [(64, 3, w) for w in (16, 8, 4)] + [(128, 3, 4), (256, 3, 4)] +
-                 [(b, 4, w) for b in (64, 128, 256) for w in (16, 8, 4)])
+                 [(b, st, w) for st in (4, 5) for b in (64, 128, 256) for w in (16, 8, 4)])
         for cand in cands:
             sim_b, _, _ = make_batch(rga, np, n, args.max_cars, args.seed)
             try:

@@ -102,8 +102,19 @@ constexpr int kMaxStepBlock = 256;
 // Records are copied in before the first tick and back after the last.  LDS holds one slice per active lane of
 // the workgroup: blockDim / 64 waves x arenas_per_wave lanes.
 //   4  records in place, and 64 / arenas_per_wave lanes per arena: the lanes of an arena run its tick together
-//      and split its mesh-tree walks (GroupWalk), which shortens the chain of dependent node loads.
-enum StageMode : int { kStageInPlace = 0, kStagePrivate = 1, kStageLdsScratch = 2, kStageLds = 3, kStageGroup = 4 };
+//      and split its mesh-tree walks (GroupWalk), which shortens the chain of dependent node loads;
+//   5  as 4, with the arena's per-tick contact scratch in LDS, one slice shared by the arena's lanes (instead of a
+//      private copy on every lane).
+enum StageMode : int {
+    kStageInPlace = 0,
+    kStagePrivate = 1,
+    kStageLdsScratch = 2,
+    kStageLds = 3,
+    kStageGroup = 4,
+    kStageGroupLds = 5
+};
+
+constexpr bool IsGroupStage(int stage) { return stage == kStageGroup || stage == kStageGroupLds; }
 
 struct LdsSlice {
     core::BallTickScratch scratch;
@@ -112,8 +123,10 @@ struct LdsSlice {
     EnvState env;
 };
 
+// LDS per arena of a launch (a workgroup holds blockDim / 64 x arenas_per_wave arenas).
 __host__ __device__ constexpr size_t LdsBytesPerLane(int stage) {
-    return stage == kStageLds ? sizeof(LdsSlice) : (stage == kStageLdsScratch ? sizeof(core::BallTickScratch) : 0);
+    return stage == kStageLds ? sizeof(LdsSlice)
+                              : (stage == kStageLdsScratch || stage == kStageGroupLds ? sizeof(core::BallTickScratch) : 0);
 }
 
 // One lane per arena; its ticks back to back.
@@ -205,15 +218,18 @@ __global__ void __launch_bounds__(64) ResetToKickoffKernel(StepArgs a, uint64_t 
 
 // G lanes per arena (stage 4).  Work item i = thread / G; every lane of the group steps arena e with the same
 // data, so their in-place writes agree.
-template <int G>
+template <int G, bool kLdsScratch>
 __global__ void __launch_bounds__(kMaxStepBlock) StepBallArenasGroupKernel(StepArgs a, int32_t ticks) {
+    extern __shared__ __align__(16) unsigned char lds[];
     const int g = static_cast<int>(blockIdx.x) * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x);
     const int i = g / G;
     if (i >= a.count) return;
     const int e = a.arena_ids ? a.arena_ids[i] : i;
     if (e < 0 || e >= a.env_count) return;
     const int n = a.arena_ticks ? a.arena_ticks[i] : ticks;
-    core::BallTickScratch scratch;
+    core::BallTickScratch private_scratch;
+    core::BallTickScratch& scratch =
+        kLdsScratch ? reinterpret_cast<core::BallTickScratch*>(lds)[static_cast<int>(threadIdx.x) / G] : private_scratch;
     for (int t = 0; t < n; ++t) {
         // The predictive ray's walk is short and stays serial (every lane walks it, with the same loads).
         core::StepBallArena<GroupWalk<G>, core::SerialWalk>(a.balls[e], a.envs[e], a.internal[e], a.mesh, a.params,
@@ -563,7 +579,7 @@ public:
         block_ = block;
     }
     void SetStage(int stage) {
-        if (stage < kStageInPlace || stage > kStageGroup) throw InvalidArgument("stage must be 0, 1, 2, 3 or 4");
+        if (stage < kStageInPlace || stage > kStageGroupLds) throw InvalidArgument("stage must be 0 to 5");
         stage_ = stage;
     }
     void SetArenasPerWave(int n) {
@@ -574,8 +590,8 @@ public:
 
     // Rejects a shape whose LDS slices do not fit one workgroup's LDS.
     void CheckLaunchShape() const {
-        if (stage_ == kStageGroup && per_wave_ != 16 && per_wave_ != 8 && per_wave_ != 4) {
-            throw InvalidArgument("stage 4 needs 16, 8 or 4 arenas per wave (4, 8 or 16 lanes per arena)");
+        if (IsGroupStage(stage_) && per_wave_ != 16 && per_wave_ != 8 && per_wave_ != 4) {
+            throw InvalidArgument("stages 4 and 5 need 16, 8 or 4 arenas per wave (4, 8 or 16 lanes per arena)");
         }
         const size_t lds = LdsBytesPerLane(stage_) * static_cast<size_t>(block_ / 64) * per_wave_;
         int device = 0, limit = 0;
@@ -589,6 +605,12 @@ public:
             Check(hipFuncSetAttribute(reinterpret_cast<const void*>(&StepBallArenasKernel<kStageLdsScratch>),
                                       hipFuncAttributeMaxDynamicSharedMemorySize, limit), "hipFuncSetAttribute");
             Check(hipFuncSetAttribute(reinterpret_cast<const void*>(&StepBallArenasKernel<kStageLds>),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, limit), "hipFuncSetAttribute");
+            Check(hipFuncSetAttribute(reinterpret_cast<const void*>(&StepBallArenasGroupKernel<4, true>),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, limit), "hipFuncSetAttribute");
+            Check(hipFuncSetAttribute(reinterpret_cast<const void*>(&StepBallArenasGroupKernel<8, true>),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, limit), "hipFuncSetAttribute");
+            Check(hipFuncSetAttribute(reinterpret_cast<const void*>(&StepBallArenasGroupKernel<16, true>),
                                       hipFuncAttributeMaxDynamicSharedMemorySize, limit), "hipFuncSetAttribute");
         }
     }
@@ -729,23 +751,28 @@ private:
         a.arena_ticks = ticks_list;
         a.count = count;
         a.arenas_per_wave = per_wave_;
-        if (stage_ == kStageGroup) {
+        if (IsGroupStage(stage_)) {
             // 64 / per_wave lanes per arena.
             const int lanes = 64 / per_wave_;
             const int64_t threads = static_cast<int64_t>(count) * lanes;
-            const int grid = static_cast<int>((threads + block_ - 1) / block_);
+            const dim3 grid(static_cast<unsigned>((threads + block_ - 1) / block_)), block(block_);
+            const size_t lds = LdsBytesPerLane(stage_) * static_cast<size_t>(block_ / lanes);
+            const bool in_lds = stage_ == kStageGroupLds;
             switch (lanes) {
             case 4:
-                hipLaunchKernelGGL(StepBallArenasGroupKernel<4>, dim3(grid), dim3(block_), 0, s, a, ticks);
+                if (in_lds) hipLaunchKernelGGL((StepBallArenasGroupKernel<4, true>), grid, block, lds, s, a, ticks);
+                else hipLaunchKernelGGL((StepBallArenasGroupKernel<4, false>), grid, block, 0, s, a, ticks);
                 break;
             case 8:
-                hipLaunchKernelGGL(StepBallArenasGroupKernel<8>, dim3(grid), dim3(block_), 0, s, a, ticks);
+                if (in_lds) hipLaunchKernelGGL((StepBallArenasGroupKernel<8, true>), grid, block, lds, s, a, ticks);
+                else hipLaunchKernelGGL((StepBallArenasGroupKernel<8, false>), grid, block, 0, s, a, ticks);
                 break;
             case 16:
-                hipLaunchKernelGGL(StepBallArenasGroupKernel<16>, dim3(grid), dim3(block_), 0, s, a, ticks);
+                if (in_lds) hipLaunchKernelGGL((StepBallArenasGroupKernel<16, true>), grid, block, lds, s, a, ticks);
+                else hipLaunchKernelGGL((StepBallArenasGroupKernel<16, false>), grid, block, 0, s, a, ticks);
                 break;
             default:
-                throw InvalidArgument("stage 4 needs 16, 8 or 4 arenas per wave (4, 8 or 16 lanes per arena)");
+                throw InvalidArgument("stages 4 and 5 need 16, 8 or 4 arenas per wave (4, 8 or 16 lanes per arena)");
             }
             Check(hipGetLastError(), "StepBallArenasGroupKernel launch");
             return;
