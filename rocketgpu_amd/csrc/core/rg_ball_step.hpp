@@ -702,6 +702,7 @@ RG_HD void BallMeshContacts(BallState& ball, const StepParams& p, ArenaInternal&
             struct LeafContact {
                 BallManifoldPoint np;
                 int32_t hit;
+                RG_HD bool keep() const { return hit != 0; }  // commit changes nothing otherwise
             };
             Walk::Run(
                 mesh.nodes, info.node_start, info.node_start + info.node_count, qmin, qmax, AcceptAll{},

@@ -126,7 +126,8 @@ struct AcceptAll {
 
 // How a query walks one mesh tree.  A walk policy runs, for every leaf the stackless walk visits (in node order),
 // either serial_leaf(local_tri), or compute(local_tri) then commit(result): compute may run for several leaves at
-// once on other lanes; commit runs in node order.  Both forms give the same result.  SerialWalk is the one-lane
+// once on other lanes; commit runs in node order, and may be left out for a result whose keep() is false (its
+// commit changes nothing).  Both forms give the same result.  SerialWalk is the one-lane
 // walk; a cooperative device walk lives with the kernels.
 struct SerialWalk {
     template <typename Accept, typename SerialLeaf, typename Compute, typename Commit>
@@ -219,6 +220,7 @@ struct RayCandidate {
     Vec3 normal;
     int32_t tri;
     int32_t valid;
+    RG_HD bool keep() const { return valid != 0; }  // CommitRayCandidate changes nothing otherwise
 };
 
 RG_HD RayCandidate RayTriangleCandidate(const MeshTriangle& tri, int32_t tri_idx, const Vec3& from, const Vec3& to) {

@@ -96,10 +96,14 @@ struct GroupWalk {
             }
             const int far_land = __shfl(land, first + (far >= 0 ? far : 0));
             Result mine{};
-            if (((visit >> k) & 1u) != 0u) mine = compute(node.escape_or_tri);
-            for (uint32_t m = visit; m != 0u; m &= m - 1u) {
-                const int j = __builtin_ctz(m);
-                commit(ShuffleRecord(mine, first + j));
+            bool keep = false;
+            if (((visit >> k) & 1u) != 0u) {
+                mine = compute(node.escape_or_tri);
+                keep = mine.keep();
+            }
+            // Only the results whose commit changes something travel between lanes, in node order.
+            for (uint32_t m = Bits(__ballot(keep), first); m != 0u; m &= m - 1u) {
+                commit(ShuffleRecord(mine, first + __builtin_ctz(m)));
             }
             base += far >= 0 ? far_land : pos;
         }
