@@ -704,8 +704,10 @@ RG_HD void BallMeshContacts(BallState& ball, const StepParams& p, ArenaInternal&
                 int32_t hit;
                 RG_HD bool keep() const { return hit != 0; }  // commit changes nothing otherwise
             };
+            constexpr float kLeafReachMarginBt = 0.01f;
+            const SphereNearLeaf near_leaf(info.q, center, r_th + kLeafReachMarginBt);
             Walk::Run(
-                mesh.nodes, info.node_start, info.node_start + info.node_count, qmin, qmax, AcceptAll{},
+                mesh.nodes, info.node_start, info.node_start + info.node_count, qmin, qmax, near_leaf,
                 [&](int32_t local) {
                     BallManifoldPoint np;
                     if (contact(local, np)) InsertManifoldPoint(man.pts, count, np, th);

@@ -288,6 +288,28 @@ RG_HD RayHit CastArenaRay(const ArenaMeshView& m, const Vec3& from, const Vec3& 
     return best;
 }
 
+// Leaf filter of a sphere query: keeps the leaves whose (dequantized) box comes within `reach` of the center.  A
+// triangle lies in its leaf's box, and SphereTriangleContact reports a contact only for a point of the triangle
+// (computed to about 1e-4 bt) closer than r + threshold to the center, so with reach = r + threshold + 0.01 bt
+// every leaf this drops would have given no contact: the walk's result is unchanged, without the triangle's load
+// and test.
+struct SphereNearLeaf {
+    const MeshQuantization* q;
+    Vec3 center;
+    float reach_sq;
+
+    RG_HD SphereNearLeaf(const MeshQuantization& quant, const Vec3& c, float reach)
+        : q(&quant), center(c), reach_sq(reach * reach) {}
+    RG_HD bool operator()(const uint16_t* qmin, const uint16_t* qmax) const {
+        const Vec3 lo = UnquantizePoint(*q, qmin);
+        const Vec3 hi = UnquantizePoint(*q, qmax);
+        const float dx = Max(Max(lo.x - center.x, center.x - hi.x), 0.0f);
+        const float dy = Max(Max(lo.y - center.y, center.y - hi.y), 0.0f);
+        const float dz = Max(Max(lo.z - center.z, center.z - hi.z), 0.0f);
+        return dx * dx + dy * dy + dz * dz <= reach_sq;
+    }
+};
+
 // ------------------------------------------------------------------------------------------------------------
 // Sphere against triangle (Bullet's SphereTriangleDetector::collide; reference arena_geometry.hpp:1905-1990).
 
