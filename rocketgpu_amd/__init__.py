@@ -21,7 +21,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "_build", "librocketgpu_amd.so")
+# The built library; RG_LIBRARY_PATH points the package at another build of it.
+LIB_PATH = os.environ.get("RG_LIBRARY_PATH") or os.path.join(HERE, "_build", "librocketgpu_amd.so")
 
 RG_ARRAY = {"car_states": 0, "ball_states": 1, "env_states": 2, "boost_pad_states": 3, "controls": 4,
             "env_car_count": 5}

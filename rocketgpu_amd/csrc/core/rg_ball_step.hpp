@@ -492,6 +492,19 @@ RG_HD void RefreshManifold(BallManifoldPoint* c, int& count, const BallState& ba
 // With the ball alone the manifold list holds at most one manifold per static mesh plus the predictive one.
 constexpr int kMaxBallManifoldEntries = kMaxArenaMeshes + 2;
 
+// A retained ball-mesh contact point with what its rows need.
+struct BallContactPoint {
+    Vec3 normal;
+    float pen_uu;
+    float rel_len_bt;
+    Vec3 rel_pos1_bt;
+    float pen_bt;
+    int16_t static_id;
+};
+
+constexpr int kMaxBallContactPoints = 16;
+
+// Per-tick working data of one arena (no state survives the tick).  Kept in one place so a launch can put it in LDS.
 struct BallTickScratch {
     BodyProxy ball_proxy;
     Vec3 predict_vel_bt;
@@ -501,6 +514,8 @@ struct BallTickScratch {
     ManifoldEntry entries[kMaxBallManifoldEntries];
     int entry_count;
     BallRows rows;
+    BallContactPoint cps[kMaxBallContactPoints];  // the contact points of the tick's ball-mesh manifolds
+    BallPointFull full[4];                        // one manifold's refreshed points
 };
 
 // Predictive contact ray of the ball (Bullet's createPredictiveContacts for the CCD body, as a ray from the center;
@@ -638,16 +653,8 @@ RG_HD void BallBroadphase(ArenaInternal& st, const ArenaMeshView& mesh, BallTick
 template <typename Walk = SerialWalk>
 RG_HD void BallMeshContacts(BallState& ball, const StepParams& p, ArenaInternal& st, const ArenaMeshView& mesh,
                             BallTickScratch& t, const Vec3& ext_lin_uu, const Vec3& solver_vel_bt, float dt) {
-    struct Cp {
-        Vec3 normal;
-        float pen_uu;
-        float rel_len_bt;
-        Vec3 rel_pos1_bt;
-        float pen_bt;
-        int16_t static_id;
-    };
-    constexpr int kMaxCps = 16;
-    Cp cps[kMaxCps];
+    constexpr int kMaxCps = kMaxBallContactPoints;
+    BallContactPoint* cps = t.cps;
     int n_cp = 0;
     int agg_count = 0;
     Vec3 agg_normal(0.0f, 0.0f, 0.0f);
@@ -721,13 +728,13 @@ RG_HD void BallMeshContacts(BallState& ball, const StepParams& p, ArenaInternal&
                     if (c.hit) InsertManifoldPoint(man.pts, count, c.np, th);
                 });
         }
-        BallPointFull full[4];
+        BallPointFull* full = t.full;
         RefreshManifold(man.pts, count, ball, center, th, full);
         man.count = static_cast<uint8_t>(count);
         next_retained = count > 0 ? (next_retained | (1u << mi)) : (next_retained & ~(1u << mi));
         for (int i = 0; i < count; ++i) {
             if (n_cp < kMaxCps) {
-                Cp& c = cps[n_cp++];
+                BallContactPoint& c = cps[n_cp++];
                 c.normal = full[i].p.normal;
                 c.pen_uu = full[i].p.pen_uu;
                 c.rel_len_bt = full[i].rel_len_bt;

@@ -218,8 +218,11 @@ __global__ void __launch_bounds__(64) ResetToKickoffKernel(StepArgs a, uint64_t 
 
 // G lanes per arena (stage 4).  Work item i = thread / G; every lane of the group steps arena e with the same
 // data, so their in-place writes agree.
+// At most 256 registers per lane, so that two waves fit each SIMD (8 arenas per wave fill 8 waves per CU at 16,384
+// arenas).
 template <int G, bool kLdsScratch>
-__global__ void __launch_bounds__(kMaxStepBlock) StepBallArenasGroupKernel(StepArgs a, int32_t ticks) {
+__global__ void __launch_bounds__(kMaxStepBlock) __attribute__((amdgpu_waves_per_eu(2)))
+StepBallArenasGroupKernel(StepArgs a, int32_t ticks) {
     extern __shared__ __align__(16) unsigned char lds[];
     const int g = static_cast<int>(blockIdx.x) * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x);
     const int i = g / G;
