@@ -36,6 +36,14 @@ sys.path.insert(0, REPO)
 HBM_PEAK_GBS = 8000.0
 
 
+def kernel_name(shape):
+    """The step kernel a launch shape runs (rocketgpu_amd/csrc/rg_batch.hip)."""
+    block, stage, per_wave = shape
+    if stage in (4, 5):
+        return f"StepBallArenasGroupKernel<{64 // per_wave}, {'true' if stage == 5 else 'false'}>"
+    return f"StepBallArenasKernel<{stage}>"
+
+
 # Algorithmic bytes per arena-tick at n cars: B(n) = 2 * (12588 + 2456 n + 130 n (n - 1)) + 144 n + 4.
 def bytes_per_arena_tick(n_cars: float) -> float:
     return 2.0 * (12588 + 2456 * n_cars + 130 * n_cars * (n_cars - 1)) + 144 * n_cars + 4
@@ -192,8 +200,11 @@ def main():
     if os.path.exists(args.traffic):
         with open(args.traffic) as fh:
             tr = json.load(fh)
-        if tr.get("arenas") == n and tr.get("ticks") == T:
-            traffic = tr.get("hbm_bytes_per_launch")
+        # Measured per kernel form: the staging and lanes per arena decide the traffic, the block size does not.
+        for entry in tr.get("entries", []):
+            if (tr.get("arenas") == n and tr.get("ticks") == T and entry.get("stage") == shape[1]
+                    and entry.get("arenas_per_wave") == shape[2]):
+                traffic = entry.get("hbm_bytes_per_launch")
 
     envs = sim.download_state_array("env_states")
     ticks_done = envs[:, 0:4].copy().view(np.uint32)[:, 0]
@@ -244,7 +255,7 @@ def main():
                    "launch_shape": {"block": shape[0], "stage": shape[1], "arenas_per_wave": shape[2]}, "launch_shape_sweep_ms": sweep or None},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "StepBallArenasKernel", "kernel_avg_ms": avg_kernel_s * 1e3,
+                     "kernel": kernel_name(shape), "kernel_avg_ms": avg_kernel_s * 1e3,
                      "bytes_per_arena_tick": bytes_per_arena_tick(0)},
         "forms": forms,
         "serving": serving,
