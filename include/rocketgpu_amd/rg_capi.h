@@ -140,9 +140,11 @@ int rg_batch_step_arenas(rg_batch* batch, const int32_t* device_arena_ids, const
 int rg_batch_step_graph(rg_batch* batch, int32_t ticks_per_launch, int32_t launches, void* stream);
 /* Launch shape of the step kernel: lanes per workgroup (64, 128 or 256); staging (0: arena records stepped in place
  * in HBM; 1: records copied into lane-private memory for the launch; 2: records in place, per-tick contact scratch
- * in LDS; 3: records and scratch in LDS); arenas per 64-lane wavefront (64, 32, 16, 8 or 4).  A shape whose LDS
- * exceeds the device's per-workgroup limit is rejected (RG_EINVAL) and the previous shape kept.  Results are
- * identical for every shape; the benchmark tunes it.  Defaults 64, 0, 64. */
+ * in LDS; 3: records and scratch in LDS; 4: 64/arenas_per_wave lanes share each arena and split its contact walks;
+ * 5: as 4, with the per-tick scratch in LDS, one slice per arena); arenas per 64-lane wavefront (64, 32, 16, 8 or
+ * 4; stages 4 and 5 take 16, 8 or 4).  A shape whose LDS exceeds the device's per-workgroup limit is rejected
+ * (RG_EINVAL) and the previous shape kept.  Results are identical for every shape; the benchmark tunes it.
+ * Defaults 64, 0, 64. */
 int rg_batch_set_launch_shape(rg_batch* batch, int32_t block, int32_t stage, int32_t arenas_per_wave);
 
 /* ball_path(samples, ticks, goal_y, out, result, stream)  (rocketgpu.h:1069-1073): every arena's ball stepped in
