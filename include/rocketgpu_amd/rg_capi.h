@@ -111,6 +111,18 @@ int rg_batch_set_car_configs(rg_batch* batch, const void* host_configs, size_t c
  * asleep at the spawn, as RocketSim does. */
 int rg_batch_set_kickoff_settled(rg_batch* batch, int32_t on);
 
+/* set_physics_event_stream(stream)  (rocketgpu.h:1015; PhysicsEventStream rocketgpu.h:68-89): device buffers of
+ * capacity_per_env PhysicsEvent records per env and one PhysicsEventWindow per env (rg_types.h).  All zero disables
+ * the stream; otherwise env_count and cars_per_env must be the batch's and the buffers non-null (RG_EINVAL,
+ * "physics event stream must match the complete batch").  Every step form (step, step_with_device_controls,
+ * step_with_device_commands, step_arenas, step_graph, ball_path) then stamps window.tick with the env's tick at the
+ * start of each tick and appends the tick's events: BallWorldContact for each mesh triangle the ball touches, Goal
+ * for a goal (goal_only nonzero keeps only these).  The caller clears window.count and window.overflow between
+ * reads; a full window sets overflow. */
+int rg_batch_set_physics_event_stream(rg_batch* batch, void* device_records, void* device_windows,
+                                      uint32_t capacity_per_env, uint32_t env_count, uint32_t cars_per_env,
+                                      int32_t goal_only);
+
 /* reset_to_kickoff(seed, stream)  (rocketgpu.h:1065): every env to a new match's kickoff. */
 int rg_batch_reset_to_kickoff(rg_batch* batch, uint64_t seed, void* stream);
 /* apply_env_commands(host_commands, count, stream)  (rocketgpu.h:1066): EnvCommand i (16 B records,

@@ -1,8 +1,9 @@
 // Public state records of the batched arena step.  Every record here is byte-compatible with the reference's
 // public structs (include/rocketgpu/rocketgpu.h:36-706): same field order, types, defaults and therefore the same
 // sizes (CarState 880 B, BallState 288 B, EnvState 44 B, BoostPadState 16 B, CarControls 24 B, CarConfig 120 B,
-// EnvCommand 16 B), so arrays downloaded from either implementation can be compared byte for byte and fed to the
-// other.  Field names follow the reference because state_layouts() and download_state_array() expose them.
+// EnvCommand 16 B, PhysicsEvent 104 B, PhysicsEventWindow 16 B), so arrays downloaded from either implementation can
+// be compared byte for byte and fed to the other.  Field names follow the reference because state_layouts() and
+// download_state_array() expose them.
 //
 // The records are plain aggregates usable from host C++ and from gfx950 device code.
 #pragma once
@@ -290,6 +291,37 @@ struct EnvCommand {
     uint64_t kickoff_seed = 0;
 };
 
+// Physics event stream (rocketgpu.h:45-98).  The caller owns both buffers: records holds capacity_per_env records
+// per env, windows one PhysicsEventWindow per env.  The step sets each env's window.tick to the env's tick at the
+// start of every tick and appends that tick's events after window.count; the caller clears count and overflow
+// between reads.  A full window sets overflow and drops the event.
+enum class PhysicsEventType : uint8_t { BallTouchStart, BallWorldContact, CarWorldContact, BoostPickup, CarBump, Goal };
+
+struct PhysicsEvent {
+    uint64_t tick = 0;
+    PhysicsEventType type = PhysicsEventType::BallTouchStart;
+    uint32_t car_id = 0;
+    uint32_t other_car_id = 0;
+    int32_t team = -1;
+    uint32_t pad_index = 0;
+    uint8_t car_on_ground = 0;
+    uint8_t is_big_pad = 0;
+    uint8_t is_demo = 0;
+    Vec3 position = {};
+    Vec3 normal = {};
+    Vec3 car_velocity = {};
+    Vec3 ball_position = {};
+    Vec3 ball_velocity = {};
+    float boost_before = 0;
+    float boost_after = 0;
+};
+
+struct PhysicsEventWindow {
+    uint64_t tick = 0;
+    uint32_t count = 0;
+    uint32_t overflow = 0;
+};
+
 static_assert(sizeof(CarControls) == 24, "CarControls layout");
 static_assert(sizeof(CarConfig) == 120, "CarConfig layout");
 static_assert(sizeof(CarState) == 880, "CarState layout");
@@ -297,5 +329,7 @@ static_assert(sizeof(BallState) == 288, "BallState layout");
 static_assert(sizeof(EnvState) == 44, "EnvState layout");
 static_assert(sizeof(BoostPadState) == 16, "BoostPadState layout");
 static_assert(sizeof(EnvCommand) == 16, "EnvCommand layout");
+static_assert(sizeof(PhysicsEvent) == 104, "PhysicsEvent layout");
+static_assert(sizeof(PhysicsEventWindow) == 16, "PhysicsEventWindow layout");
 
 }  // namespace rg

@@ -104,7 +104,7 @@ extern "C" {
 const char* rgref_last_error() { return g_error.c_str(); }
 
 // Struct sizes of the reference build: 0 CarState, 1 BallState, 2 EnvState, 3 BoostPadState, 4 CarControls,
-// 5 CarConfig, 6 EnvCommand.
+// 5 CarConfig, 6 EnvCommand, 7 PhysicsEvent, 8 PhysicsEventWindow.
 long rgref_sizeof(int which) {
     switch (which) {
     case 0: return sizeof(CarState);
@@ -114,6 +114,8 @@ long rgref_sizeof(int which) {
     case 4: return sizeof(CarControls);
     case 5: return sizeof(CarConfig);
     case 6: return sizeof(EnvCommand);
+    case 7: return sizeof(PhysicsEvent);
+    case 8: return sizeof(PhysicsEventWindow);
     default: return -1;
     }
 }
@@ -174,6 +176,17 @@ long rgref_offsetof(int which, const char* field) {
     } else if (which == 3) {
         RGREF_FIELD(BoostPadState, cooldown) RGREF_FIELD(BoostPadState, is_active)
         RGREF_FIELD(BoostPadState, pickup_frac_q) RGREF_FIELD(BoostPadState, cur_locked_car_id)
+    } else if (which == 7) {
+        RGREF_FIELD(PhysicsEvent, tick) RGREF_FIELD(PhysicsEvent, type) RGREF_FIELD(PhysicsEvent, car_id)
+        RGREF_FIELD(PhysicsEvent, other_car_id) RGREF_FIELD(PhysicsEvent, team) RGREF_FIELD(PhysicsEvent, pad_index)
+        RGREF_FIELD(PhysicsEvent, car_on_ground) RGREF_FIELD(PhysicsEvent, is_big_pad)
+        RGREF_FIELD(PhysicsEvent, is_demo) RGREF_FIELD(PhysicsEvent, position) RGREF_FIELD(PhysicsEvent, normal)
+        RGREF_FIELD(PhysicsEvent, car_velocity) RGREF_FIELD(PhysicsEvent, ball_position)
+        RGREF_FIELD(PhysicsEvent, ball_velocity) RGREF_FIELD(PhysicsEvent, boost_before)
+        RGREF_FIELD(PhysicsEvent, boost_after)
+    } else if (which == 8) {
+        RGREF_FIELD(PhysicsEventWindow, tick) RGREF_FIELD(PhysicsEventWindow, count)
+        RGREF_FIELD(PhysicsEventWindow, overflow)
     }
 #undef RGREF_SUB
 #undef RGREF_FIELD
@@ -269,6 +282,25 @@ int rgref_download(void* p, int which, void* dst, long n) {
 int rgref_apply_env_commands(void* p, const void* cmds, long n) {
     try { static_cast<RefSim*>(p)->sim->apply_env_commands(static_cast<const EnvCommand*>(cmds), static_cast<size_t>(n)); return 0; }
     catch (const std::exception& e) { return Fail(e); }
+}
+
+// BatchSim::set_physics_event_stream with host buffers (the CPU build's step writes host memory): capacity records
+// per env and one window per env; all-null disables.
+int rgref_set_event_stream(void* p, void* records, void* windows, uint32_t capacity, int goal_only) {
+    try {
+        RefSim& r = *static_cast<RefSim*>(p);
+        PhysicsEventStream st;
+        if (records || windows || capacity) {
+            st.records = static_cast<PhysicsEvent*>(records);
+            st.windows = static_cast<PhysicsEventWindow*>(windows);
+            st.capacity_per_env = capacity;
+            st.env_count = static_cast<uint32_t>(r.env_count);
+            st.cars_per_env = static_cast<uint32_t>(r.max_cars);
+            st.goal_only = goal_only != 0;
+        }
+        r.sim->set_physics_event_stream(st);
+        return 0;
+    } catch (const std::exception& e) { return Fail(e); }
 }
 
 int rgref_step(void* p, int ticks) {

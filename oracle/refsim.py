@@ -55,6 +55,7 @@ def lib():
             ("rgref_step_with_commands", [vp, cp, ctypes.c_int, cp, ctypes.c_int, ctypes.c_int]),
             ("rgref_apply_env_commands_device", [vp, cp, lg]),
             ("rgref_env_digests", [vp, cp, cp]),
+            ("rgref_set_event_stream", [vp, cp, cp, ctypes.c_uint32, ctypes.c_int]),
         ]:
             f = getattr(L, name)
             f.argtypes = args
@@ -161,6 +162,14 @@ class RefSim:
     def apply_env_commands_device(self, cmds: np.ndarray):
         cmds = np.ascontiguousarray(cmds, dtype=np.uint8)
         self._check(self._L.rgref_apply_env_commands_device(self.h, _ptr(cmds), cmds.shape[0]))
+
+    def set_event_stream(self, records, windows, capacity: int, goal_only: bool = False):
+        """BatchSim::set_physics_event_stream with host buffers: records [env_count * capacity, 104] u8, windows
+        [env_count, 16] u8 (numpy arrays the caller keeps alive); None/None/0 disables."""
+        self._events = (records, windows)
+        rp = _ptr(records) if records is not None else None
+        wp = _ptr(windows) if windows is not None else None
+        self._check(self._L.rgref_set_event_stream(self.h, rp, wp, capacity, 1 if goal_only else 0))
 
     def step(self, ticks: int = 1):
         self._check(self._L.rgref_step(self.h, ticks))

@@ -33,6 +33,29 @@ ITEMSIZE = {"car_states": 880, "ball_states": 288, "env_states": 44, "boost_pad_
 ENV_COMMAND_DTYPE = np.dtype([("reset_to_kickoff", "u1"), ("reset_scoreboard", "u1"), ("reset_tick_count", "u1"),
                               ("kickoff_seed_mode", "u1"), ("new_round_bodies", "u1"), ("keep_boost_pads", "u1"),
                               ("kickoff_countdown_ticks", "<u2"), ("kickoff_seed", "<u8")])
+# Physics event stream records (PhysicsEvent / PhysicsEventWindow, include/rocketgpu/rocketgpu.h:45-72 of the
+# reference), 104 and 16 bytes; padding bytes are unspecified.
+PHYSICS_EVENT_TYPES = ("BallTouchStart", "BallWorldContact", "CarWorldContact", "BoostPickup", "CarBump", "Goal")
+PHYSICS_EVENT_DTYPE = np.dtype({
+    "names": ["tick", "type", "car_id", "other_car_id", "team", "pad_index", "car_on_ground", "is_big_pad", "is_demo",
+              "position", "normal", "car_velocity", "ball_position", "ball_velocity", "boost_before", "boost_after"],
+    "formats": ["<u8", "u1", "<u4", "<u4", "<i4", "<u4", "u1", "u1", "u1", ("<f4", 3), ("<f4", 3), ("<f4", 3),
+                ("<f4", 3), ("<f4", 3), "<f4", "<f4"],
+    "offsets": [0, 8, 12, 16, 20, 24, 28, 29, 30, 32, 44, 56, 68, 80, 92, 96],
+    "itemsize": 104})
+PHYSICS_EVENT_WINDOW_DTYPE = np.dtype([("tick", "<u8"), ("count", "<u4"), ("overflow", "<u4")])
+
+
+def physics_event_layout() -> dict:
+    """rocketgpu_py.physics_event_layout() (python/rocketgpu_bindings.cpp:77-86 of the reference)."""
+    return {"record_size": PHYSICS_EVENT_DTYPE.itemsize, "window_size": PHYSICS_EVENT_WINDOW_DTYPE.itemsize,
+            "type_offset": PHYSICS_EVENT_DTYPE.fields["type"][1],
+            "velocity_offset": PHYSICS_EVENT_DTYPE.fields["ball_velocity"][1],
+            "count_offset": PHYSICS_EVENT_WINDOW_DTYPE.fields["count"][1],
+            "overflow_offset": PHYSICS_EVENT_WINDOW_DTYPE.fields["overflow"][1],
+            "goal_type": PHYSICS_EVENT_TYPES.index("Goal")}
+
+
 KICKOFF_COUNTDOWN_GAME_TIMERS = 0xFFFF  # kKickoffCountdownGameTimers
 KICKOFF_COUNTDOWN_TICKS = 480           # kKickoffCountdownTicks
 
@@ -124,6 +147,8 @@ def lib():
         L.rg_batch_set_teams.argtypes = [vp, vp, ctypes.c_size_t, vp]
         L.rg_batch_set_car_configs.argtypes = [vp, vp, ctypes.c_size_t, vp]
         L.rg_batch_set_kickoff_settled.argtypes = [vp, ctypes.c_int32]
+        L.rg_batch_set_physics_event_stream.argtypes = [vp, vp, vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                                        ctypes.c_int32]
         L.rg_batch_reset_to_kickoff.argtypes = [vp, ctypes.c_uint64, vp]
         L.rg_batch_apply_env_commands.argtypes = [vp, vp, ctypes.c_size_t, vp]
         L.rg_batch_apply_env_commands_device.argtypes = [vp, vp, ctypes.c_size_t, vp]
@@ -233,6 +258,18 @@ class BatchSim:
         """MutatorConfig::kickoff_settled: a kickoff reset without a countdown puts the ball where the countdown
         leaves it (True, the default) or asleep at its spawn (False)."""
         _check(lib().rg_batch_set_kickoff_settled(self._h, 1 if on else 0))
+
+    def set_physics_event_stream(self, records: int, windows: int, capacity: int, goal_only: bool = False):
+        """BatchSim.set_physics_event_stream(records, windows, capacity, goal_only) (python/rocketgpu_bindings.cpp:
+        832-845 of the reference): device addresses of env_count * capacity PHYSICS_EVENT_DTYPE records and env_count
+        PHYSICS_EVENT_WINDOW_DTYPE windows.  Pass zeros to disable; synchronize before freeing the buffers."""
+        envs = cars = 0
+        if records or windows or capacity:
+            envs = self.env_count
+            cars = self.car_capacity // max(self.env_count, 1)
+        _check(lib().rg_batch_set_physics_event_stream(self._h, ctypes.c_void_p(records or None),
+                                                      ctypes.c_void_p(windows or None), capacity, envs, cars,
+                                                      1 if goal_only else 0))
 
     def reset_to_kickoff(self, seed: int = 0, stream=None):
         _check(lib().rg_batch_reset_to_kickoff(self._h, seed, _stream_ptr(stream)))

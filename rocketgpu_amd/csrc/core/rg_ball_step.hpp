@@ -44,6 +44,54 @@ struct StepParams {
     uint64_t pad_big_mask;  // bit k: pad k is a big pad
 };
 
+// Where a tick's physics events go (PhysicsEventStream::Emit, rocketgpu.h:75-88): this env's window and its
+// capacity records, or nowhere when the stream is off.  `writer` is false on lanes that mirror the lane writing an
+// arena's records (the lanes of a group run the same tick on the same data).
+struct EventSink {
+    PhysicsEventWindow* window = nullptr;
+    PhysicsEvent* records = nullptr;
+    uint32_t capacity = 0;
+    bool goal_only = false;
+    bool writer = true;
+    RG_HD bool on() const { return window != nullptr; }
+    // Events other than goals are wanted (PhysicsEventStream::HasNonGoalEvents).
+    RG_HD bool non_goal() const { return window != nullptr && !goal_only; }
+    // The env pre-tick stamps the window with the env's tick (env_step.hpp:6695-6697).
+    RG_HD void SetTick(uint32_t tick) const {
+        if (window != nullptr && writer) window->tick = tick;
+    }
+    RG_HD void Emit(PhysicsEvent ev) const {
+        if (goal_only && ev.type != PhysicsEventType::Goal) return;
+        if (window == nullptr || !writer) return;
+        ev.tick = window->tick;
+        if (window->count >= capacity) {
+            window->overflow = 1;
+            return;
+        }
+        records[window->count++] = ev;
+    }
+    // A ball-vs-world contact: every mesh triangle the ball touches, with the triangle's own contact normal (before
+    // the internal-edge correction), in the order the contact query finds them (arena_ball.hpp:1927-1936).
+    RG_HD void BallWorldContact(const BallState& ball, const Vec3& normal) const {
+        PhysicsEvent ev;
+        ev.type = PhysicsEventType::BallWorldContact;
+        ev.position = ball.pos;
+        ev.normal = normal;
+        ev.ball_position = ball.pos;
+        ev.ball_velocity = ball.vel;
+        Emit(ev);
+    }
+    // A goal on this tick (env_step.hpp:7025-7032).
+    RG_HD void Goal(const BallState& ball, int32_t team) const {
+        PhysicsEvent ev;
+        ev.type = PhysicsEventType::Goal;
+        ev.team = team;
+        ev.ball_position = ball.pos;
+        ev.ball_velocity = ball.vel;
+        Emit(ev);
+    }
+};
+
 struct BallManifoldPoint {
     Vec3 normal;
     float pen_uu;
@@ -652,7 +700,8 @@ RG_HD void BallBroadphase(ArenaInternal& st, const ArenaMeshView& mesh, BallTick
 // Ball-vs-mesh contacts and their rows (AppendBallWorldSpecialContactRows for meshes only).
 template <typename Walk = SerialWalk>
 RG_HD void BallMeshContacts(BallState& ball, const StepParams& p, ArenaInternal& st, const ArenaMeshView& mesh,
-                            BallTickScratch& t, const Vec3& ext_lin_uu, const Vec3& solver_vel_bt, float dt) {
+                            BallTickScratch& t, const Vec3& ext_lin_uu, const Vec3& solver_vel_bt, float dt,
+                            const EventSink& events) {
     constexpr int kMaxCps = kMaxBallContactPoints;
     BallContactPoint* cps = t.cps;
     int n_cp = 0;
@@ -688,9 +737,9 @@ RG_HD void BallMeshContacts(BallState& ball, const StepParams& p, ArenaInternal&
             BallBasis(ball, bx, by, bz);
             // A leaf's contact point (independent of the other leaves), then its insertion into the manifold (in
             // node order).
-            const auto contact = [&](int32_t local, BallManifoldPoint& np) {
+            const auto contact = [&](int32_t local, BallManifoldPoint& np, Vec3& normal) {
                 const MeshTriangle& tri = mesh.tris[info.tri_start + local];
-                Vec3 normal, point;
+                Vec3 point;
                 float depth, pen_uu;
                 if (!SphereTriangleContact(center, radius_bt, th, tri.a, tri.b, tri.c, normal, point, depth, pen_uu)) {
                     return false;
@@ -708,6 +757,7 @@ RG_HD void BallMeshContacts(BallState& ball, const StepParams& p, ArenaInternal&
             };
             struct LeafContact {
                 BallManifoldPoint np;
+                Vec3 raw_normal;
                 int32_t hit;
                 RG_HD bool keep() const { return hit != 0; }  // commit changes nothing otherwise
             };
@@ -717,15 +767,22 @@ RG_HD void BallMeshContacts(BallState& ball, const StepParams& p, ArenaInternal&
                 mesh.nodes, info.node_start, info.node_start + info.node_count, qmin, qmax, near_leaf,
                 [&](int32_t local) {
                     BallManifoldPoint np;
-                    if (contact(local, np)) InsertManifoldPoint(man.pts, count, np, th);
+                    Vec3 raw_normal;
+                    if (contact(local, np, raw_normal)) {
+                        if (events.non_goal()) events.BallWorldContact(ball, raw_normal);
+                        InsertManifoldPoint(man.pts, count, np, th);
+                    }
                 },
                 [&](int32_t local) {
                     LeafContact c;
-                    c.hit = contact(local, c.np) ? 1 : 0;
+                    c.hit = contact(local, c.np, c.raw_normal) ? 1 : 0;
                     return c;
                 },
                 [&](const LeafContact& c) {
-                    if (c.hit) InsertManifoldPoint(man.pts, count, c.np, th);
+                    if (c.hit) {
+                        if (events.non_goal()) events.BallWorldContact(ball, c.raw_normal);
+                        InsertManifoldPoint(man.pts, count, c.np, th);
+                    }
                 });
         }
         BallPointFull* full = t.full;
@@ -1007,9 +1064,10 @@ RG_HD void CountdownUpdateTimers(EnvState& env, BallState& ball, float frame_dt)
 // ContactWalk answers the contact query's mesh walks, RayWalk the predictive ray's.
 template <typename ContactWalk = SerialWalk, typename RayWalk = ContactWalk>
 RG_HD void StepBallArena(BallState& ball, EnvState& env, ArenaInternal& st, const ArenaMeshView& mesh,
-                         const StepParams& p, BallTickScratch& t) {
+                         const StepParams& p, BallTickScratch& t, const EventSink& events = EventSink{}) {
     const float dt = p.tick_dt;
     // 1. env pre-tick.
+    events.SetTick(env.tick);
     env.goal_scored_this_tick = 0;
     // 2. ball pre-tick.
     BallPreTick(ball, p, t.predict_vel_bt);
@@ -1061,7 +1119,9 @@ RG_HD void StepBallArena(BallState& ball, EnvState& env, ArenaInternal& st, cons
         any_retained = any_retained || (((st.ball_mesh_retained >> mi) & 1u) != 0u && st.ball_mesh[mi].count != 0);
     }
     const bool dispatched = !sleeping || vehicle_woke || any_retained;
-    if (awake_rows && dispatched) BallMeshContacts<ContactWalk>(ball, p, st, mesh, t, ext_lin_uu, solver_vel_bt, dt);
+    if (awake_rows && dispatched) {
+        BallMeshContacts<ContactWalk>(ball, p, st, mesh, t, ext_lin_uu, solver_vel_bt, dt, events);
+    }
     // 3e. predictive rows.
     if (t.predictive_hit) BallPredictiveRows(ball, p, st, t, ext_lin_uu, solver_vel_bt, dt);
     // 3f. solve.
@@ -1106,6 +1166,7 @@ RG_HD void StepBallArena(BallState& ball, EnvState& env, ArenaInternal& st, cons
         }
         env.goal_latched = env.goal_scored_this_tick;
     }
+    if (env.goal_scored_this_tick && events.on()) events.Goal(ball, env.last_scored_team);
     env.frame_ticks += 1;
     if (p.frame_frac >= 0.0f) {
         // The render frame ends on this tick: the countdown timers advance by its DeltaSeconds.

@@ -28,7 +28,23 @@ struct Twin {
     bool kickoff_settled = true;
     bool settled_ready = false;
     BallState settled{};
+    // Physics event stream in host memory (null windows: off).
+    PhysicsEvent* event_records = nullptr;
+    PhysicsEventWindow* event_windows = nullptr;
+    uint32_t event_capacity = 0;
+    bool event_goal_only = false;
 };
+
+core::EventSink EnvEvents(const Twin& t, size_t e) {
+    core::EventSink s;
+    if (t.event_windows != nullptr) {
+        s.window = t.event_windows + e;
+        s.records = t.event_records + e * t.event_capacity;
+        s.capacity = t.event_capacity;
+        s.goal_only = t.event_goal_only;
+    }
+    return s;
+}
 
 BoostPadState* EnvPads(Twin& t, size_t e) {
     return t.pads.empty() ? nullptr : t.pads.data() + e * static_cast<size_t>(t.params.pad_count);
@@ -236,10 +252,19 @@ int rgt_step_with_commands(void* p, const void* commands, int stride, int ticks)
             core::ApplyTickCommandNoCars(t.balls[e], t.envs[e], t.internal[e], EnvPads(t, e), static_cast<int>(e),
                                          c[static_cast<size_t>(k) * static_cast<size_t>(stride) + e], settled,
                                          t.params);
-            core::StepBallArena(t.balls[e], t.envs[e], t.internal[e], t.view, t.params, *scratch);
+            core::StepBallArena(t.balls[e], t.envs[e], t.internal[e], t.view, t.params, *scratch, EnvEvents(t, e));
         }
     }
     return 0;
+}
+
+// The physics event stream in host buffers (capacity records per env, one window per env); null windows: off.
+void rgt_set_event_stream(void* p, void* records, void* windows, uint32_t capacity, int goal_only) {
+    Twin& t = *static_cast<Twin*>(p);
+    t.event_records = static_cast<PhysicsEvent*>(records);
+    t.event_windows = static_cast<PhysicsEventWindow*>(windows);
+    t.event_capacity = capacity;
+    t.event_goal_only = goal_only != 0;
 }
 
 int rgt_step(void* p, int ticks) {
@@ -247,7 +272,7 @@ int rgt_step(void* p, int ticks) {
     auto scratch = std::make_unique<core::BallTickScratch>();
     for (int k = 0; k < ticks; ++k) {
         for (size_t e = 0; e < t.balls.size(); ++e) {
-            core::StepBallArena(t.balls[e], t.envs[e], t.internal[e], t.view, t.params, *scratch);
+            core::StepBallArena(t.balls[e], t.envs[e], t.internal[e], t.view, t.params, *scratch, EnvEvents(t, e));
         }
     }
     return 0;
@@ -276,7 +301,7 @@ int rgt_ball_path(void* p, int samples, int ticks, float goal_y, float* out, int
         float* o = out + e * static_cast<size_t>(samples) * 6;
         for (int s = 0; s < samples; ++s) {
             for (int k = 0; k < ticks && !scored; ++k) {
-                core::StepBallArena(ball, t.envs[e], t.internal[e], t.view, t.params, *scratch);
+                core::StepBallArena(ball, t.envs[e], t.internal[e], t.view, t.params, *scratch, EnvEvents(t, e));
                 if (std::fabs(ball.pos.y) >= goal_y) {
                     scored = true;
                     goal_tick = s * ticks + k + 1;

@@ -64,6 +64,16 @@ private:
     size_t size_ = 0;
 };
 
+// The caller's physics event buffers (PhysicsEventStream, rocketgpu.h:68-89), device addresses.
+struct EventStream {
+    PhysicsEvent* records = nullptr;
+    PhysicsEventWindow* windows = nullptr;
+    uint32_t capacity_per_env = 0;
+    uint32_t env_count = 0;
+    uint32_t cars_per_env = 0;
+    bool goal_only = false;
+};
+
 struct StepArgs {
     BallState* balls;
     EnvState* envs;
@@ -89,7 +99,26 @@ struct StepArgs {
     const BallState* settled;
     const EnvCommand* commands;
     int32_t command_stride;
+    // Physics event stream (null windows: off): env e's window is event_windows[e], its records start at
+    // event_records + e * event_capacity.
+    PhysicsEventWindow* event_windows;
+    PhysicsEvent* event_records;
+    uint32_t event_capacity;
+    uint8_t event_goal_only;
 };
+
+// Env e's event sink; `writer` is false on the lanes that mirror an arena's writing lane.
+__device__ inline core::EventSink EnvEvents(const StepArgs& a, int e, bool writer = true) {
+    core::EventSink s;
+    if (a.event_windows != nullptr) {
+        s.window = a.event_windows + e;
+        s.records = a.event_records + static_cast<size_t>(e) * a.event_capacity;
+        s.capacity = a.event_capacity;
+        s.goal_only = a.event_goal_only != 0;
+        s.writer = writer;
+    }
+    return s;
+}
 
 constexpr int kMaxStepBlock = 256;
 
@@ -142,30 +171,33 @@ __global__ void __launch_bounds__(kMaxStepBlock) StepBallArenasKernel(StepArgs a
     if (e < 0 || e >= a.env_count) return;
     const int n = a.arena_ticks ? a.arena_ticks[i] : ticks;
     const int slot = (static_cast<int>(threadIdx.x) >> 6) * a.arenas_per_wave + lane;
+    const core::EventSink events = EnvEvents(a, e);
     if constexpr (kStage == kStageInPlace || kStage == kStagePrivate) {
         core::BallTickScratch scratch;
         if constexpr (kStage == kStagePrivate) {
             BallState ball = a.balls[e];
             EnvState env = a.envs[e];
             core::ArenaInternal st = a.internal[e];
-            for (int t = 0; t < n; ++t) core::StepBallArena(ball, env, st, a.mesh, a.params, scratch);
+            for (int t = 0; t < n; ++t) core::StepBallArena(ball, env, st, a.mesh, a.params, scratch, events);
             a.internal[e] = st;
             a.envs[e] = env;
             a.balls[e] = ball;
         } else {
             for (int t = 0; t < n; ++t) {
-                core::StepBallArena(a.balls[e], a.envs[e], a.internal[e], a.mesh, a.params, scratch);
+                core::StepBallArena(a.balls[e], a.envs[e], a.internal[e], a.mesh, a.params, scratch, events);
             }
         }
     } else if constexpr (kStage == kStageLdsScratch) {
         core::BallTickScratch& scratch = reinterpret_cast<core::BallTickScratch*>(lds)[slot];
-        for (int t = 0; t < n; ++t) core::StepBallArena(a.balls[e], a.envs[e], a.internal[e], a.mesh, a.params, scratch);
+        for (int t = 0; t < n; ++t) {
+            core::StepBallArena(a.balls[e], a.envs[e], a.internal[e], a.mesh, a.params, scratch, events);
+        }
     } else {
         LdsSlice& sl = reinterpret_cast<LdsSlice*>(lds)[slot];
         sl.ball = a.balls[e];
         sl.env = a.envs[e];
         sl.st = a.internal[e];
-        for (int t = 0; t < n; ++t) core::StepBallArena(sl.ball, sl.env, sl.st, a.mesh, a.params, sl.scratch);
+        for (int t = 0; t < n; ++t) core::StepBallArena(sl.ball, sl.env, sl.st, a.mesh, a.params, sl.scratch, events);
         a.internal[e] = sl.st;
         a.envs[e] = sl.env;
         a.balls[e] = sl.ball;
@@ -190,11 +222,12 @@ __global__ void __launch_bounds__(kMaxStepBlock) StepBallArenasCommandsKernel(St
     if (e >= a.env_count) return;
     core::BallTickScratch scratch;
     BoostPadState* pads = EnvPads(a, e);
+    const core::EventSink events = EnvEvents(a, e);
     for (int t = 0; t < ticks; ++t) {
         const EnvCommand c = a.commands[static_cast<size_t>(t) * a.command_stride + e];
         core::ApplyTickCommandNoCars(a.balls[e], a.envs[e], a.internal[e], pads, e, c, a.settled, a.params);
         if (c.reset_to_kickoff) ClearEnvControls(a, e);
-        core::StepBallArena(a.balls[e], a.envs[e], a.internal[e], a.mesh, a.params, scratch);
+        core::StepBallArena(a.balls[e], a.envs[e], a.internal[e], a.mesh, a.params, scratch, events);
     }
 }
 
@@ -233,10 +266,12 @@ StepBallArenasGroupKernel(StepArgs a, int32_t ticks) {
     core::BallTickScratch private_scratch;
     core::BallTickScratch& scratch =
         kLdsScratch ? reinterpret_cast<core::BallTickScratch*>(lds)[static_cast<int>(threadIdx.x) / G] : private_scratch;
+    // The group's first lane writes its events; the others compute the same ones.
+    const core::EventSink events = EnvEvents(a, e, (static_cast<int>(threadIdx.x) % G) == 0);
     for (int t = 0; t < n; ++t) {
         // The predictive ray's walk is short and stays serial (every lane walks it, with the same loads).
         core::StepBallArena<GroupWalk<G>, core::SerialWalk>(a.balls[e], a.envs[e], a.internal[e], a.mesh, a.params,
-                                                             scratch);
+                                                             scratch, events);
     }
 }
 
@@ -262,9 +297,11 @@ __global__ void __launch_bounds__(64) BallPathKernel(StepArgs a, int32_t samples
     bool scored = fabsf(ball.pos.y) >= goal_y;
     int goal_tick = 0;
     float* o = out + static_cast<size_t>(e) * samples * 6;
+    // The ticks are ordinary steps, events included.
+    const core::EventSink events = EnvEvents(a, e);
     for (int s = 0; s < samples; ++s) {
         for (int k = 0; k < ticks && !scored; ++k) {
-            core::StepBallArena(ball, a.envs[e], a.internal[e], a.mesh, a.params, scratch);
+            core::StepBallArena(ball, a.envs[e], a.internal[e], a.mesh, a.params, scratch, events);
             if (fabsf(ball.pos.y) >= goal_y) {
                 scored = true;
                 goal_tick = s * ticks + k + 1;
@@ -394,6 +431,20 @@ public:
     // MutatorConfig::kickoff_settled (rocketgpu.h:827-832): resets without a countdown put the ball where the
     // game's countdown leaves it.  On by default, as in the reference.
     void SetKickoffSettled(bool on) { kickoff_settled_ = on; }
+
+    // set_physics_event_stream (src/rocketgpu.cpp:2896-2905): all zero disables the stream; otherwise it must cover
+    // the whole batch.  Every step form writes to it from then on (a captured graph is captured again).  Arenas
+    // here are Soccar, the mode the reference requires for detailed events.
+    void SetPhysicsEventStream(const EventStream& st) {
+        const bool disabled = !st.records && !st.windows && !st.capacity_per_env && !st.env_count && !st.cars_per_env;
+        if (!disabled && (!st.records || !st.windows || !st.capacity_per_env ||
+                          st.env_count != static_cast<uint32_t>(cfg_.env_count) ||
+                          st.cars_per_env != static_cast<uint32_t>(cfg_.max_cars_per_env))) {
+            throw InvalidArgument("physics event stream must match the complete batch");
+        }
+        events_ = disabled ? EventStream{} : st;
+        DestroyGraph();
+    }
 
     // reset_to_kickoff (src/rocketgpu.cpp:2085-2133): every env to a new match's kickoff.
     void ResetToKickoff(uint64_t seed, hipStream_t s) {
@@ -692,6 +743,10 @@ private:
         a.controls = controls_.data();
         a.max_cars = cfg_.max_cars_per_env;
         a.settled = kickoff_settled_ && settled_ready_ ? settled_.data() : nullptr;
+        a.event_windows = events_.windows;
+        a.event_records = events_.records;
+        a.event_capacity = events_.capacity_per_env;
+        a.event_goal_only = events_.goal_only ? 1 : 0;
         return a;
     }
 
@@ -732,6 +787,8 @@ private:
         a.balls = settled_.data();
         a.envs = d_env.data();
         a.internal = d_st.data();
+        a.event_windows = nullptr;  // a private arena: no events
+        a.event_records = nullptr;
         a.env_count = 1;
         a.count = 1;
         a.arenas_per_wave = 64;
@@ -823,6 +880,7 @@ private:
     DeviceArray<BallState> settled_;
     bool kickoff_settled_ = true;
     bool settled_ready_ = false;
+    EventStream events_{};
     DeviceArray<uint64_t> digests_;
     int block_ = 64;
     int stage_ = kStageInPlace;
@@ -982,6 +1040,21 @@ int rg_batch_set_car_configs(rg_batch* b, const void* host_configs, size_t count
 
 int rg_batch_set_kickoff_settled(rg_batch* b, int32_t on) {
     return Guard([&] { b->sim->SetKickoffSettled(on != 0); });
+}
+
+int rg_batch_set_physics_event_stream(rg_batch* b, void* device_records, void* device_windows,
+                                      uint32_t capacity_per_env, uint32_t env_count, uint32_t cars_per_env,
+                                      int32_t goal_only) {
+    return Guard([&] {
+        rg::EventStream st;
+        st.records = static_cast<rg::PhysicsEvent*>(device_records);
+        st.windows = static_cast<rg::PhysicsEventWindow*>(device_windows);
+        st.capacity_per_env = capacity_per_env;
+        st.env_count = env_count;
+        st.cars_per_env = cars_per_env;
+        st.goal_only = goal_only != 0;
+        b->sim->SetPhysicsEventStream(st);
+    });
 }
 
 int rg_batch_reset_to_kickoff(rg_batch* b, uint64_t seed, void* stream) {

@@ -53,6 +53,8 @@ def twin():
     T.rgt_reset_to_kickoff.argtypes = [vp, ctypes.c_uint64]
     T.rgt_apply_env_commands.argtypes = [vp, vp, ctypes.c_long]
     T.rgt_step_with_commands.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int]
+    T.rgt_set_event_stream.argtypes = [vp, vp, vp, ctypes.c_uint32, ctypes.c_int]
+    T.rgt_set_event_stream.restype = None
     return T
 
 
