@@ -123,6 +123,18 @@ int rg_batch_set_physics_event_stream(rg_batch* batch, void* device_records, voi
                                       uint32_t capacity_per_env, uint32_t env_count, uint32_t cars_per_env,
                                       int32_t goal_only);
 
+/* Full-state snapshot, the checkpoint form of the reference's Arena::GetFullState / SetFullState / Serialize
+ * (src/rocketsim_compat_dev.cpp:369-447, include/rocketgpu/rocketsim_compat.h:293-325): one flat host blob with a
+ * header naming the batch shape and arena geometry, the public records in the reference's layouts (car states,
+ * controls, car configs, teams, ball states, env states, boost pads, car counts), then this build's internal
+ * per-arena state (broadphase, retained ball manifolds) and the settled kickoff ball.  A loaded snapshot continues
+ * bit for bit as the saved batch would have.  rg_batch_load_state rejects (RG_EINVAL) a blob of another size, shape,
+ * arena or version, one whose digest does not match its data, and one holding an out-of-range index, before writing
+ * anything.  Both calls synchronize `stream`. */
+int rg_batch_state_snapshot_size(const rg_batch* batch, size_t* bytes);
+int rg_batch_save_state(rg_batch* batch, void* host_dst, size_t bytes, void* stream);
+int rg_batch_load_state(rg_batch* batch, const void* host_src, size_t bytes, void* stream);
+
 /* reset_to_kickoff(seed, stream)  (rocketgpu.h:1065): every env to a new match's kickoff. */
 int rg_batch_reset_to_kickoff(rg_batch* batch, uint64_t seed, void* stream);
 /* apply_env_commands(host_commands, count, stream)  (rocketgpu.h:1066): EnvCommand i (16 B records,

@@ -149,6 +149,9 @@ def lib():
         L.rg_batch_set_kickoff_settled.argtypes = [vp, ctypes.c_int32]
         L.rg_batch_set_physics_event_stream.argtypes = [vp, vp, vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
                                                         ctypes.c_int32]
+        L.rg_batch_state_snapshot_size.argtypes = [vp, ctypes.POINTER(ctypes.c_size_t)]
+        L.rg_batch_save_state.argtypes = [vp, vp, ctypes.c_size_t, vp]
+        L.rg_batch_load_state.argtypes = [vp, vp, ctypes.c_size_t, vp]
         L.rg_batch_reset_to_kickoff.argtypes = [vp, ctypes.c_uint64, vp]
         L.rg_batch_apply_env_commands.argtypes = [vp, vp, ctypes.c_size_t, vp]
         L.rg_batch_apply_env_commands_device.argtypes = [vp, vp, ctypes.c_size_t, vp]
@@ -270,6 +273,38 @@ class BatchSim:
         _check(lib().rg_batch_set_physics_event_stream(self._h, ctypes.c_void_p(records or None),
                                                       ctypes.c_void_p(windows or None), capacity, envs, cars,
                                                       1 if goal_only else 0))
+
+    def save_state(self, stream=None) -> bytes:
+        """The batch's full state as one snapshot blob (rg_batch_save_state)."""
+        n = ctypes.c_size_t()
+        _check(lib().rg_batch_state_snapshot_size(self._h, ctypes.byref(n)))
+        buf = ctypes.create_string_buffer(n.value)
+        _check(lib().rg_batch_save_state(self._h, buf, n.value, _stream_ptr(stream)))
+        return buf.raw
+
+    def load_state(self, blob: bytes, stream=None):
+        """Restore a snapshot taken from a batch of the same shape and arena (rg_batch_load_state)."""
+        buf = ctypes.create_string_buffer(bytes(blob), len(blob))
+        _check(lib().rg_batch_load_state(self._h, buf, len(blob), _stream_ptr(stream)))
+
+    def get_full_state(self, arrays=None) -> dict:
+        """Arena::GetFullState(arrays) (src/rocketsim_compat_dev.cpp:369-388 of the reference) over the public
+        arrays: {"array.field": bytes}, each field's bytes for every element in order, fields as state_layouts()
+        lists them."""
+        layouts = state_layouts()
+        names = [a for a in ("car_states", "ball_states", "env_states", "boost_pad_states", "controls")]
+        for a in arrays or ():
+            if a not in names:
+                raise ValueError("GetFullState: unknown array " + a)
+        out = {}
+        for a in names:
+            if arrays and a not in arrays:
+                continue
+            raw = self.download_state_array(a).view(np.uint8).reshape(-1, layouts[a]["itemsize"])
+            for name, fmt, off, shape in layouts[a]["fields"]:
+                size = np.dtype(fmt).itemsize * int(np.prod(shape, dtype=np.int64))
+                out[a + "." + name] = np.ascontiguousarray(raw[:, off:off + size]).tobytes()
+        return out
 
     def reset_to_kickoff(self, seed: int = 0, stream=None):
         _check(lib().rg_batch_reset_to_kickoff(self._h, seed, _stream_ptr(stream)))

@@ -13,6 +13,7 @@
 #include <new>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "batch_common.hpp"
@@ -577,6 +578,171 @@ public:
         Check(hipStreamSynchronize(s), "hipStreamSynchronize");
     }
 
+    // ---- Full-state snapshot (checkpoint / resume) -------------------------------------------------------------
+    // The reference's full-state surface is Arena::GetFullState / SetFullState and Serialize
+    // (src/rocketsim_compat_dev.cpp:369-447, include/rocketgpu/rocketsim_compat.h:293-325): every per-slot and
+    // per-env array, retained manifolds included, so that a restored arena continues bit for bit.  Here the
+    // snapshot is one flat blob: a header naming the batch it came from, then the public records in the
+    // reference's layouts (car states, controls, car configs, teams, ball states, env states, boost pads, car
+    // counts), then this build's own per-arena internal state (broadphase and retained ball manifolds) and the
+    // settled kickoff ball.  A blob loads only into a batch of the same shape and arena geometry; its digest and
+    // every index it holds are checked before anything is written.
+    struct SnapshotHeader {
+        char magic[8];
+        uint32_t version;
+        uint32_t env_count;
+        uint32_t max_cars;
+        uint32_t pad_count;
+        uint32_t mesh_count;
+        uint32_t tri_count;
+        uint32_t node_count;
+        uint32_t internal_bytes;
+        uint64_t geometry_digest;
+        uint64_t body_bytes;
+        uint64_t body_digest;
+        uint8_t kickoff_settled;
+        uint8_t settled_ready;
+        uint8_t reserved[6];
+    };
+    static constexpr char kSnapshotMagic[8] = {'R', 'G', 'S', 'N', 'A', 'P', 'S', 'T'};
+    static constexpr uint32_t kSnapshotVersion = 1;
+
+    size_t SnapshotBodyBytes() const {
+        const size_t cars = static_cast<size_t>(car_capacity());
+        const size_t envs = static_cast<size_t>(cfg_.env_count);
+        return cars * (sizeof(CarState) + sizeof(CarControls) + sizeof(CarConfig) + 1) +
+               envs * (sizeof(BallState) + sizeof(EnvState) + sizeof(int32_t) + sizeof(core::ArenaInternal)) +
+               pads_.size() * sizeof(BoostPadState) + sizeof(BallState);
+    }
+    size_t SnapshotBytes() const { return sizeof(SnapshotHeader) + SnapshotBodyBytes(); }
+
+    // Digest of the arena geometry and step constants, over their fields (padding bytes excluded).
+    uint64_t GeometryDigest() const {
+        uint64_t h = rgw::kFnvOffset;
+        const auto mix = [&h](const auto& v) { h = rgw::Fnv1a(h, &v, sizeof(v)); };
+        for (const core::MeshTriangle& t : mesh_host_.tris) {
+            mix(t.a), mix(t.b), mix(t.c), mix(t.normal), mix(t.min), mix(t.max), mix(t.flags);
+            mix(t.edge_angle_v0v1), mix(t.edge_angle_v1v2), mix(t.edge_angle_v2v0), mix(t.qmin), mix(t.qmax);
+        }
+        for (const core::MeshBvhNode& n : mesh_host_.nodes) mix(n.qmin), mix(n.qmax), mix(n.escape_or_tri);
+        for (const core::MeshInfo& m : mesh_host_.meshes) {
+            mix(m.aabb_min), mix(m.aabb_max), mix(m.q.bvh_min), mix(m.q.bvh_max), mix(m.q.scale);
+            mix(m.mesh_index), mix(m.tri_start), mix(m.tri_count), mix(m.node_start), mix(m.node_count);
+            mix(m.ray_order);
+        }
+        const core::StepParams& p = params_;
+        mix(p.tick_dt), mix(p.gravity_z), mix(p.ball_radius), mix(p.ball_mass), mix(p.ball_drag);
+        mix(p.ball_damping_scale), mix(p.ball_friction), mix(p.ball_restitution), mix(p.ball_max_speed);
+        mix(p.ball_max_ang_speed), mix(p.frame_frac), mix(p.enable_scoring), mix(p.goal_point), mix(p.goal_normal);
+        mix(p.goal_min), mix(p.goal_max), mix(p.ball_spawn), mix(p.pad_count), mix(p.pad_big_mask);
+        return h;
+    }
+
+    void SaveState(void* dst, size_t bytes, hipStream_t s) {
+        if (bytes < SnapshotBytes()) {
+            throw InvalidArgument("snapshot buffer holds " + std::to_string(bytes) + " B; the batch needs " +
+                                  std::to_string(SnapshotBytes()));
+        }
+        if (!dst) throw InvalidArgument("snapshot buffer must not be null");
+        unsigned char* body = static_cast<unsigned char*>(dst) + sizeof(SnapshotHeader);
+        unsigned char* at = body;
+        const auto take = [&](const auto& arr) {
+            using T = typename std::remove_pointer<decltype(arr.data())>::type;
+            arr.Download(reinterpret_cast<T*>(at), arr.size(), s);
+            at += arr.size() * sizeof(T);
+        };
+        take(cars_);
+        take(controls_);
+        take(configs_);
+        take(teams_);
+        take(balls_);
+        take(envs_);
+        take(pads_);
+        take(car_count_);
+        take(internal_);
+        Check(hipStreamSynchronize(s), "hipStreamSynchronize");
+        BallState settled{};
+        if (settled_ready_) {
+            settled_.Download(&settled, 1, s);
+            Check(hipStreamSynchronize(s), "hipStreamSynchronize");
+        }
+        std::memcpy(at, &settled, sizeof(settled));
+        at += sizeof(settled);
+        SnapshotHeader hdr{};
+        std::memcpy(hdr.magic, kSnapshotMagic, sizeof(hdr.magic));
+        FillHeader(hdr);
+        hdr.body_bytes = static_cast<uint64_t>(at - body);
+        hdr.body_digest = rgw::Fnv1a(rgw::kFnvOffset, body, static_cast<size_t>(at - body));
+        hdr.kickoff_settled = kickoff_settled_ ? 1 : 0;
+        hdr.settled_ready = settled_ready_ ? 1 : 0;
+        std::memcpy(dst, &hdr, sizeof(hdr));
+    }
+
+    void LoadState(const void* src, size_t bytes, hipStream_t s) {
+        if (!src) throw InvalidArgument("snapshot must not be null");
+        if (bytes != SnapshotBytes()) {
+            throw InvalidArgument("snapshot is " + std::to_string(bytes) + " B; this batch's snapshots are " +
+                                  std::to_string(SnapshotBytes()));
+        }
+        SnapshotHeader hdr;
+        std::memcpy(&hdr, src, sizeof(hdr));
+        SnapshotHeader want{};
+        FillHeader(want);
+        if (std::memcmp(hdr.magic, kSnapshotMagic, sizeof(hdr.magic)) != 0 || hdr.version != kSnapshotVersion) {
+            throw InvalidArgument("not a batch snapshot of this version");
+        }
+        if (hdr.env_count != want.env_count || hdr.max_cars != want.max_cars || hdr.pad_count != want.pad_count ||
+            hdr.mesh_count != want.mesh_count || hdr.tri_count != want.tri_count || hdr.node_count != want.node_count ||
+            hdr.internal_bytes != want.internal_bytes || hdr.geometry_digest != want.geometry_digest ||
+            hdr.body_bytes != SnapshotBodyBytes()) {
+            throw InvalidArgument("snapshot was taken from a batch of another shape or arena");
+        }
+        const unsigned char* body = static_cast<const unsigned char*>(src) + sizeof(SnapshotHeader);
+        if (rgw::Fnv1a(rgw::kFnvOffset, body, hdr.body_bytes) != hdr.body_digest) {
+            throw InvalidArgument("snapshot digest mismatch (the data is damaged)");
+        }
+        // Section offsets, then the internal records' indices checked before any upload.
+        const size_t cars = static_cast<size_t>(car_capacity()), envs = static_cast<size_t>(cfg_.env_count);
+        const size_t off_internal = cars * (sizeof(CarState) + sizeof(CarControls) + sizeof(CarConfig) + 1) +
+                                    envs * (sizeof(BallState) + sizeof(EnvState) + sizeof(int32_t)) +
+                                    pads_.size() * sizeof(BoostPadState);
+        std::vector<core::ArenaInternal> internal(envs);
+        std::memcpy(static_cast<void*>(internal.data()), body + off_internal, envs * sizeof(core::ArenaInternal));
+        for (size_t e = 0; e < envs; ++e) {
+            if (!InternalRecordValid(internal[e])) {
+                throw InvalidArgument("snapshot holds an invalid internal record for env " + std::to_string(e));
+            }
+        }
+        const size_t off_counts = off_internal - envs * sizeof(int32_t);
+        for (size_t e = 0; e < envs; ++e) {
+            int32_t c;
+            std::memcpy(&c, body + off_counts + e * sizeof(int32_t), sizeof(c));
+            if (c < 0 || c > cfg_.max_cars_per_env) throw InvalidArgument("snapshot holds an invalid car count");
+        }
+        const unsigned char* at = body;
+        const auto put = [&](auto& arr) {
+            using T = typename std::remove_pointer<decltype(arr.data())>::type;
+            arr.Upload(reinterpret_cast<const T*>(at), arr.size(), s);
+            at += arr.size() * sizeof(T);
+        };
+        put(cars_);
+        put(controls_);
+        put(configs_);
+        put(teams_);
+        put(balls_);
+        put(envs_);
+        put(pads_);
+        put(car_count_);
+        put(internal_);
+        kickoff_settled_ = hdr.kickoff_settled != 0;
+        if (hdr.settled_ready) {
+            if (settled_.size() == 0) settled_.Resize(1);
+            settled_.Upload(reinterpret_cast<const BallState*>(at), 1, s);
+        }
+        settled_ready_ = hdr.settled_ready != 0;
+        Check(hipStreamSynchronize(s), "hipStreamSynchronize");
+    }
+
     void Step(int32_t ticks, hipStream_t s) {
         if (ticks < 0) throw InvalidArgument("ticks must be non-negative");
         if (ticks == 0) return;
@@ -714,6 +880,69 @@ public:
     }
 
 private:
+    void FillHeader(SnapshotHeader& h) const {
+        std::memcpy(h.magic, kSnapshotMagic, sizeof(h.magic));
+        h.version = kSnapshotVersion;
+        h.env_count = static_cast<uint32_t>(cfg_.env_count);
+        h.max_cars = static_cast<uint32_t>(cfg_.max_cars_per_env);
+        h.pad_count = static_cast<uint32_t>(params_.pad_count);
+        h.mesh_count = static_cast<uint32_t>(mesh_host_.meshes.size());
+        h.tri_count = static_cast<uint32_t>(mesh_host_.tris.size());
+        h.node_count = static_cast<uint32_t>(mesh_host_.nodes.size());
+        h.internal_bytes = static_cast<uint32_t>(sizeof(core::ArenaInternal));
+        h.geometry_digest = GeometryDigest();
+    }
+
+    // Every count and index of an internal record within the arrays the step indexes with it.
+    bool InternalRecordValid(const core::ArenaInternal& st) const {
+        const int meshes = static_cast<int>(mesh_host_.meshes.size());
+        const core::SweepState& h = st.bp.sweep;
+        if (h.handle_count > core::kSweepMaxHandles || h.static_count > h.handle_count ||
+            h.next_fresh > core::kSweepMaxHandles + 1 || h.free_count > core::kMaxBodies ||
+            h.obj_count > core::kMaxBodies) {
+            return false;
+        }
+        for (int i = 0; i < core::kMaxBodies; ++i) {
+            if (h.free_list[i] > core::kSweepMaxHandles || h.body_handle[i] > core::kSweepMaxHandles ||
+                h.obj[i] >= core::kMaxBodies) {
+                return false;
+            }
+        }
+        for (int a = 0; a < 3; ++a) {
+            for (int i = 0; i < core::kSweepMaxEdges; ++i) {
+                if (h.edge_handle[a][i] > core::kSweepMaxHandles) return false;
+            }
+            for (int k = 0; k <= core::kSweepMaxHandles; ++k) {
+                if (h.min_edge[k][a] >= core::kSweepMaxEdges || h.max_edge[k][a] >= core::kSweepMaxEdges) return false;
+            }
+        }
+        for (int k = 0; k <= core::kSweepMaxHandles; ++k) {
+            if (h.static_id[k] >= meshes || h.static_id[k] < -core::kMaxBodies) return false;
+        }
+        if (st.bp.pair_count > core::kMaxManifoldEntries || st.bp.manifold_count > core::kMaxManifoldEntries ||
+            st.bp.predictive_count > core::kMaxBodies) {
+            return false;
+        }
+        const auto key_ok = [&](uint32_t key) {
+            const int sid = core::KeyStatic(key);
+            return sid == -1 || sid == core::kPredictiveStaticId || (sid >= 0 && sid < meshes);
+        };
+        for (int i = 0; i < st.bp.pair_count; ++i) {
+            if (!key_ok(st.bp.pairs[i])) return false;
+        }
+        for (int i = 0; i < st.bp.manifold_count; ++i) {
+            if (!key_ok(st.bp.manifolds[i])) return false;
+        }
+        for (int i = 0; i < st.bp.predictive_count; ++i) {
+            if (!key_ok(st.bp.predictive[i])) return false;
+        }
+        if (meshes < 16 && (st.ball_mesh_retained >> meshes) != 0) return false;
+        for (int m = 0; m < core::kMaxArenaMeshes; ++m) {
+            if (st.ball_mesh[m].count > 4) return false;
+        }
+        return true;
+    }
+
     void DestroyGraph() {
         if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
         graph_exec_ = nullptr;
@@ -1055,6 +1284,21 @@ int rg_batch_set_physics_event_stream(rg_batch* b, void* device_records, void* d
         st.goal_only = goal_only != 0;
         b->sim->SetPhysicsEventStream(st);
     });
+}
+
+int rg_batch_state_snapshot_size(const rg_batch* b, size_t* bytes) {
+    return Guard([&] {
+        if (!bytes) throw rg::InvalidArgument("bytes must not be null");
+        *bytes = b->sim->SnapshotBytes();
+    });
+}
+
+int rg_batch_save_state(rg_batch* b, void* host_dst, size_t bytes, void* stream) {
+    return Guard([&] { b->sim->SaveState(host_dst, bytes, S(stream)); });
+}
+
+int rg_batch_load_state(rg_batch* b, const void* host_src, size_t bytes, void* stream) {
+    return Guard([&] { b->sim->LoadState(host_src, bytes, S(stream)); });
 }
 
 int rg_batch_reset_to_kickoff(rg_batch* b, uint64_t seed, void* stream) {
