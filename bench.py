@@ -40,8 +40,8 @@ def kernel_name(shape):
     """The step kernel a launch shape runs (rocketgpu_amd/csrc/rg_batch.hip)."""
     block, stage, per_wave = shape
     if stage in (4, 5):
-        return f"StepBallArenasGroupKernel<{64 // per_wave}, {'true' if stage == 5 else 'false'}>"
-    return f"StepBallArenasKernel<{stage}>"
+        return f"StepBallArenasGroupKernel<{64 // per_wave}, {'true' if stage == 5 else 'false'}, false>"
+    return f"StepBallArenasKernel<{stage}, false>"
 
 
 # Algorithmic bytes per arena-tick at n cars: B(n) = 2 * (12588 + 2456 n + 130 n (n - 1)) + 144 n + 4.

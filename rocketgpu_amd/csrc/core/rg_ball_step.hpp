@@ -18,6 +18,8 @@
 // Everything is evaluated in the reference's float operation order; see rg_math.hpp for the arithmetic contract.
 #pragma once
 
+#include <type_traits>
+
 #include "rg_broadphase.hpp"
 
 namespace rg {
@@ -698,7 +700,22 @@ RG_HD void BallBroadphase(ArenaInternal& st, const ArenaMeshView& mesh, BallTick
 }
 
 // Ball-vs-mesh contacts and their rows (AppendBallWorldSpecialContactRows for meshes only).
-template <typename Walk = SerialWalk>
+// A leaf's contact for the cooperative walk: what its commit needs, plus the triangle's own normal when events are
+// written.
+struct LeafContactPlain {
+    BallManifoldPoint np;
+    int32_t hit;
+    RG_HD bool keep() const { return hit != 0; }  // commit changes nothing otherwise
+};
+struct LeafContactWithNormal {
+    BallManifoldPoint np;
+    Vec3 raw_normal;
+    int32_t hit;
+    RG_HD bool keep() const { return hit != 0; }
+};
+
+// kEvents: the caller writes physics events (compiled out otherwise).
+template <typename Walk = SerialWalk, bool kEvents = false>
 RG_HD void BallMeshContacts(BallState& ball, const StepParams& p, ArenaInternal& st, const ArenaMeshView& mesh,
                             BallTickScratch& t, const Vec3& ext_lin_uu, const Vec3& solver_vel_bt, float dt,
                             const EventSink& events) {
@@ -755,12 +772,7 @@ RG_HD void BallMeshContacts(BallState& ball, const StepParams& p, ArenaInternal&
                 np.local_b = on_b;
                 return true;
             };
-            struct LeafContact {
-                BallManifoldPoint np;
-                Vec3 raw_normal;
-                int32_t hit;
-                RG_HD bool keep() const { return hit != 0; }  // commit changes nothing otherwise
-            };
+            using LeafContact = typename std::conditional<kEvents, LeafContactWithNormal, LeafContactPlain>::type;
             constexpr float kLeafReachMarginBt = 0.01f;
             const SphereNearLeaf near_leaf(info.q, center, r_th + kLeafReachMarginBt);
             Walk::Run(
@@ -769,18 +781,22 @@ RG_HD void BallMeshContacts(BallState& ball, const StepParams& p, ArenaInternal&
                     BallManifoldPoint np;
                     Vec3 raw_normal;
                     if (contact(local, np, raw_normal)) {
-                        if (events.non_goal()) events.BallWorldContact(ball, raw_normal);
+                        if (kEvents && events.non_goal()) events.BallWorldContact(ball, raw_normal);
                         InsertManifoldPoint(man.pts, count, np, th);
                     }
                 },
                 [&](int32_t local) {
                     LeafContact c;
-                    c.hit = contact(local, c.np, c.raw_normal) ? 1 : 0;
+                    Vec3 raw_normal;
+                    c.hit = contact(local, c.np, raw_normal) ? 1 : 0;
+                    if constexpr (kEvents) c.raw_normal = raw_normal;
                     return c;
                 },
                 [&](const LeafContact& c) {
                     if (c.hit) {
-                        if (events.non_goal()) events.BallWorldContact(ball, c.raw_normal);
+                        if constexpr (kEvents) {
+                            if (events.non_goal()) events.BallWorldContact(ball, c.raw_normal);
+                        }
                         InsertManifoldPoint(man.pts, count, c.np, th);
                     }
                 });
@@ -1061,13 +1077,14 @@ RG_HD void CountdownUpdateTimers(EnvState& env, BallState& ball, float frame_dt)
     }
 }
 
-// ContactWalk answers the contact query's mesh walks, RayWalk the predictive ray's.
-template <typename ContactWalk = SerialWalk, typename RayWalk = ContactWalk>
+// ContactWalk answers the contact query's mesh walks, RayWalk the predictive ray's.  With kEvents the tick writes
+// its physics events to `events` (an off sink writes nothing); without it event code is compiled out.
+template <typename ContactWalk = SerialWalk, typename RayWalk = ContactWalk, bool kEvents = false>
 RG_HD void StepBallArena(BallState& ball, EnvState& env, ArenaInternal& st, const ArenaMeshView& mesh,
                          const StepParams& p, BallTickScratch& t, const EventSink& events = EventSink{}) {
     const float dt = p.tick_dt;
     // 1. env pre-tick.
-    events.SetTick(env.tick);
+    if (kEvents) events.SetTick(env.tick);
     env.goal_scored_this_tick = 0;
     // 2. ball pre-tick.
     BallPreTick(ball, p, t.predict_vel_bt);
@@ -1120,7 +1137,7 @@ RG_HD void StepBallArena(BallState& ball, EnvState& env, ArenaInternal& st, cons
     }
     const bool dispatched = !sleeping || vehicle_woke || any_retained;
     if (awake_rows && dispatched) {
-        BallMeshContacts<ContactWalk>(ball, p, st, mesh, t, ext_lin_uu, solver_vel_bt, dt, events);
+        BallMeshContacts<ContactWalk, kEvents>(ball, p, st, mesh, t, ext_lin_uu, solver_vel_bt, dt, events);
     }
     // 3e. predictive rows.
     if (t.predictive_hit) BallPredictiveRows(ball, p, st, t, ext_lin_uu, solver_vel_bt, dt);
@@ -1166,7 +1183,7 @@ RG_HD void StepBallArena(BallState& ball, EnvState& env, ArenaInternal& st, cons
         }
         env.goal_latched = env.goal_scored_this_tick;
     }
-    if (env.goal_scored_this_tick && events.on()) events.Goal(ball, env.last_scored_team);
+    if (kEvents && env.goal_scored_this_tick && events.on()) events.Goal(ball, env.last_scored_team);
     env.frame_ticks += 1;
     if (p.frame_frac >= 0.0f) {
         // The render frame ends on this tick: the countdown timers advance by its DeltaSeconds.

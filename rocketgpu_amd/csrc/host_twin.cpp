@@ -83,6 +83,12 @@ const BallState* Settled(Twin& t) {
     return &t.settled;
 }
 
+// A tick with the event code compiled in (an off sink writes nothing).
+void StepWithEvents(BallState& ball, EnvState& env, core::ArenaInternal& st, const Twin& t,
+                    core::BallTickScratch& scratch, const core::EventSink& events) {
+    core::StepBallArena<core::SerialWalk, core::SerialWalk, true>(ball, env, st, t.view, t.params, scratch, events);
+}
+
 thread_local std::string g_err;
 
 }  // namespace
@@ -252,7 +258,7 @@ int rgt_step_with_commands(void* p, const void* commands, int stride, int ticks)
             core::ApplyTickCommandNoCars(t.balls[e], t.envs[e], t.internal[e], EnvPads(t, e), static_cast<int>(e),
                                          c[static_cast<size_t>(k) * static_cast<size_t>(stride) + e], settled,
                                          t.params);
-            core::StepBallArena(t.balls[e], t.envs[e], t.internal[e], t.view, t.params, *scratch, EnvEvents(t, e));
+            StepWithEvents(t.balls[e], t.envs[e], t.internal[e], t, *scratch, EnvEvents(t, e));
         }
     }
     return 0;
@@ -272,7 +278,7 @@ int rgt_step(void* p, int ticks) {
     auto scratch = std::make_unique<core::BallTickScratch>();
     for (int k = 0; k < ticks; ++k) {
         for (size_t e = 0; e < t.balls.size(); ++e) {
-            core::StepBallArena(t.balls[e], t.envs[e], t.internal[e], t.view, t.params, *scratch, EnvEvents(t, e));
+            StepWithEvents(t.balls[e], t.envs[e], t.internal[e], t, *scratch, EnvEvents(t, e));
         }
     }
     return 0;
@@ -301,7 +307,7 @@ int rgt_ball_path(void* p, int samples, int ticks, float goal_y, float* out, int
         float* o = out + e * static_cast<size_t>(samples) * 6;
         for (int s = 0; s < samples; ++s) {
             for (int k = 0; k < ticks && !scored; ++k) {
-                core::StepBallArena(ball, t.envs[e], t.internal[e], t.view, t.params, *scratch, EnvEvents(t, e));
+                StepWithEvents(ball, t.envs[e], t.internal[e], t, *scratch, EnvEvents(t, e));
                 if (std::fabs(ball.pos.y) >= goal_y) {
                     scored = true;
                     goal_tick = s * ticks + k + 1;

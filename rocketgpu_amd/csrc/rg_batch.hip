@@ -121,6 +121,13 @@ __device__ inline core::EventSink EnvEvents(const StepArgs& a, int e, bool write
     return s;
 }
 
+// One tick of a one-lane kernel (serial mesh walks).
+template <bool kEvents>
+__device__ inline void Tick(BallState& ball, EnvState& env, core::ArenaInternal& st, const StepArgs& a,
+                            core::BallTickScratch& scratch, const core::EventSink& events) {
+    core::StepBallArena<core::SerialWalk, core::SerialWalk, kEvents>(ball, env, st, a.mesh, a.params, scratch, events);
+}
+
 constexpr int kMaxStepBlock = 256;
 
 // Where a launch keeps an arena's records while it steps them (the launch shape's `stage`):
@@ -159,8 +166,8 @@ __host__ __device__ constexpr size_t LdsBytesPerLane(int stage) {
                               : (stage == kStageLdsScratch || stage == kStageGroupLds ? sizeof(core::BallTickScratch) : 0);
 }
 
-// One lane per arena; its ticks back to back.
-template <int kStage>
+// One lane per arena; its ticks back to back.  kEvents: the batch has a physics event stream.
+template <int kStage, bool kEvents>
 __global__ void __launch_bounds__(kMaxStepBlock) StepBallArenasKernel(StepArgs a, int32_t ticks) {
     extern __shared__ __align__(16) unsigned char lds[];
     const int g = static_cast<int>(blockIdx.x) * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x);
@@ -179,26 +186,22 @@ __global__ void __launch_bounds__(kMaxStepBlock) StepBallArenasKernel(StepArgs a
             BallState ball = a.balls[e];
             EnvState env = a.envs[e];
             core::ArenaInternal st = a.internal[e];
-            for (int t = 0; t < n; ++t) core::StepBallArena(ball, env, st, a.mesh, a.params, scratch, events);
+            for (int t = 0; t < n; ++t) Tick<kEvents>(ball, env, st, a, scratch, events);
             a.internal[e] = st;
             a.envs[e] = env;
             a.balls[e] = ball;
         } else {
-            for (int t = 0; t < n; ++t) {
-                core::StepBallArena(a.balls[e], a.envs[e], a.internal[e], a.mesh, a.params, scratch, events);
-            }
+            for (int t = 0; t < n; ++t) Tick<kEvents>(a.balls[e], a.envs[e], a.internal[e], a, scratch, events);
         }
     } else if constexpr (kStage == kStageLdsScratch) {
         core::BallTickScratch& scratch = reinterpret_cast<core::BallTickScratch*>(lds)[slot];
-        for (int t = 0; t < n; ++t) {
-            core::StepBallArena(a.balls[e], a.envs[e], a.internal[e], a.mesh, a.params, scratch, events);
-        }
+        for (int t = 0; t < n; ++t) Tick<kEvents>(a.balls[e], a.envs[e], a.internal[e], a, scratch, events);
     } else {
         LdsSlice& sl = reinterpret_cast<LdsSlice*>(lds)[slot];
         sl.ball = a.balls[e];
         sl.env = a.envs[e];
         sl.st = a.internal[e];
-        for (int t = 0; t < n; ++t) core::StepBallArena(sl.ball, sl.env, sl.st, a.mesh, a.params, sl.scratch, events);
+        for (int t = 0; t < n; ++t) Tick<kEvents>(sl.ball, sl.env, sl.st, a, sl.scratch, events);
         a.internal[e] = sl.st;
         a.envs[e] = sl.env;
         a.balls[e] = sl.ball;
@@ -215,6 +218,7 @@ __device__ inline void ClearEnvControls(const StepArgs& a, int e) {
 
 // step_with_device_commands: each tick starts with the env's command for that tick (StepEnvPreTickEnv), then the
 // tick.  Records stay in place; the work mapping is the step kernel's.
+template <bool kEvents>
 __global__ void __launch_bounds__(kMaxStepBlock) StepBallArenasCommandsKernel(StepArgs a, int32_t ticks) {
     const int g = static_cast<int>(blockIdx.x) * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x);
     const int lane = g & 63;
@@ -228,7 +232,7 @@ __global__ void __launch_bounds__(kMaxStepBlock) StepBallArenasCommandsKernel(St
         const EnvCommand c = a.commands[static_cast<size_t>(t) * a.command_stride + e];
         core::ApplyTickCommandNoCars(a.balls[e], a.envs[e], a.internal[e], pads, e, c, a.settled, a.params);
         if (c.reset_to_kickoff) ClearEnvControls(a, e);
-        core::StepBallArena(a.balls[e], a.envs[e], a.internal[e], a.mesh, a.params, scratch, events);
+        Tick<kEvents>(a.balls[e], a.envs[e], a.internal[e], a, scratch, events);
     }
 }
 
@@ -254,7 +258,7 @@ __global__ void __launch_bounds__(64) ResetToKickoffKernel(StepArgs a, uint64_t 
 // data, so their in-place writes agree.
 // At most 256 registers per lane, so that two waves fit each SIMD (8 arenas per wave fill 8 waves per CU at 16,384
 // arenas).
-template <int G, bool kLdsScratch>
+template <int G, bool kLdsScratch, bool kEvents>
 __global__ void __launch_bounds__(kMaxStepBlock) __attribute__((amdgpu_waves_per_eu(2)))
 StepBallArenasGroupKernel(StepArgs a, int32_t ticks) {
     extern __shared__ __align__(16) unsigned char lds[];
@@ -271,8 +275,8 @@ StepBallArenasGroupKernel(StepArgs a, int32_t ticks) {
     const core::EventSink events = EnvEvents(a, e, (static_cast<int>(threadIdx.x) % G) == 0);
     for (int t = 0; t < n; ++t) {
         // The predictive ray's walk is short and stays serial (every lane walks it, with the same loads).
-        core::StepBallArena<GroupWalk<G>, core::SerialWalk>(a.balls[e], a.envs[e], a.internal[e], a.mesh, a.params,
-                                                             scratch, events);
+        core::StepBallArena<GroupWalk<G>, core::SerialWalk, kEvents>(a.balls[e], a.envs[e], a.internal[e], a.mesh,
+                                                                      a.params, scratch, events);
     }
 }
 
@@ -289,6 +293,7 @@ __device__ inline uint64_t Fnv(uint64_t h, const void* data, int n) {
 // after each `ticks` ticks its pos and vel go to out[arena][sample][6].  Once |pos.y| reaches goal_y the arena is
 // not stepped any more (the remaining samples repeat that state); result[arena] = {goal side (1 +y, 2 -y, 0 none),
 // the tick it crossed on (0 if it started there)}.
+template <bool kEvents>
 __global__ void __launch_bounds__(64) BallPathKernel(StepArgs a, int32_t samples, int32_t ticks, float goal_y,
                                                       float* out, int32_t* result) {
     const int e = static_cast<int>(blockIdx.x) * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x);
@@ -302,7 +307,7 @@ __global__ void __launch_bounds__(64) BallPathKernel(StepArgs a, int32_t samples
     const core::EventSink events = EnvEvents(a, e);
     for (int s = 0; s < samples; ++s) {
         for (int k = 0; k < ticks && !scored; ++k) {
-            core::StepBallArena(ball, a.envs[e], a.internal[e], a.mesh, a.params, scratch, events);
+            Tick<kEvents>(ball, a.envs[e], a.internal[e], a, scratch, events);
             if (fabsf(ball.pos.y) >= goal_y) {
                 scored = true;
                 goal_tick = s * ticks + k + 1;
@@ -506,7 +511,11 @@ public:
         a.arenas_per_wave = per_wave_;
         const int64_t threads = (static_cast<int64_t>(cfg_.env_count) + per_wave_ - 1) / per_wave_ * 64;
         const int grid = static_cast<int>((threads + block_ - 1) / block_);
-        hipLaunchKernelGGL(StepBallArenasCommandsKernel, dim3(grid), dim3(block_), 0, s, a, ticks);
+        if (a.event_windows) {
+            hipLaunchKernelGGL(StepBallArenasCommandsKernel<true>, dim3(grid), dim3(block_), 0, s, a, ticks);
+        } else {
+            hipLaunchKernelGGL(StepBallArenasCommandsKernel<false>, dim3(grid), dim3(block_), 0, s, a, ticks);
+        }
         Check(hipGetLastError(), "StepBallArenasCommandsKernel launch");
         if (car_capacity() > 0) {
             Check(hipMemcpyAsync(controls_.data(), controls + static_cast<size_t>(ticks - 1) * control_stride,
@@ -822,16 +831,8 @@ public:
                                   std::to_string(limit));
         }
         if (lds > 65536) {
-            Check(hipFuncSetAttribute(reinterpret_cast<const void*>(&StepBallArenasKernel<kStageLdsScratch>),
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, limit), "hipFuncSetAttribute");
-            Check(hipFuncSetAttribute(reinterpret_cast<const void*>(&StepBallArenasKernel<kStageLds>),
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, limit), "hipFuncSetAttribute");
-            Check(hipFuncSetAttribute(reinterpret_cast<const void*>(&StepBallArenasGroupKernel<4, true>),
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, limit), "hipFuncSetAttribute");
-            Check(hipFuncSetAttribute(reinterpret_cast<const void*>(&StepBallArenasGroupKernel<8, true>),
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, limit), "hipFuncSetAttribute");
-            Check(hipFuncSetAttribute(reinterpret_cast<const void*>(&StepBallArenasGroupKernel<16, true>),
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, limit), "hipFuncSetAttribute");
+            AllowLargeLds<false>(limit);
+            AllowLargeLds<true>(limit);
         }
     }
     int block() const { return block_; }
@@ -867,7 +868,11 @@ public:
         if (!out || !result) throw InvalidArgument("ball_path output pointers must not be null");
         const StepArgs a = BaseArgs();
         const int grid = (cfg_.env_count + 63) / 64;
-        hipLaunchKernelGGL(BallPathKernel, dim3(grid), dim3(64), 0, s, a, samples, ticks, goal_y, out, result);
+        if (a.event_windows) {
+            hipLaunchKernelGGL(BallPathKernel<true>, dim3(grid), dim3(64), 0, s, a, samples, ticks, goal_y, out, result);
+        } else {
+            hipLaunchKernelGGL(BallPathKernel<false>, dim3(grid), dim3(64), 0, s, a, samples, ticks, goal_y, out, result);
+        }
         Check(hipGetLastError(), "BallPathKernel launch");
     }
 
@@ -1023,7 +1028,7 @@ private:
         a.arenas_per_wave = 64;
         int32_t ticks = core::kKickoffCountdownTicks;
         for (int k = 0;; ++k) {
-            hipLaunchKernelGGL(StepBallArenasKernel<kStageInPlace>, dim3(1), dim3(64), 0, nullptr, a, ticks);
+            hipLaunchKernelGGL((StepBallArenasKernel<kStageInPlace, false>), dim3(1), dim3(64), 0, nullptr, a, ticks);
             Check(hipGetLastError(), "StepBallArenasKernel launch");
             d_env.Download(env.data(), 1, nullptr);
             Check(hipStreamSynchronize(nullptr), "hipStreamSynchronize");
@@ -1040,6 +1045,28 @@ private:
         a.arena_ticks = ticks_list;
         a.count = count;
         a.arenas_per_wave = per_wave_;
+        if (a.event_windows) {
+            LaunchStep<true>(a, count, ticks, s);
+        } else {
+            LaunchStep<false>(a, count, ticks, s);
+        }
+    }
+
+    // The LDS-staged kernels may use more than 64 KB of dynamic LDS.
+    template <bool kEvents>
+    static void AllowLargeLds(int limit) {
+        const void* kernels[] = {reinterpret_cast<const void*>(&StepBallArenasKernel<kStageLdsScratch, kEvents>),
+                                 reinterpret_cast<const void*>(&StepBallArenasKernel<kStageLds, kEvents>),
+                                 reinterpret_cast<const void*>(&StepBallArenasGroupKernel<4, true, kEvents>),
+                                 reinterpret_cast<const void*>(&StepBallArenasGroupKernel<8, true, kEvents>),
+                                 reinterpret_cast<const void*>(&StepBallArenasGroupKernel<16, true, kEvents>)};
+        for (const void* k : kernels) {
+            Check(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, limit), "hipFuncSetAttribute");
+        }
+    }
+
+    template <bool kEvents>
+    void LaunchStep(const StepArgs& a, int32_t count, int32_t ticks, hipStream_t s) {
         if (IsGroupStage(stage_)) {
             // 64 / per_wave lanes per arena.
             const int lanes = 64 / per_wave_;
@@ -1049,16 +1076,19 @@ private:
             const bool in_lds = stage_ == kStageGroupLds;
             switch (lanes) {
             case 4:
-                if (in_lds) hipLaunchKernelGGL((StepBallArenasGroupKernel<4, true>), grid, block, lds, s, a, ticks);
-                else hipLaunchKernelGGL((StepBallArenasGroupKernel<4, false>), grid, block, 0, s, a, ticks);
+                if (in_lds) hipLaunchKernelGGL((StepBallArenasGroupKernel<4, true, kEvents>), grid, block, lds, s, a, ticks);
+                else hipLaunchKernelGGL((StepBallArenasGroupKernel<4, false, kEvents>), grid, block, 0, s, a, ticks);
                 break;
             case 8:
-                if (in_lds) hipLaunchKernelGGL((StepBallArenasGroupKernel<8, true>), grid, block, lds, s, a, ticks);
-                else hipLaunchKernelGGL((StepBallArenasGroupKernel<8, false>), grid, block, 0, s, a, ticks);
+                if (in_lds) hipLaunchKernelGGL((StepBallArenasGroupKernel<8, true, kEvents>), grid, block, lds, s, a, ticks);
+                else hipLaunchKernelGGL((StepBallArenasGroupKernel<8, false, kEvents>), grid, block, 0, s, a, ticks);
                 break;
             case 16:
-                if (in_lds) hipLaunchKernelGGL((StepBallArenasGroupKernel<16, true>), grid, block, lds, s, a, ticks);
-                else hipLaunchKernelGGL((StepBallArenasGroupKernel<16, false>), grid, block, 0, s, a, ticks);
+                if (in_lds) {
+                    hipLaunchKernelGGL((StepBallArenasGroupKernel<16, true, kEvents>), grid, block, lds, s, a, ticks);
+                } else {
+                    hipLaunchKernelGGL((StepBallArenasGroupKernel<16, false, kEvents>), grid, block, 0, s, a, ticks);
+                }
                 break;
             default:
                 throw InvalidArgument("stages 4 and 5 need 16, 8 or 4 arenas per wave (4, 8 or 16 lanes per arena)");
@@ -1071,16 +1101,17 @@ private:
         const size_t lds = LdsBytesPerLane(stage_) * static_cast<size_t>(block_ / 64) * per_wave_;
         switch (stage_) {
         case kStageInPlace:
-            hipLaunchKernelGGL(StepBallArenasKernel<kStageInPlace>, dim3(grid), dim3(block_), 0, s, a, ticks);
+            hipLaunchKernelGGL((StepBallArenasKernel<kStageInPlace, kEvents>), dim3(grid), dim3(block_), 0, s, a, ticks);
             break;
         case kStagePrivate:
-            hipLaunchKernelGGL(StepBallArenasKernel<kStagePrivate>, dim3(grid), dim3(block_), 0, s, a, ticks);
+            hipLaunchKernelGGL((StepBallArenasKernel<kStagePrivate, kEvents>), dim3(grid), dim3(block_), 0, s, a, ticks);
             break;
         case kStageLdsScratch:
-            hipLaunchKernelGGL(StepBallArenasKernel<kStageLdsScratch>, dim3(grid), dim3(block_), lds, s, a, ticks);
+            hipLaunchKernelGGL((StepBallArenasKernel<kStageLdsScratch, kEvents>), dim3(grid), dim3(block_), lds, s, a,
+                               ticks);
             break;
         default:
-            hipLaunchKernelGGL(StepBallArenasKernel<kStageLds>, dim3(grid), dim3(block_), lds, s, a, ticks);
+            hipLaunchKernelGGL((StepBallArenasKernel<kStageLds, kEvents>), dim3(grid), dim3(block_), lds, s, a, ticks);
             break;
         }
         Check(hipGetLastError(), "StepBallArenasKernel launch");
