@@ -291,6 +291,38 @@ struct EnvCommand {
     uint64_t kickoff_seed = 0;
 };
 
+// Per-car-slot wheel and vehicle runtime state carried across ticks (the reference's detail::WheelRuntimeState,
+// include/rocketgpu/detail/step/arena_geometry.hpp:1545-1578; BatchSim::download_wheel_runtime_states,
+// rocketgpu.h:1037), 292 bytes.
+struct WheelRuntimeState {
+    int32_t event_world_ids[4] = {-1, -1, -1, -1};
+    Vec3 event_world_normals[4] = {};
+    float steer_angle[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    float engine_force[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    float brake[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    float lat_friction[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    float long_friction[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    float extra_pushback[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    float solver_time_step = 1.0f / 60.0f;
+    Vec3 accumulated_force = Vec3(0.0f, 0.0f, 0.0f);
+    Vec3 accumulated_ang_accel = Vec3(0.0f, 0.0f, 0.0f);
+    Vec3 accumulated_torque_bt = Vec3(0.0f, 0.0f, 0.0f);
+    Quat pre_solve_rot = Quat(1.0f, 0.0f, 0.0f, 0.0f);
+    Vec3 pre_solve_basis_forward = Vec3(1.0f, 0.0f, 0.0f);
+    Vec3 pre_solve_basis_right = Vec3(0.0f, 1.0f, 0.0f);
+    Vec3 pre_solve_basis_up = Vec3(0.0f, 0.0f, 1.0f);
+    uint8_t pre_solve_basis_authoritative = 0;
+    Vec3 cal_prev_sticky_dir = Vec3(0.0f, 0.0f, 0.0f);
+    Vec3 cal_prev_wheel_normal_sum = Vec3(0.0f, 0.0f, 0.0f);
+    uint8_t cal_prev_world_wheel_contacts = 0;
+    uint8_t cal_game_jumped = 0;
+    uint8_t cal_game_double_jumped = 0;
+    uint8_t cal_cached_ground = 0;
+    float cal_jump_inactive_time = 0.0f;
+    uint8_t cal_frame_open = 0;
+    uint8_t cal_restored = 0;
+};
+
 // Physics event stream (rocketgpu.h:45-98).  The caller owns both buffers: records holds capacity_per_env records
 // per env, windows one PhysicsEventWindow per env.  The step sets each env's window.tick to the env's tick at the
 // start of every tick and appends that tick's events after window.count; the caller clears count and overflow
@@ -330,6 +362,7 @@ static_assert(sizeof(EnvState) == 44, "EnvState layout");
 static_assert(sizeof(BoostPadState) == 16, "BoostPadState layout");
 static_assert(sizeof(EnvCommand) == 16, "EnvCommand layout");
 static_assert(sizeof(PhysicsEvent) == 104, "PhysicsEvent layout");
+static_assert(sizeof(WheelRuntimeState) == 292, "WheelRuntimeState layout");
 static_assert(sizeof(PhysicsEventWindow) == 16, "PhysicsEventWindow layout");
 
 }  // namespace rg

@@ -158,11 +158,136 @@ long rgref_offsetof(int which, const char* field) {
         RGREF_FIELD(CarConfig, dodge_deadzone) RGREF_FIELD(CarConfig, three_wheels)
     }
     if (which == 0) {
-        RGREF_FIELD(CarState, pos) RGREF_FIELD(CarState, rot) RGREF_FIELD(CarState, vel) RGREF_FIELD(CarState, ang_vel)
-        RGREF_FIELD(CarState, boost) RGREF_FIELD(CarState, is_on_ground) RGREF_FIELD(CarState, is_demoed)
-        RGREF_FIELD(CarState, pos_bt) RGREF_FIELD(CarState, vel_bt) RGREF_FIELD(CarState, basis_forward)
-        RGREF_FIELD(CarState, tick_count_since_update) RGREF_FIELD(CarState, ball_hit_info)
-        RGREF_FIELD(CarState, last_controls) RGREF_FIELD(CarState, auto_flip_torque_scale)
+        RGREF_FIELD(CarState, pos)
+        RGREF_FIELD(CarState, rot)
+        RGREF_FIELD(CarState, vel)
+        RGREF_FIELD(CarState, ang_vel)
+        RGREF_FIELD(CarState, flip_rel_torque)
+        RGREF_FIELD(CarState, jump_time)
+        RGREF_FIELD(CarState, flip_time)
+        RGREF_FIELD(CarState, air_time)
+        RGREF_FIELD(CarState, air_time_since_jump)
+        RGREF_FIELD(CarState, boost)
+        RGREF_FIELD(CarState, time_since_boosted)
+        RGREF_FIELD(CarState, boosting_time)
+        RGREF_FIELD(CarState, supersonic_time)
+        RGREF_FIELD(CarState, handbrake_val)
+        RGREF_FIELD(CarState, auto_flip_timer)
+        RGREF_FIELD(CarState, boost_amount)
+        RGREF_FIELD(CarState, demo_respawn_timer)
+        RGREF_FIELD(CarState, tick_count_since_update)
+        RGREF_FIELD(CarState, ball_contact_last_tick)
+        RGREF_FIELD(CarState, ball_contact_this_tick)
+        RGREF_FIELD(CarState, is_on_ground)
+        RGREF_FIELD(CarState, wheels_with_contact)
+        RGREF_FIELD(CarState, has_jumped)
+        RGREF_FIELD(CarState, has_double_jumped)
+        RGREF_FIELD(CarState, has_flipped)
+        RGREF_FIELD(CarState, is_flipping)
+        RGREF_FIELD(CarState, is_jumping)
+        RGREF_FIELD(CarState, is_boosting)
+        RGREF_FIELD(CarState, is_supersonic)
+        RGREF_FIELD(CarState, is_auto_flipping)
+        RGREF_FIELD(CarState, is_demoed)
+        RGREF_FIELD(CarState, just_respawned)
+        RGREF_FIELD(CarState, demo_no_contact_response)
+        RGREF_FIELD(CarState, respawn_slot_override_plus1)
+        RGREF_FIELD(CarState, rlbotv5_boost_hold)
+        if (f == "world_contact.has_contact") return static_cast<long>(offsetof(CarState, world_contact) + offsetof(decltype(CarState::world_contact), has_contact));
+        if (f == "world_contact.wheel_ball_mask") return static_cast<long>(offsetof(CarState, world_contact) + offsetof(decltype(CarState::world_contact), wheel_ball_mask));
+        if (f == "world_contact.reserved1") return static_cast<long>(offsetof(CarState, world_contact) + offsetof(decltype(CarState::world_contact), reserved1));
+        if (f == "world_contact.reserved2") return static_cast<long>(offsetof(CarState, world_contact) + offsetof(decltype(CarState::world_contact), reserved2));
+        if (f == "world_contact.contact_normal") return static_cast<long>(offsetof(CarState, world_contact) + offsetof(decltype(CarState::world_contact), contact_normal));
+        if (f == "car_contact.other_car_id") return static_cast<long>(offsetof(CarState, car_contact) + offsetof(decltype(CarState::car_contact), other_car_id));
+        if (f == "car_contact.cooldown_timer") return static_cast<long>(offsetof(CarState, car_contact) + offsetof(decltype(CarState::car_contact), cooldown_timer));
+        RGREF_FIELD(CarState, vel_impulse_cache)
+        if (f == "ball_hit_info.is_valid") return static_cast<long>(offsetof(CarState, ball_hit_info) + offsetof(decltype(CarState::ball_hit_info), is_valid));
+        if (f == "ball_hit_info.extra_pending") return static_cast<long>(offsetof(CarState, ball_hit_info) + offsetof(decltype(CarState::ball_hit_info), extra_pending));
+        if (f == "ball_hit_info.extra_tick_is_previous") return static_cast<long>(offsetof(CarState, ball_hit_info) + offsetof(decltype(CarState::ball_hit_info), extra_tick_is_previous));
+        if (f == "ball_hit_info.tick_count_when_hit") return static_cast<long>(offsetof(CarState, ball_hit_info) + offsetof(decltype(CarState::ball_hit_info), tick_count_when_hit));
+        if (f == "ball_hit_info.tick_count_when_extra_impulse_applied") return static_cast<long>(offsetof(CarState, ball_hit_info) + offsetof(decltype(CarState::ball_hit_info), tick_count_when_extra_impulse_applied));
+        if (f == "ball_hit_info.hit_car_id") return static_cast<long>(offsetof(CarState, ball_hit_info) + offsetof(decltype(CarState::ball_hit_info), hit_car_id));
+        if (f == "ball_hit_info.relative_pos_on_ball") return static_cast<long>(offsetof(CarState, ball_hit_info) + offsetof(decltype(CarState::ball_hit_info), relative_pos_on_ball));
+        if (f == "ball_hit_info.ball_pos") return static_cast<long>(offsetof(CarState, ball_hit_info) + offsetof(decltype(CarState::ball_hit_info), ball_pos));
+        if (f == "ball_hit_info.extra_hit_vel") return static_cast<long>(offsetof(CarState, ball_hit_info) + offsetof(decltype(CarState::ball_hit_info), extra_hit_vel));
+        if (f == "last_controls.throttle") return static_cast<long>(offsetof(CarState, last_controls) + offsetof(decltype(CarState::last_controls), throttle));
+        if (f == "last_controls.steer") return static_cast<long>(offsetof(CarState, last_controls) + offsetof(decltype(CarState::last_controls), steer));
+        if (f == "last_controls.pitch") return static_cast<long>(offsetof(CarState, last_controls) + offsetof(decltype(CarState::last_controls), pitch));
+        if (f == "last_controls.yaw") return static_cast<long>(offsetof(CarState, last_controls) + offsetof(decltype(CarState::last_controls), yaw));
+        if (f == "last_controls.roll") return static_cast<long>(offsetof(CarState, last_controls) + offsetof(decltype(CarState::last_controls), roll));
+        if (f == "last_controls.jump") return static_cast<long>(offsetof(CarState, last_controls) + offsetof(decltype(CarState::last_controls), jump));
+        if (f == "last_controls.boost") return static_cast<long>(offsetof(CarState, last_controls) + offsetof(decltype(CarState::last_controls), boost));
+        if (f == "last_controls.handbrake") return static_cast<long>(offsetof(CarState, last_controls) + offsetof(decltype(CarState::last_controls), handbrake));
+        RGREF_FIELD(CarState, pos_bt)
+        RGREF_FIELD(CarState, vel_bt)
+        RGREF_FIELD(CarState, tick_start_pos_bt)
+        RGREF_FIELD(CarState, pad_frame_pos_bt)
+        RGREF_FIELD(CarState, vel_touched_solve_stage)
+        RGREF_FIELD(CarState, basis_authoritative)
+        RGREF_FIELD(CarState, ext_force_impulse_bt_valid)
+        RGREF_FIELD(CarState, pad_frame_pos_valid)
+        RGREF_FIELD(CarState, basis_forward)
+        RGREF_FIELD(CarState, basis_right)
+        RGREF_FIELD(CarState, basis_up)
+        RGREF_FIELD(CarState, ext_force_impulse_bt)
+        RGREF_FIELD(CarState, ext_torque_impulse)
+        RGREF_FIELD(CarState, solve_delta_linear_bt)
+        RGREF_FIELD(CarState, vel_impulse_cache_bt)
+        RGREF_FIELD(CarState, solve_delta_angular)
+        RGREF_FIELD(CarState, bump_event_bumps)
+        RGREF_FIELD(CarState, bump_event_demos)
+        RGREF_FIELD(CarState, boost_pad_touching)
+        RGREF_FIELD(CarState, bump_event_reserved1)
+        RGREF_FIELD(CarState, previous_ball_contact_start)
+        RGREF_FIELD(CarState, last_ball_contact_start)
+        RGREF_FIELD(CarState, last_world_surface_contact)
+        RGREF_FIELD(CarState, first_world_after_previous_ball)
+        RGREF_FIELD(CarState, first_world_after_last_ball)
+        RGREF_FIELD(CarState, last_flip_or_jump_start)
+        RGREF_FIELD(CarState, last_directional_flip_start)
+        RGREF_FIELD(CarState, last_ball_started_with_world)
+        RGREF_FIELD(CarState, flip_or_jump_last_tick)
+        RGREF_FIELD(CarState, internal_jump_time)
+        RGREF_FIELD(CarState, internal_flip_time)
+        RGREF_FIELD(CarState, flip_timer_live)
+        RGREF_FIELD(CarState, air_state)
+        RGREF_FIELD(CarState, driving_disabled)
+        RGREF_FIELD(CarState, dodge_timeout)
+        RGREF_FIELD(CarState, dodge_elapsed)
+        RGREF_FIELD(CarState, dodge_dir_x)
+        RGREF_FIELD(CarState, dodge_dir_y)
+        RGREF_FIELD(CarState, latest_touch_tick)
+        RGREF_FIELD(CarState, latest_touch_location)
+        RGREF_FIELD(CarState, latest_touch_normal)
+        RGREF_FIELD(CarState, internal_vel)
+        RGREF_FIELD(CarState, internal_ang_vel)
+        RGREF_FIELD(CarState, reported_vel)
+        RGREF_FIELD(CarState, reported_ang_vel)
+        RGREF_FIELD(CarState, internal_has_jumped)
+        RGREF_FIELD(CarState, internal_has_double_jumped)
+        RGREF_FIELD(CarState, internal_has_flipped)
+        RGREF_FIELD(CarState, internal_is_supersonic)
+        RGREF_FIELD(CarState, reported_flags)
+        RGREF_FIELD(CarState, game_has_jumped)
+        RGREF_FIELD(CarState, double_jump_ticks)
+        RGREF_FIELD(CarState, jump_hold_broken)
+        RGREF_FIELD(CarState, supersonic_set_v3)
+        RGREF_FIELD(CarState, rlbotv5_dodged)
+        RGREF_FIELD(CarState, set_wheel_normals)
+        RGREF_FIELD(CarState, set_wheel_normals_mask)
+        RGREF_FIELD(CarState, cal_pre_flags)
+        RGREF_FIELD(CarState, cal_pre_forward_speed_uu)
+        RGREF_FIELD(CarState, cal_pre_pitchlock_restart_time)
+        RGREF_FIELD(CarState, cal_tick_start_ang_vel)
+        RGREF_FIELD(CarState, tick_start_basis_forward)
+        RGREF_FIELD(CarState, tick_start_basis_right)
+        RGREF_FIELD(CarState, tick_start_basis_up)
+        RGREF_FIELD(CarState, restored_inv_inertia_bt)
+        RGREF_FIELD(CarState, restored_inv_inertia_valid)
+        RGREF_FIELD(CarState, pending_bump_bt)
+        RGREF_FIELD(CarState, pending_bump_hitter)
+        RGREF_FIELD(CarState, pending_bump_count)
+        RGREF_FIELD(CarState, auto_flip_torque_scale)
     } else if (which == 1) {
         RGREF_FIELD(BallState, pos) RGREF_FIELD(BallState, rot) RGREF_FIELD(BallState, vel)
         RGREF_FIELD(BallState, ang_vel) RGREF_FIELD(BallState, pos_bt) RGREF_FIELD(BallState, vel_bt)
@@ -312,6 +437,24 @@ int rgref_step(void* p, int ticks) {
 int rgref_ball_path(void* p, int samples, int ticks, float goal_y, float* out, int32_t* result) {
     try { static_cast<RefSim*>(p)->sim->ball_path(samples, ticks, goal_y, out, result); return 0; }
     catch (const std::exception& e) { return Fail(e); }
+}
+
+// BatchSim::download_wheel_runtime_states / upload_wheel_runtime_states (rocketgpu.h:1037-1039): n records of the
+// reference's detail::WheelRuntimeState (292 bytes each).
+int rgref_download_wheels(void* p, void* dst, long n) {
+    try {
+        static_cast<RefSim*>(p)->sim->download_wheel_runtime_states(static_cast<detail::WheelRuntimeState*>(dst),
+                                                                   static_cast<size_t>(n));
+        return 0;
+    } catch (const std::exception& e) { return Fail(e); }
+}
+
+int rgref_download_car_configs(void* p, void* dst, long n) {
+    try {
+        const DeviceView v = static_cast<RefSim*>(p)->sim->device_view();
+        std::memcpy(dst, v.car_configs, static_cast<size_t>(n) * sizeof(CarConfig));
+        return 0;
+    } catch (const std::exception& e) { return Fail(e); }
 }
 
 int rgref_step_with_controls(void* p, const void* controls_by_tick, int ticks, int stride) {

@@ -56,6 +56,10 @@ def lib():
             ("rgref_apply_env_commands_device", [vp, cp, lg]),
             ("rgref_env_digests", [vp, cp, cp]),
             ("rgref_set_event_stream", [vp, cp, cp, ctypes.c_uint32, ctypes.c_int]),
+            ("rgref_download_wheels", [vp, cp, lg]),
+            ("rgref_download_car_configs", [vp, cp, lg]),
+            ("rgref_car_pretick", [lg, cp, cp, cp, cp]),
+            ("rgref_car_params", [cp]),
         ]:
             f = getattr(L, name)
             f.argtypes = args
@@ -66,6 +70,10 @@ def lib():
         L.rgref_workload_ball_states.argtypes = [ctypes.c_uint64, ctypes.c_long, ctypes.c_long, ctypes.c_double, cp]
         L.rgref_math.argtypes = [ctypes.c_int, cp, cp, cp, ctypes.c_long]
         L.rgref_mesh_tables.argtypes = [vp, cp, cp, cp, cp, cp]
+        L.rgref_quantize_axis.argtypes = [cp, cp, lg]
+        L.rgref_quantize_axis.restype = None
+        L.rgref_default_wheel_state.argtypes = [cp]
+        L.rgref_default_wheel_state.restype = None
         L.rgref_run.restype = ctypes.c_long
         L.rgref_run.argtypes = [vp, cp, ctypes.c_uint64, ctypes.c_float, ctypes.c_int, cp, ctypes.c_int, cp]
         _lib = L
@@ -163,6 +171,17 @@ class RefSim:
         cmds = np.ascontiguousarray(cmds, dtype=np.uint8)
         self._check(self._L.rgref_apply_env_commands_device(self.h, _ptr(cmds), cmds.shape[0]))
 
+    def download_wheels(self, count: int) -> np.ndarray:
+        """BatchSim::download_wheel_runtime_states: [count, 292] u8."""
+        out = np.zeros((count, WHEEL_STATE_BYTES), dtype=np.uint8)
+        self._check(self._L.rgref_download_wheels(self.h, _ptr(out), count))
+        return out
+
+    def download_car_configs(self, count: int) -> np.ndarray:
+        out = np.zeros((count, sizeof(5)), dtype=np.uint8)
+        self._check(self._L.rgref_download_car_configs(self.h, _ptr(out), count))
+        return out
+
     def set_event_stream(self, records, windows, capacity: int, goal_only: bool = False):
         """BatchSim::set_physics_event_stream with host buffers: records [env_count * capacity, 104] u8, windows
         [env_count, 16] u8 (numpy arrays the caller keeps alive); None/None/0 disables."""
@@ -209,3 +228,72 @@ class RefSim:
         if got < 0:
             raise RuntimeError(self._L.rgref_last_error().decode())
         return batch, env[:got]
+
+
+# ----------------------------------------------------------------------------------------------------------------
+# Single phases of the reference step (oracle/ref_phases.cpp).
+
+WHEEL_STATE_BYTES = 292
+CONTROLS_DTYPE = np.dtype([("throttle", "<f4"), ("steer", "<f4"), ("pitch", "<f4"), ("yaw", "<f4"), ("roll", "<f4"),
+                           ("jump", "u1"), ("boost", "u1"), ("handbrake", "u1"), ("pad", "u1")])
+
+
+def car_params() -> np.ndarray:
+    """The reference's default MutatorConfig values the car pre-tick reads (ref_phases.cpp lists the order)."""
+    out = np.zeros(64, np.float32)
+    n = lib().rgref_car_params(_ptr(out))
+    return out[:n]
+
+
+def quantize_axis(x) -> np.ndarray:
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    out = np.zeros_like(x)
+    lib().rgref_quantize_axis(_ptr(x), _ptr(out), x.size)
+    return out
+
+
+def car_pretick(cars, controls, configs, wheels):
+    """StepEnvPreTickCar on independent cars: returns (cars, wheels) after the pre-tick (inputs are not changed)."""
+    c = np.ascontiguousarray(cars, dtype=np.uint8).copy()
+    w = np.ascontiguousarray(wheels, dtype=np.uint8).copy()
+    ctl = np.ascontiguousarray(controls, dtype=np.uint8)
+    cfg = np.ascontiguousarray(configs, dtype=np.uint8)
+    if lib().rgref_car_pretick(c.shape[0], _ptr(c), _ptr(ctl), _ptr(cfg), _ptr(w)) != 0:
+        raise RuntimeError("rgref_car_pretick failed")
+    return c, w
+
+
+def car_arena_inputs(n_env: int, cars_per_env: int, ticks: int, seed: int, p_jump: float = 0.25):
+    """Inputs of the car pre-tick as car arenas meet them: the reference steps n_env arenas of cars_per_env cars
+    (hitbox presets cycling through the six bodies, alternating teams) from a kickoff, with random controls held for
+    1-30 ticks (axes uniform in [-1, 1], jump / boost / handbrake with probability p_jump / 0.3 / 0.05).  Returns
+    the per-tick records before each tick, stacked [ticks * cars]: cars, wheels, controls, configs."""
+    rng = np.random.default_rng(seed)
+    s = RefSim(n_env, cars_per_env)
+    cap = n_env * cars_per_env
+    s.set_env_car_count(np.full(n_env, cars_per_env, np.int32))
+    s.set_teams(np.arange(cap) % 2)
+    s.set_car_presets(np.arange(cap) % 6)
+    s.reset_to_kickoff(seed)
+    configs = s.download_car_configs(cap)
+    ctl = np.zeros(cap, CONTROLS_DTYPE)
+    left = np.zeros(cap, np.int64)
+    out_cars, out_wheels, out_ctl = [], [], []
+    for _ in range(ticks):
+        for i in range(cap):
+            if left[i] <= 0:
+                left[i] = rng.integers(1, 31)
+                for k in ("throttle", "steer", "pitch", "yaw", "roll"):
+                    ctl[i][k] = rng.uniform(-1.0, 1.0)
+                ctl[i]["jump"] = rng.random() < p_jump
+                ctl[i]["boost"] = rng.random() < 0.3
+                ctl[i]["handbrake"] = rng.random() < 0.05
+            left[i] -= 1
+        cb = ctl.view(np.uint8).reshape(cap, 24).copy()
+        out_cars.append(s.download(CAR, cap))
+        out_wheels.append(s.download_wheels(cap))
+        out_ctl.append(cb)
+        s.step_with_controls(cb, 1, cap)
+    s.close()
+    return (np.concatenate(out_cars), np.concatenate(out_wheels), np.concatenate(out_ctl),
+            np.tile(configs, (ticks, 1)))

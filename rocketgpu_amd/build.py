@@ -2,6 +2,7 @@
 
   rocketgpu_amd/_build/librocketgpu_amd.so   the product: gfx950 kernels + BatchSim + C-ABI (hipcc)
   rocketgpu_amd/_build/librg_twin.so         test-only host compile of the same step core (g++)
+  rocketgpu_amd/_build/librg_phases.so       test-only gfx950 kernels of single car phases (hipcc)
   oracle/_ref/librgref.so                    test-only reference CPU build (make -C oracle), when the reference
                                              sources are present
 
@@ -23,6 +24,7 @@ INCLUDE = os.path.join(REPO, "include")
 
 PRODUCT = os.path.join(BUILD, "librocketgpu_amd.so")
 TWIN = os.path.join(BUILD, "librg_twin.so")
+PHASES = os.path.join(BUILD, "librg_phases.so")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("RG_OFFLOAD_ARCH", "gfx950")
@@ -55,6 +57,17 @@ def build_twin():
     os.replace(tmp, TWIN)
 
 
+def build_phases():
+    """Test-only gfx950 kernels of single car phases (checked against the reference's phase functions)."""
+    os.makedirs(BUILD, exist_ok=True)
+    tmp = PHASES + ".tmp"
+    _run([HIPCC, f"--offload-arch={ARCH}", *COMMON,
+          "-fhip-fp32-correctly-rounded-divide-sqrt", "-fno-gpu-flush-denormals-to-zero",
+          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+          "-shared", "-o", tmp, os.path.join(CSRC, "phase_kernels.hip")])
+    os.replace(tmp, PHASES)
+
+
 def build_oracle():
     ref = os.environ.get("REFERENCE_DIR", "/root/reference")
     if not os.path.isdir(ref) or shutil.which("make") is None:
@@ -66,10 +79,12 @@ def build_oracle():
 def build_all():
     build_product()
     build_twin()
+    build_phases()
     build_oracle()
 
 
 if __name__ == "__main__":
     what = sys.argv[1:] or ["all"]
     for w in what:
-        {"all": build_all, "product": build_product, "twin": build_twin, "oracle": build_oracle}[w]()
+        {"all": build_all, "product": build_product, "twin": build_twin, "phases": build_phases,
+         "oracle": build_oracle}[w]()

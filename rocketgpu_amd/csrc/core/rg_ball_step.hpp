@@ -187,23 +187,6 @@ RG_HD Vec3 BallInvInertiaUu(const StepParams& p, const Vec3& t) {
 // ------------------------------------------------------------------------------------------------------------
 // Pre-tick: speed caps and the game's replicated-state quantization (raycast_wheels_and_contact_prep.hpp:3757-3898).
 
-RG_HD float QuantRoundPos(float x, float scale) { return floorf(x * scale + 0.5f) * (1.0f / scale); }
-
-RG_HD float QuantTruncOffset(float x, float scale) {
-    const float inv_scale = 1.0f / scale;
-    const float k = static_cast<float>(static_cast<int32_t>(x * scale));
-    const float rounded = k * inv_scale;
-    const float mag = 0.1f * inv_scale;
-    const float sgn = rounded == 0.0f ? 0.0f : (rounded > 0.0f ? 1.0f : -1.0f);
-    return rounded + sgn * mag;
-}
-
-RG_HD Vec3 ClampLength(const Vec3& v, float max_len) {
-    const float len_sq = LengthSq(v);
-    if (len_sq <= max_len * max_len) return v;
-    return (v * Rsqrt(len_sq)) * max_len;
-}
-
 RG_HD void BallPreTick(BallState& ball, const StepParams& p, Vec3& predict_vel_bt) {
     // Speed caps.
     Vec3 vel_uu = ball.vel_bt * 50.0f;

@@ -1,0 +1,764 @@
+// Car pre-tick: what happens to one car at the start of a tick, before the vehicle update and the world step.
+//
+// The reference's StepEnvPreTickCar (include/rocketgpu/detail/step/env_step.hpp:6751-6873):
+//   1. start: restore the working fields the last report replaced, derive the internal jump/flip timers of a car
+//      whose state was just set, count a demoed car's respawn timer down              env_step.hpp:6751-6816
+//   2. tick-start speed caps and the game's replicated-state quantization         raycast_wheels...:3993-4008
+//   3. finish: the tick-start rigid-body snapshot, the mid-flip z damping, the jump / double jump / dodge / auto-flip
+//      state machine with its impulses, and the body velocity clamp               env_step.hpp:6818-6852, 2981-3120
+//
+// Every expression follows the reference's float operation order (no contraction, correctly rounded division and
+// square root); the arithmetic contract is the one rg_math.hpp states.  Cars that respawn this tick (a demoed car
+// whose timer runs out) and three-wheel bodies are outside this file: CarPreTickSupported() says which cars it
+// handles, and the caller rejects the rest.
+#pragma once
+
+#include "rg_math.hpp"
+
+namespace rg {
+namespace core {
+
+// The MutatorConfig fields the car pre-tick reads (rocketgpu.h:708-815), with the reference's defaults.
+struct CarParams {
+    float tick_dt = 1.0f / 120.0f;
+    float frame_frac = 0.5f;  // boost_pad_frame_frac: negative on ticks inside a render frame
+    float car_mass = 180.0f;
+    float car_max_speed = 2300.0f;
+    float car_max_ang_speed = 5.5f;
+    float supersonic_start_speed = 2200.0f;
+    float supersonic_maintain_min_speed = 2100.0f;
+    float supersonic_maintain_max_time = 1.0f;
+    float jump_immediate_speed = 875.0f / 3.0f;
+    float jump_min_time = 0.0249f;
+    float jump_reset_time_pad = 1.0f / 40.0f;
+    float jump_max_time = 0.20f;
+    float doublejump_max_delay = 1.25f;
+    float flip_torque_time = 0.65f;
+    float flip_initial_vel_scale = 500.0f;
+    float flip_forward_impulse_max_speed_scale = 1.0f;
+    float flip_side_impulse_max_speed_scale = 1.9f;
+    float flip_backward_impulse_max_speed_scale = 2.5f;
+    float flip_backward_impulse_scale_x = 16.0f / 15.0f;
+    float boost_spawn_amount = 33.3f;
+    uint8_t unlimited_flips = 0;
+    uint8_t unlimited_double_jumps = 0;
+};
+
+// ------------------------------------------------------------------------------------------------------------
+// Car body helpers (arena_geometry.hpp:1799-1818).
+
+RG_HD void SetCarPosBt(CarState& car, const Vec3& p) {
+    car.pos_bt = p;
+    car.pos = p * 50.0f;
+}
+RG_HD void SetCarVelBt(CarState& car, const Vec3& v) {
+    car.vel_bt = v;
+    car.vel = v * 50.0f;
+}
+
+// Rotation matrix columns of a quaternion (math.hpp:901-927): |q|^2 summed x, y, z, then w.
+RG_HD void QuatBasis(const Quat& q, Vec3& fwd, Vec3& right, Vec3& up) {
+    const float len_sq = ((q.x * q.x + q.y * q.y) + q.z * q.z) + q.w * q.w;
+    if (len_sq <= 1e-12f) {
+        fwd = Vec3(1.0f, 0.0f, 0.0f);
+        right = Vec3(0.0f, 1.0f, 0.0f);
+        up = Vec3(0.0f, 0.0f, 1.0f);
+        return;
+    }
+    const float s = 2.0f / len_sq;
+    const float xs = q.x * s, ys = q.y * s, zs = q.z * s;
+    const float wx = q.w * xs, wy = q.w * ys, wz = q.w * zs;
+    const float xx = q.x * xs, xy = q.x * ys, xz = q.x * zs;
+    const float yy = q.y * ys, yz = q.y * zs, zz = q.z * zs;
+    fwd = Vec3(1.0f - (yy + zz), xy + wz, xz - wy);
+    right = Vec3(xy - wz, 1.0f - (xx + zz), yz + wx);
+    up = Vec3(xz + wy, yz - wx, 1.0f - (xx + yy));
+}
+
+// The car's orientation columns: the authoritative basis when it has one, else from its quaternion
+// (math.hpp:1392-1400).
+RG_HD void CarBasis(const CarState& car, Vec3& fwd, Vec3& right, Vec3& up) {
+    if (car.basis_authoritative) {
+        fwd = car.basis_forward;
+        right = car.basis_right;
+        up = car.basis_up;
+        return;
+    }
+    QuatBasis(car.rot, fwd, right, up);
+}
+
+// The game's axes of an FQuat (math.hpp:1133-1139), used for the jump direction and the auto-flip.
+RG_HD void UeQuatAxes(const Quat& q, Vec3& fwd, Vec3& right, Vec3& up) {
+    const float z2 = q.z + q.z, y2 = q.y + q.y;
+    const float x2x = (q.x + q.x) * q.x, x2w = (q.x + q.x) * q.w;
+    fwd = Vec3(1.0f - (z2 * q.z + y2 * q.y), z2 * q.w + y2 * q.x, z2 * q.x - y2 * q.w);
+    right = Vec3(y2 * q.x - z2 * q.w, 1.0f - (z2 * q.z + x2x), x2w + z2 * q.y);
+    up = Vec3(y2 * q.w + z2 * q.x, z2 * q.y - x2w, 1.0f - (y2 * q.y + x2x));
+}
+
+// UE3 rotator units: 65536 per turn; the sine table has 16384 entries (math.hpp:1442-1453).
+struct Ue3Rotator {
+    int pitch, yaw, roll;
+};
+RG_HD float Ue3SinTab(int j) {
+    const float a = ((static_cast<float>(j) + static_cast<float>(j)) * 3.14159274f) * 6.103515625e-05f;
+    return UcrtSinf(a);
+}
+RG_HD float Ue3RotatorSin(int a) { return Ue3SinTab((a >> 2) & 16383); }
+RG_HD int Ue3Round(float v) { return static_cast<int>(rintf(v + v + 0.5f)) >> 1; }
+
+// FQuat -> FRotator (math.hpp:1492-1511).
+RG_HD Ue3Rotator Ue3QuatToRotator(const Quat& q) {
+    const float x2 = q.x + q.x, y2 = q.y + q.y, z2 = q.z + q.z;
+    const float xx = q.x * x2, xy = q.x * y2, xz = q.x * z2;
+    const float yy = q.y * y2, yz = q.y * z2, zz = q.z * z2;
+    const float wx = q.w * x2, wy = q.w * y2, wz = q.w * z2;
+    const Vec3 fwd(1.0f - (yy + zz), xy + wz, xz - wy);
+    const Vec3 right(xy - wz, 1.0f - (xx + zz), yz + wx);
+    const Vec3 up(xz + wy, yz - wx, 1.0f - (xx + yy));
+    Ue3Rotator r;
+    r.yaw = Ue3Round(UcrtAtan2f(fwd.y, fwd.x) * 32768.0f / 3.14159274f);
+    const float fwd_xy = static_cast<float>(sqrt(static_cast<double>(fwd.y * fwd.y + fwd.x * fwd.x)));
+    r.pitch = Ue3Round(UcrtAtan2f(fwd.z, fwd_xy) * 32768.0f / 3.14159274f);
+    const float sy = Ue3RotatorSin(r.yaw);
+    const float cy = Ue3RotatorSin(r.yaw + 16384);
+    const float z_dot = (up.y * cy + up.x * -sy) + up.z * -0.0f;
+    const float y_dot = (right.y * cy + right.x * -sy) + right.z * -0.0f;
+    r.roll = Ue3Round(UcrtAtan2f(z_dot, y_dot) * 32768.0f / 3.14159274f);
+    return r;
+}
+
+// Up axis of a rotator (math.hpp:1524-1533).
+RG_HD Vec3 Ue3RotatorUpAxis(const Ue3Rotator& r) {
+    const float sy = Ue3RotatorSin(r.yaw);
+    const float cy = Ue3RotatorSin(r.yaw + 16384);
+    const float sp = Ue3RotatorSin(r.pitch);
+    const float cp = Ue3RotatorSin(r.pitch + 16384);
+    const float sr = Ue3RotatorSin(r.roll);
+    const float cr = Ue3RotatorSin(r.roll + 16384);
+    const float cr_sp = cr * sp;
+    return Vec3(-(cr_sp * cy + sy * sr), cy * sr - cr_sp * sy, cp * cr);
+}
+
+// ------------------------------------------------------------------------------------------------------------
+// Inertia (host_validation_and_contact_debug.hpp:156-174, 232-256, 274-289, 528-558).
+
+// Diagonal inertia of the hitbox as Bullet's btBoxShape computes it, in bt.
+RG_HD Vec3 BoxInertiaDiagBt(const CarConfig& cfg, float mass) {
+    const Vec3 sized = cfg.hitbox_published + Vec3(2.5f, 2.5f, 2.5f);
+    const bool has_published = sized.x == cfg.hitbox_size.x && sized.y == cfg.hitbox_size.y && sized.z == cfg.hitbox_size.z;
+    const Vec3 published = has_published ? cfg.hitbox_published : cfg.hitbox_size - Vec3(2.5f, 2.5f, 2.5f);
+    const Vec3 setup_half = (published * 0.02f) * 0.5f + Vec3(0.025f, 0.025f, 0.025f);
+    const Vec3 box_half = (setup_half * 50.0f) * 0.02f;
+    const float margin = Min(0.04f, 0.1f * Min(Min(box_half.x, box_half.y), box_half.z));
+    const Vec3 margin_v(margin, margin, margin);
+    const Vec3 half = (box_half - margin_v) + margin_v;
+    const float x = half.x * 2.0f, y = half.y * 2.0f, z = half.z * 2.0f;
+    const float x2 = x * x, y2 = y * y, z2 = z * z;
+    const float scale = mass / 12.0f;
+    return Vec3(scale * (y2 + z2), scale * (x2 + z2), scale * (x2 + y2));
+}
+
+RG_HD Vec3 InvInertiaLocalBt(const CarConfig& cfg, float mass) {
+    const Vec3 i = BoxInertiaDiagBt(cfg, mass);
+    return Vec3(i.x != 0.0f ? 1.0f / i.x : 0.0f, i.y != 0.0f ? 1.0f / i.y : 0.0f, i.z != 0.0f ? 1.0f / i.z : 0.0f);
+}
+
+// B diag(inv) B^T from the basis columns, row by row.
+RG_HD void InvInertiaWorldRows(const Vec3& f, const Vec3& r, const Vec3& u, const Vec3& inv, Vec3& row0, Vec3& row1,
+                               Vec3& row2) {
+    const float b00 = f.x, b01 = r.x, b02 = u.x;
+    const float b10 = f.y, b11 = r.y, b12 = u.y;
+    const float b20 = f.z, b21 = r.z, b22 = u.z;
+    const float s00 = b00 * inv.x, s01 = b01 * inv.y, s02 = b02 * inv.z;
+    const float s10 = b10 * inv.x, s11 = b11 * inv.y, s12 = b12 * inv.z;
+    const float s20 = b20 * inv.x, s21 = b21 * inv.y, s22 = b22 * inv.z;
+    row0 = Vec3(s00 * b00 + s01 * b01 + s02 * b02, s00 * b10 + s01 * b11 + s02 * b12, s00 * b20 + s01 * b21 + s02 * b22);
+    row1 = Vec3(s10 * b00 + s11 * b01 + s12 * b02, s10 * b10 + s11 * b11 + s12 * b12, s10 * b20 + s11 * b21 + s12 * b22);
+    row2 = Vec3(s20 * b00 + s21 * b01 + s22 * b02, s20 * b10 + s21 * b11 + s22 * b12, s20 * b20 + s21 * b21 + s22 * b22);
+}
+
+// Bullet's btMatrix3x3::inverse (cofactors over the determinant; 0 for a singular matrix).
+RG_HD void InverseRows(const Vec3& a, const Vec3& b, const Vec3& c, Vec3& o0, Vec3& o1, Vec3& o2) {
+    const float co_x = b.y * c.z - b.z * c.y;
+    const float co_y = b.z * c.x - b.x * c.z;
+    const float co_z = b.x * c.y - b.y * c.x;
+    const float det = (a.x * co_x + a.y * co_y) + a.z * co_z;
+    const float s = det != 0.0f ? 1.0f / det : 0.0f;
+    o0 = Vec3(co_x * s, (a.z * c.y - a.y * c.z) * s, (a.y * b.z - a.z * b.y) * s);
+    o1 = Vec3(co_y * s, (a.x * c.z - a.z * c.x) * s, (a.z * b.x - a.x * b.z) * s);
+    o2 = Vec3(co_z * s, (a.y * c.x - a.x * c.y) * s, (a.x * b.y - a.y * b.x) * s);
+}
+
+RG_HD Vec3 MulRows(const Vec3& r0, const Vec3& r1, const Vec3& r2, const Vec3& v) {
+    return Vec3(Dot(r0, v), Dot(r1, v), Dot(r2, v));
+}
+
+// The per-tick car frame the reference builds when the jump logic needs it (MakeCarFrame,
+// host_validation_and_contact_debug.hpp:267-296): the orientation columns and the world inverse inertia of the
+// tick-start basis (or the restored one).
+struct CarFrame {
+    Vec3 fwd, right, up;
+    Vec3 inv_row0, inv_row1, inv_row2;
+};
+
+RG_HD CarFrame MakeCarFrame(const CarState& car, const CarConfig& cfg, float mass) {
+    CarFrame f;
+    CarBasis(car, f.fwd, f.right, f.up);
+    if (car.restored_inv_inertia_valid) {
+        f.inv_row0 = car.restored_inv_inertia_bt[0];
+        f.inv_row1 = car.restored_inv_inertia_bt[1];
+        f.inv_row2 = car.restored_inv_inertia_bt[2];
+    } else {
+        InvInertiaWorldRows(car.tick_start_basis_forward, car.tick_start_basis_right, car.tick_start_basis_up,
+                            InvInertiaLocalBt(cfg, mass), f.inv_row0, f.inv_row1, f.inv_row2);
+    }
+    return f;
+}
+
+// ------------------------------------------------------------------------------------------------------------
+// Impulses (host_validation_and_contact_debug.hpp:488-509).
+
+RG_HD void ApplyCentralImpulseBt(CarState& car, const Vec3& impulse_bt, float mass) {
+    const float inv_mass = 1.0f / mass;
+    const bool was_touched = car.vel_touched_solve_stage != 0;
+    SetCarVelBt(car, car.vel_bt + impulse_bt * inv_mass);
+    if (was_touched || car.is_on_ground) car.vel_touched_solve_stage = 1;
+}
+
+// An impulse given as a velocity change in uu, taken through bt (mass multiplied in, then divided out).
+RG_HD void ApplyCentralImpulseUuDeltaV(CarState& car, const Vec3& dv_uu, float mass) {
+    ApplyCentralImpulseBt(car, (dv_uu * (1.0f / 50.0f)) * mass, mass);
+}
+
+// The jump impulse or force along the car's up axis, scale in uu, result in bt.
+RG_HD Vec3 JumpUpScaledBt(const CarState& car, float scale_uu) {
+    Vec3 f, r, u;
+    UeQuatAxes(car.rot, f, r, u);
+    return Vec3((scale_uu * u.x) * 0.02f, (scale_uu * u.y) * 0.02f, (scale_uu * u.z) * 0.02f);
+}
+
+// ------------------------------------------------------------------------------------------------------------
+// Controls (raycast_wheels_and_contact_prep.hpp:2977-3030; env_step.hpp:2961-2978).
+
+// An axis input as the game replicates it: one byte, 128 steps below zero and 127 above.
+RG_HD float QuantizeAxisInput(float x) {
+    const float c = Clamp(x, -1.0f, 1.0f);
+    const float y = c < 0.0f ? Max(c * 128.0f, -128.0f) : Min(c * 127.0f, 127.0f);
+    const float w = ((y + 128.0f) + (y + 128.0f)) + 0.5f;
+    const int byte = (static_cast<int>(rintf(w)) >> 1) & 0xFF;
+    const float s = static_cast<float>(byte) - 128.0f;
+    // s / 127 as one product and one fused correction step (the reference's table, QuantizeAxisByteValue).
+    constexpr float kInv127 = 1.0f / 127.0f;
+    const float q = s * kInv127;
+    return byte < 0x80 ? s * (1.0f / 128.0f) : fmaf(fmaf(-127.0f, q, s), kInv127, q);
+}
+
+// Clamped, quantized axes; `raw_pitch` receives the clamped pitch before quantization.
+RG_HD CarControls QuantizeControls(CarControls c, float* raw_pitch) {
+    c.roll = Clamp(c.roll, -1.0f, 1.0f);
+    c.jump = c.jump ? 1 : 0;
+    c.boost = c.boost ? 1 : 0;
+    c.handbrake = c.handbrake ? 1 : 0;
+    if (raw_pitch != nullptr) *raw_pitch = Clamp(c.pitch, -1.0f, 1.0f);
+    c.throttle = QuantizeAxisInput(c.throttle);
+    c.steer = QuantizeAxisInput(c.steer);
+    c.pitch = QuantizeAxisInput(c.pitch);
+    c.yaw = QuantizeAxisInput(c.yaw);
+    c.roll = QuantizeAxisInput(c.roll);
+    return c;
+}
+
+// The controls the jump logic sees: quantized, and only throttle while driving is disabled.
+RG_HD CarControls PreTickControls(const CarState& car, CarControls c, float& raw_pitch) {
+    c = QuantizeControls(c, &raw_pitch);
+    if (car.driving_disabled) {
+        const float throttle = c.throttle;
+        c = CarControls{};
+        c.throttle = throttle;
+    }
+    return c;
+}
+
+// ------------------------------------------------------------------------------------------------------------
+// Timers (raycast_wheels_and_contact_prep.hpp:3653-3699).
+
+// n ticks of dt summed one at a time (the game accumulates its timers tick by tick).
+RG_HD float AccumulatedTicks(int n, float dt) {
+    if (n > 1024) return static_cast<float>(n) * dt;
+    float t = 0.0f;
+    for (int i = 0; i < n; ++i) t += dt;
+    return t;
+}
+
+RG_HD float FlipRestartTimeOnGround(const CarState& car, const CarParams& m, float dt) {
+    if (car.is_flipping) return 0.0f;
+    if (!car.has_flipped) return car.internal_flip_time;
+    if (car.internal_flip_time < m.flip_torque_time) return 0.0f;
+    int k = static_cast<int>(rintf(car.internal_flip_time / dt)) - static_cast<int>(rintf(m.flip_torque_time / dt));
+    if (k < 1) k = 1;
+    if (k > 64) return static_cast<float>(k) * dt;
+    float t = 0.0f;
+    for (int i = 0; i < k; ++i) t += dt;
+    return t;
+}
+
+// Internal jump/flip timers of a car whose state was just set (tick_count_since_update == 0).
+RG_HD void InternalTimersFromSetState(CarState& car, const CarParams& m, float dt) {
+    car.internal_jump_time = (car.is_jumping || car.has_jumped) ? car.jump_time : 0.0f;
+    const float flip_ticks = rintf(car.flip_time / dt);
+    const int n = flip_ticks >= 0.0f ? static_cast<int>(Min(flip_ticks, 1.0e6f)) : 0;
+    if (car.is_flipping) {
+        car.internal_flip_time = AccumulatedTicks(n > 1 ? n - 1 : 0, dt);
+    } else if (car.has_flipped) {
+        car.internal_flip_time = AccumulatedTicks(n + static_cast<int>(rintf(m.flip_torque_time / dt)), dt);
+    } else {
+        car.internal_flip_time = car.flip_time;
+    }
+}
+
+// The supersonic flag and its maintain timer (arena_ball.hpp:3168-3181).
+RG_HD void UpdateSupersonic(CarState& car, const Vec3& vel_uu, const CarParams& m, float dt) {
+    const float speed_sq = LengthSq(vel_uu);
+    if (speed_sq >= (m.supersonic_start_speed * m.supersonic_start_speed)) {
+        car.is_supersonic = 1;
+        car.supersonic_time = 0.0f;
+    } else if (car.is_supersonic) {
+        car.supersonic_time += dt;
+        if (car.supersonic_time >= m.supersonic_maintain_max_time ||
+            speed_sq < (m.supersonic_maintain_min_speed * m.supersonic_maintain_min_speed)) {
+            car.is_supersonic = 0;
+            car.supersonic_time = 0.0f;
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------------------------------
+// Jump, double jump, dodge and auto-flip (raycast_wheels_and_contact_prep.hpp:3388-3397, 3485-3750).
+
+RG_HD void StartJump(CarState& car, const CarParams& m) {
+    car.is_jumping = 1;
+    car.internal_jump_time = 0.0f;
+    ApplyCentralImpulseBt(car, JumpUpScaledBt(car, m.jump_immediate_speed * m.car_mass), m.car_mass);
+}
+
+// The wheels-down auto-flip can start: touching a surface that faces up while the car's roof faces it.
+RG_HD bool AutoFlipGate(const CarState& car) {
+    return car.world_contact.has_contact && car.world_contact.contact_normal.z > 0.707f &&
+           Dot(car.world_contact.contact_normal, Ue3RotatorUpAxis(Ue3QuatToRotator(car.rot))) < -0.966f;
+}
+
+RG_HD void ApplyAutoFlip(CarState& car, const CarParams& m, float dt, bool starts, const CarFrame* frame) {
+    if (starts) {
+        Vec3 f(1.0f, 0.0f, 0.0f), r(0.0f, 1.0f, 0.0f), u(0.0f, 0.0f, 1.0f);
+        if (frame) {
+            f = frame->fwd, r = frame->right, u = frame->up;
+        } else {
+            CarBasis(car, f, r, u);
+        }
+        const float roll = -UcrtAtan2f(r.z, u.z);
+        const float abs_roll = Abs(roll);
+        constexpr float kFlipTime = 0.4f;
+        car.auto_flip_timer = static_cast<float>(static_cast<double>(kFlipTime) *
+                                                 (static_cast<double>(abs_roll) / 3.14159265358979323846));
+        car.auto_flip_torque_scale = roll > 0.0f ? 1.0f : -1.0f;
+        car.is_auto_flipping = 1;
+        Vec3 af, ar, au;
+        UeQuatAxes(car.rot, af, ar, au);
+        ApplyCentralImpulseUuDeltaV(car, (-au) * 200.0f, m.car_mass);
+    }
+    if (car.is_auto_flipping) {
+        if (car.auto_flip_timer <= 0.0f) {
+            car.is_auto_flipping = 0;
+            car.auto_flip_timer = 0.0f;
+        } else {
+            Vec3 af, ar, au;
+            UeQuatAxes(car.rot, af, ar, au);
+            const Vec3 t_flip = (af * 1000.0f) * car.auto_flip_torque_scale;
+            // Inverse inertia rows, and their inverse (the inertia); the frame always exists on this path.
+            Vec3 i0, i1, i2;
+            InverseRows(frame->inv_row0, frame->inv_row1, frame->inv_row2, i0, i1, i2);
+            car.ang_vel += MulRows(frame->inv_row0, frame->inv_row1, frame->inv_row2, MulRows(i0, i1, i2, t_flip));
+            car.auto_flip_timer -= dt;
+        }
+    }
+}
+
+// A double jump or a dodge; returns whether it was a dodge.
+RG_HD bool TryDoubleJumpOrFlip(CarState& car, const CarControls& controls_in, const CarConfig& cfg,
+                               const CarParams& m, float forward_speed_uu, float raw_pitch) {
+    CarControls c = controls_in;
+    c.pitch = -QuantizeAxisInput(-(raw_pitch <= 1.0f ? raw_pitch : c.pitch));
+    const float input_magnitude = Abs(c.yaw) + Abs(c.pitch) + Abs(c.roll);
+    const bool is_flip_input = input_magnitude >= cfg.dodge_deadzone;
+    const bool fresh = !car.has_double_jumped && !car.has_flipped;
+    const bool can_use = (fresh || (is_flip_input ? m.unlimited_flips : m.unlimited_double_jumps)) &&
+                         !car.is_auto_flipping;
+    if (!can_use) return false;
+    if (!is_flip_input) {
+        ApplyCentralImpulseBt(car, JumpUpScaledBt(car, m.jump_immediate_speed * m.car_mass), m.car_mass);
+        car.has_double_jumped = 1;
+        return false;
+    }
+    car.internal_flip_time = 0.0f;
+    car.flip_timer_live = 1;
+    car.last_directional_flip_start = car.tick_count_since_update;
+    car.has_flipped = 1;
+    car.is_flipping = 1;
+    Vec3 dir(-c.pitch, c.yaw + c.roll, 0.0f);
+    if (Abs(c.yaw + c.roll) < 0.1f && Abs(c.pitch) < 0.1f) {
+        dir = Vec3(0.0f, 0.0f, 0.0f);
+    } else {
+        dir = Normalize(dir, Vec3(0.0f, 0.0f, 0.0f));
+    }
+    car.flip_rel_torque = Vec3(-dir.y, dir.x, 0.0f);
+    if (Abs(dir.x) < 0.1f) dir.x = 0.0f;
+    if (Abs(dir.y) < 0.1f) dir.y = 0.0f;
+    if (LengthSq(dir) > 0.0f) {
+        const float speed_ratio = Abs(forward_speed_uu) / m.car_max_speed;
+        const bool backwards = Abs(forward_speed_uu) < 100.0f ? dir.x < 0.0f
+                                                               : ((dir.x >= 0.0f) != (forward_speed_uu >= 0.0f));
+        const float imp = m.flip_initial_vel_scale * m.car_mass;
+        const float imp_x = backwards ? rintf(imp * m.flip_backward_impulse_scale_x) : imp;
+        Vec3 v0(dir.x * imp_x, dir.y * imp, 0.0f);
+        const float max_scale_x = backwards ? m.flip_backward_impulse_max_speed_scale
+                                            : m.flip_forward_impulse_max_speed_scale;
+        v0.x *= ((max_scale_x - 1.0f) * speed_ratio) + 1.0f;
+        v0.y *= ((m.flip_side_impulse_max_speed_scale - 1.0f) * speed_ratio) + 1.0f;
+        const int yaw = Ue3QuatToRotator(car.rot).yaw;
+        const int yaw_idx = (yaw >> 2) & 16383;
+        const Vec3 fwd2(Ue3SinTab((yaw_idx + 4096) & 16383), Ue3SinTab(yaw_idx), 0.0f);
+        const Vec3 right2(-fwd2.y, fwd2.x, 0.0f);
+        const Vec3 dv = fwd2 * v0.x + right2 * v0.y;
+        ApplyCentralImpulseBt(car, dv * 0.02f, m.car_mass);
+    }
+    return true;
+}
+
+RG_HD void UpdateDoubleJumpOrFlipState(CarState& car, const CarControls& controls, const CarConfig& cfg,
+                                       const CarParams& m, bool jump_pressed, float forward_speed_uu, float dt,
+                                       bool press_as_air, float raw_pitch, bool* flip_started) {
+    const auto reset_on_ground = [&]() {
+        car.has_double_jumped = 0;
+        car.internal_flip_time = FlipRestartTimeOnGround(car, m, dt);
+        car.has_flipped = 0;
+        car.air_time = 0.0f;
+        car.air_time_since_jump = 0.0f;
+    };
+    if (car.is_on_ground && !press_as_air) {
+        reset_on_ground();
+    } else {
+        if (car.is_on_ground) reset_on_ground();
+        car.air_time += dt;
+        if (car.has_jumped && !car.is_jumping) {
+            car.air_time_since_jump += dt;
+        } else {
+            car.air_time_since_jump = 0.0f;
+        }
+        if (jump_pressed && car.air_time_since_jump < m.doublejump_max_delay) {
+            if (TryDoubleJumpOrFlip(car, controls, cfg, m, forward_speed_uu, raw_pitch) && flip_started) {
+                *flip_started = true;
+            }
+            if (car.is_flipping && car.internal_flip_time == 0.0f) car.internal_flip_time = -dt;
+        }
+    }
+    if (car.is_flipping || car.has_flipped || car.flip_timer_live || car.internal_flip_time != 0.0f) {
+        car.internal_flip_time += dt;
+    }
+}
+
+// ------------------------------------------------------------------------------------------------------------
+// Tick-start caps and quantization (raycast_wheels_and_contact_prep.hpp:3752-3755, 3917-3960, 3993-4008).
+
+RG_HD void QuantizeCarStateV3(CarState& car, const Vec3& vel_uu) {
+    const uint8_t touched = car.vel_touched_solve_stage;
+    const Vec3 p = car.pos_bt * 50.0f;
+    const Vec3 qp(QuantRoundPos(p.x, 100.0f), QuantRoundPos(p.y, 100.0f), QuantRoundPos(p.z, 100.0f));
+    SetCarPosBt(car, qp * (1.0f / 50.0f));
+    car.pos = qp;
+    const Vec3 q(QuantTruncOffset(vel_uu.x, 100.0f), QuantTruncOffset(vel_uu.y, 100.0f),
+                 QuantTruncOffset(vel_uu.z, 100.0f));
+    SetCarVelBt(car, q * (1.0f / 50.0f));
+    car.vel = q;
+    car.vel_touched_solve_stage = touched;
+    car.ang_vel = Vec3(QuantTruncOffset(car.ang_vel.x, 10000.0f), QuantTruncOffset(car.ang_vel.y, 10000.0f),
+                       QuantTruncOffset(car.ang_vel.z, 10000.0f));
+    Vec3 f, r, u;
+    CarBasis(car, f, r, u);
+    // The tick's inertia basis is the unquantized one.
+    car.tick_start_basis_forward = f;
+    car.tick_start_basis_right = r;
+    car.tick_start_basis_up = u;
+    car.rot = GameQuantizeColumns(f, r, u);
+    car.basis_forward = f;
+    car.basis_right = r;
+    car.basis_up = u;
+    car.basis_authoritative = 1;
+}
+
+RG_HD void TickStartLimitAndQuantize(CarState& car, const CarParams& m) {
+    if (car.is_demoed) return;
+    Vec3 vel_uu = car.vel_bt * 50.0f;
+    const float len_sq = LengthSq(vel_uu);
+    if (len_sq > m.car_max_speed * m.car_max_speed) {
+        vel_uu = (vel_uu * Rsqrt(len_sq)) * m.car_max_speed;
+        SetCarVelBt(car, vel_uu * (1.0f / 50.0f));
+    }
+    car.ang_vel = ClampLength(car.ang_vel, m.car_max_ang_speed);
+    QuantizeCarStateV3(car, vel_uu);
+}
+
+// Part of the dodge's vertical damping applied before the tick (raycast_wheels_and_contact_prep.hpp:4010-4027).
+RG_HD void FlipZDampPrepass(CarState& car, const CarParams& m, float dt) {
+    if (car.is_demoed || car.is_on_ground || !car.is_flipping) return;
+    const float flip_time = car.internal_flip_time + dt;
+    if (!(flip_time >= 0.15f && flip_time <= m.flip_torque_time)) return;
+    if (car.vel_bt.z < 0.0f || flip_time < 0.21f) {
+        const float dv_uu = (-car.vel.z) * 0.35f;
+        if (Abs(dv_uu) >= 1.0e-4f) {
+            const float inv_mass = 1.0f / m.car_mass;
+            const float impulse = (dv_uu * 0.02f) * (1.0f / inv_mass);
+            const Vec3 v = car.vel_bt;
+            SetCarVelBt(car, Vec3(v.x, v.y, v.z + impulse * inv_mass));
+        }
+    }
+}
+
+// Bullet's per-body velocity clamp (env_step.hpp:2916-2939); bit 0: linear clamped, bit 1: angular clamped.
+RG_HD int BodyDampingClampCar(CarState& car, const CarParams& m) {
+    int clamped = 0;
+    const Vec3 v = car.vel_bt;
+    const float max_lin = m.car_max_speed * 0.02f;
+    const float vsq = (v.x * v.x + v.y * v.y) + v.z * v.z;
+    if (0.0f < max_lin && max_lin * max_lin < vsq) {
+        const float s = 1.0f / sqrtf(vsq);
+        car.vel_bt = Vec3((v.x * s) * max_lin, (v.y * s) * max_lin, (v.z * s) * max_lin);
+        clamped |= 1;
+    }
+    const Vec3 w = car.ang_vel;
+    const float max_ang = m.car_max_ang_speed;
+    const float lsq = (w.x * w.x + w.y * w.y) + w.z * w.z;
+    if (0.0f < max_ang && max_ang * max_ang < lsq) {
+        const float s = 1.0f / sqrtf(lsq);
+        car.ang_vel = Vec3((w.x * s) * max_ang, (w.y * s) * max_ang, (w.z * s) * max_ang);
+        clamped |= 2;
+    }
+    return clamped;
+}
+
+// ------------------------------------------------------------------------------------------------------------
+// The pre-physics jump state machine (PreTickCarPrePhysics, env_step.hpp:2981-3120).
+
+constexpr uint8_t kCalPreAirJumpStarted = 1, kCalPreFlipStarted = 2, kCalPreFlipVelChanged = 4,
+                  kCalPreAirJumpUsedBefore = 8, kCalPreSetV3Tick = 16, kCalPreFlippingAtStart = 32,
+                  kCalPreWasJumping = 64;
+
+// Forward speed of the tick-start rotation applied to a velocity (SnapshotForwardSpeed,
+// raycast_wheels_and_contact_prep.hpp:989-1015): the quaternion's x axis rotated the game's way.
+RG_HD float ForwardSpeed(const Quat& q, const Vec3& v) {
+    const float w2 = q.w + q.w;
+    const float s = q.w * q.w - ((q.y * q.y + q.x * q.x) + q.z * q.z);
+    float d = (1.0f * q.x + 0.0f * q.y) + 0.0f * q.z;
+    d = d + d;
+    const Vec3 fwd(((0.0f * q.y - 0.0f * q.z) * w2 + s * 1.0f) + d * q.x,
+                   ((1.0f * q.z - 0.0f * q.x) * w2 + s * 0.0f) + d * q.y,
+                   ((0.0f * q.x - 1.0f * q.y) * w2 + s * 0.0f) + d * q.z);
+    return (fwd.x * v.x + fwd.y * v.y) + fwd.z * v.z;
+}
+
+RG_HD void PreTickCarPrePhysics(CarState& car, CarControls controls, const CarConfig& cfg, const CarParams& m,
+                                float frame_frac, float dt, WheelRuntimeState* ws, const Quat& start_rot,
+                                const Vec3& start_vel_uu) {
+    car.cal_pre_flags = 0;
+    const uint8_t start_is_flipping = car.is_flipping;
+    const uint8_t start_has_flipped = car.has_flipped;
+    const uint8_t start_has_jumped = car.has_jumped;
+    const uint8_t start_has_double_jumped = car.has_double_jumped;
+    const uint8_t start_is_jumping = car.is_jumping;
+    const uint8_t start_is_on_ground = car.is_on_ground;
+    const float start_internal_flip_time = car.internal_flip_time;
+    const float start_internal_jump_time = car.internal_jump_time;
+    const bool start_last_jump = car.last_controls.jump != 0;
+    const uint64_t start_ticks = car.tick_count_since_update;
+    const int start_wheel_contacts = static_cast<int>(car.wheels_with_contact[0] != 0) +
+                                     static_cast<int>(car.wheels_with_contact[1] != 0) +
+                                     static_cast<int>(car.wheels_with_contact[2] != 0) +
+                                     static_cast<int>(car.wheels_with_contact[3] != 0);
+    car.demo_no_contact_response = car.is_demoed ? 1 : 0;
+    if (car.is_demoed) return;
+    if (car.driving_disabled) controls.jump = 0;
+    const bool set_v3_tick = car.supersonic_set_v3 && start_ticks == 0;
+    if (car.supersonic_set_v3) {
+        car.supersonic_set_v3 = 0;
+    } else {
+        UpdateSupersonic(car, start_vel_uu, m, dt);
+    }
+    car.vel_touched_solve_stage = 0;
+    bool flip_started = false;
+
+    const bool jump_pressed = controls.jump && !start_last_jump;
+    const bool flip_over_starts = jump_pressed && !start_is_flipping && !car.is_auto_flipping && AutoFlipGate(car);
+    const bool flip_over_takes_press = jump_pressed && !start_is_flipping && (car.is_auto_flipping || flip_over_starts);
+    const bool need_frame = car.is_auto_flipping != 0 ||
+                            (jump_pressed && car.world_contact.has_contact && car.world_contact.contact_normal.z > 0.707f);
+    CarFrame frame_storage;
+    const CarFrame* frame = nullptr;
+    if (need_frame) {
+        frame_storage = MakeCarFrame(car, cfg, m.car_mass);
+        frame = &frame_storage;
+    }
+    if (start_ticks == 0 && ws != nullptr && ws->cal_restored) car.flip_timer_live = 1;
+    const float pitchlock_restart_time =
+        (!start_has_flipped && !start_is_flipping && (start_internal_flip_time > 0.0f || car.flip_timer_live))
+            ? start_internal_flip_time
+            : -1.0f;
+
+    bool air_jump_started = false;
+    const bool cal_jump = ws != nullptr;
+    const bool cal_jump_g = cal_jump && start_wheel_contacts >= 3;
+    bool cal_jump_blocked = false;
+    if (cal_jump) {
+        WheelRuntimeState& jw = *ws;
+        if (start_ticks == 0 && !jw.cal_restored) {
+            jw.cal_game_jumped = start_has_jumped ? 1 : 0;
+            jw.cal_game_double_jumped = (start_has_double_jumped || start_has_flipped) && !cal_jump_g ? 1 : 0;
+            jw.cal_cached_ground = cal_jump_g ? 1 : 0;
+            jw.cal_jump_inactive_time = start_is_jumping ? 0.0f : start_internal_jump_time;
+            jw.cal_frame_open = 0;
+        }
+        jw.cal_restored = 0;
+        if (!jw.cal_frame_open) {
+            if (cal_jump_g && !jw.cal_cached_ground) {
+                jw.cal_game_jumped = 0;
+                jw.cal_game_double_jumped = 0;
+            }
+            jw.cal_cached_ground = cal_jump_g ? 1 : 0;
+        }
+        cal_jump_blocked = jw.cal_game_jumped != 0;
+        if (jw.cal_game_jumped && cal_jump_g && !start_is_jumping && jw.cal_jump_inactive_time >= 0.125f) {
+            jw.cal_game_jumped = 0;
+            jw.cal_game_double_jumped = 0;
+            jw.cal_cached_ground = 1;
+        }
+        jw.cal_frame_open = frame_frac < 0.0f ? 1 : 0;
+    }
+    const uint8_t was_jumping = start_is_jumping;
+    const float forward_speed_uu = ForwardSpeed(start_rot, start_vel_uu);
+    if (jump_pressed && start_is_on_ground && !start_is_jumping && !cal_jump_blocked && !flip_over_takes_press) {
+        StartJump(car, m);
+        air_jump_started = true;
+    }
+    const bool air_window_closed =
+        cal_jump && (ws->cal_game_double_jumped ||
+                     (ws->cal_game_jumped && ws->cal_jump_inactive_time > m.doublejump_max_delay));
+    const bool air_jump_pressed = jump_pressed && !(cal_jump && cal_jump_g) && !air_window_closed && !flip_over_takes_press;
+    const bool press_as_air = cal_jump && !cal_jump_g && jump_pressed && !air_window_closed && !flip_over_takes_press;
+    const uint8_t air_jump_used_before = (car.has_double_jumped || car.has_flipped) ? 1 : 0;
+
+    bool flip_vel_changed = false;
+    {
+        const bool try_air_jump = air_jump_pressed && !air_jump_started;
+        float dodge_raw_pitch = 2.0f;
+        if (try_air_jump) controls = PreTickControls(car, controls, dodge_raw_pitch);
+        const float fbw_forward_speed = try_air_jump ? ForwardSpeed(car.rot, car.vel) : 0.0f;
+        const Vec3 before = car.vel_bt;
+        UpdateDoubleJumpOrFlipState(car, controls, cfg, m, try_air_jump, fbw_forward_speed, dt,
+                                    press_as_air && !air_jump_started, dodge_raw_pitch, &flip_started);
+        flip_vel_changed = car.vel_bt.x != before.x || car.vel_bt.y != before.y || car.vel_bt.z != before.z;
+    }
+    ApplyAutoFlip(car, m, dt, flip_over_starts, frame);
+    car.cal_pre_flags = static_cast<uint8_t>(
+        (air_jump_started ? kCalPreAirJumpStarted : 0) | (flip_started ? kCalPreFlipStarted : 0) |
+        (flip_vel_changed ? kCalPreFlipVelChanged : 0) | (air_jump_used_before ? kCalPreAirJumpUsedBefore : 0) |
+        (set_v3_tick ? kCalPreSetV3Tick : 0) | (start_is_flipping ? kCalPreFlippingAtStart : 0) |
+        (was_jumping ? kCalPreWasJumping : 0));
+    car.cal_pre_forward_speed_uu = forward_speed_uu;
+    car.cal_pre_pitchlock_restart_time = pitchlock_restart_time;
+}
+
+// ------------------------------------------------------------------------------------------------------------
+// The car pre-tick (StepEnvPreTickCar).
+
+// Working fields the last report replaced (CarState::RestoreWorkingFields, rocketgpu.h:398-417), or the first-tick
+// derivations of a car whose state was just set (env_step.hpp:5914-5925).
+RG_HD void RestoreWorkingFlag(uint8_t& field, uint8_t internal, uint8_t bit, uint8_t reported, uint8_t& kept) {
+    if ((field != 0) == ((reported & bit) != 0)) {
+        field = internal;
+        kept |= bit;
+    }
+}
+
+RG_HD void RestoreStepCarFields(CarState& car) {
+    if (car.tick_count_since_update != 0) {
+        constexpr uint8_t kVel = 1, kAngVel = 2, kJumped = 4, kDoubleJumped = 8, kFlipped = 16, kSupersonic = 32,
+                          kValid = 128;
+        if (!(car.reported_flags & kValid)) return;
+        uint8_t kept = kValid;
+        if (car.vel.x == car.reported_vel.x && car.vel.y == car.reported_vel.y && car.vel.z == car.reported_vel.z) {
+            car.vel = car.internal_vel;
+            kept |= kVel;
+        }
+        if (car.ang_vel.x == car.reported_ang_vel.x && car.ang_vel.y == car.reported_ang_vel.y &&
+            car.ang_vel.z == car.reported_ang_vel.z) {
+            car.ang_vel = car.internal_ang_vel;
+            kept |= kAngVel;
+        }
+        const uint8_t rf = car.reported_flags;
+        RestoreWorkingFlag(car.has_jumped, car.internal_has_jumped, kJumped, rf, kept);
+        RestoreWorkingFlag(car.has_double_jumped, car.internal_has_double_jumped, kDoubleJumped, rf, kept);
+        RestoreWorkingFlag(car.has_flipped, car.internal_has_flipped, kFlipped, rf, kept);
+        RestoreWorkingFlag(car.is_supersonic, car.internal_is_supersonic, kSupersonic, rf, kept);
+        return;
+    }
+    if (car.supersonic_time > 0.0f) car.is_supersonic = 1;
+    car.internal_has_double_jumped = car.has_double_jumped;
+    car.game_has_jumped = car.has_jumped;
+    car.double_jump_ticks = 255;
+}
+
+// Whether this file steps the car's pre-tick: not a demoed car whose respawn falls on this tick (respawn placement
+// is not built) and not a three-wheel body.
+RG_HD bool CarPreTickSupported(const CarState& car, const CarConfig& cfg, const CarParams& m) {
+    if (cfg.three_wheels) return false;
+    return !(car.is_demoed && !(Max(0.0f, car.demo_respawn_timer - m.tick_dt) > 0.0f));
+}
+
+// One car's pre-tick (StepEnvPreTickCar without the respawn branch).  `ws` may be null (no wheel runtime state).
+RG_HD void CarPreTick(CarState& car, const CarControls& controls, const CarConfig& cfg, const CarParams& m,
+                      WheelRuntimeState* ws) {
+    const float dt = m.tick_dt;
+    // 1. start.
+    car.ball_contact_this_tick = 0;
+    RestoreStepCarFields(car);
+    if (car.tick_count_since_update == 0) InternalTimersFromSetState(car, m, dt);
+    if (car.just_respawned) car.just_respawned = 0;
+    if (car.is_demoed) {
+        car.demo_respawn_timer = Max(0.0f, car.demo_respawn_timer - dt);
+        // The respawn itself (timer at zero) is outside this file.
+        // A kickoff re-creation (respawn_slot_override_plus1 = 5 + spot, rocketgpu.h:527-529) keeps its boost.
+        constexpr int kKickoffSpawnOverridePlus1 = 5;
+        if (!(Max(0.0f, car.demo_respawn_timer - dt) > 0.0f) &&
+            !(car.respawn_slot_override_plus1 >= kKickoffSpawnOverridePlus1 &&
+              car.respawn_slot_override_plus1 < kKickoffSpawnOverridePlus1 + 5)) {
+            car.boost = m.boost_spawn_amount;
+            car.boost_amount = car.boost / 100.0f;
+        }
+    }
+    car.tick_start_pos_bt = car.pos_bt;
+    // 2. caps and quantization.
+    TickStartLimitAndQuantize(car, m);
+    // 3. finish.
+    const Quat start_rot = car.rot;
+    const Vec3 start_vel_uu = car.vel;
+    FlipZDampPrepass(car, m, dt);
+    car.cal_tick_start_ang_vel = car.ang_vel;
+    const Vec3 vel_before = car.vel_bt;
+    PreTickCarPrePhysics(car, controls, cfg, m, m.frame_frac, dt, ws, start_rot, start_vel_uu);
+    if (!car.is_demoed) {
+        const bool impulsed = car.vel_bt.x != vel_before.x || car.vel_bt.y != vel_before.y || car.vel_bt.z != vel_before.z;
+        if ((BodyDampingClampCar(car, m) & 1) && impulsed) SetCarVelBt(car, car.vel_bt);
+    }
+}
+
+}  // namespace core
+}  // namespace rg

@@ -99,6 +99,25 @@ RG_HD Vec3 Normalize(const Vec3& v, const Vec3& fallback = Vec3(0.0f, 0.0f, 0.0f
     if (len_sq <= 1e-12f) return fallback;
     return v * Rsqrt(len_sq);
 }
+// The game's replicated-state quantization (raycast_wheels_and_contact_prep.hpp:3862-3875): positions to 0.01 uu
+// rounded, velocities truncated with a tenth-step offset away from zero; and a length cap.
+RG_HD float QuantRoundPos(float x, float scale) { return floorf(x * scale + 0.5f) * (1.0f / scale); }
+
+RG_HD float QuantTruncOffset(float x, float scale) {
+    const float inv_scale = 1.0f / scale;
+    const float k = static_cast<float>(static_cast<int32_t>(x * scale));
+    const float rounded = k * inv_scale;
+    const float mag = 0.1f * inv_scale;
+    const float sgn = rounded == 0.0f ? 0.0f : (rounded > 0.0f ? 1.0f : -1.0f);
+    return rounded + sgn * mag;
+}
+
+RG_HD Vec3 ClampLength(const Vec3& v, float max_len) {
+    const float len_sq = LengthSq(v);
+    if (len_sq <= max_len * max_len) return v;
+    return (v * Rsqrt(len_sq)) * max_len;
+}
+
 RG_HD Vec3 ComponentsBtFromUu(const Vec3& v) { return Vec3(v.x / 50.0f, v.y / 50.0f, v.z / 50.0f); }
 RG_HD bool SameBits(float a, float b) { return FloatBits(a) == FloatBits(b); }
 RG_HD bool SameBits(const Vec3& a, const Vec3& b) {

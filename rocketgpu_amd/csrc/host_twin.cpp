@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "batch_common.hpp"
+#include "core/rg_car_pretick.hpp"
 #include "core/rg_env_commands.hpp"
 
 using namespace rg;
@@ -262,6 +263,44 @@ int rgt_step_with_commands(void* p, const void* commands, int stride, int ticks)
         }
     }
     return 0;
+}
+
+// The car pre-tick on n independent cars with the default car parameters (wheels may be null).  Returns the number
+// of cars it does not step (respawning or three-wheel); those are left as they were.
+long rgt_car_pretick(long n, void* cars, const void* controls, const void* configs, void* wheels) {
+    const core::CarParams m{};
+    CarState* c = static_cast<CarState*>(cars);
+    const CarControls* ctl = static_cast<const CarControls*>(controls);
+    const CarConfig* cc = static_cast<const CarConfig*>(configs);
+    WheelRuntimeState* w = static_cast<WheelRuntimeState*>(wheels);
+    long skipped = 0;
+    for (long k = 0; k < n; ++k) {
+        if (!core::CarPreTickSupported(c[k], cc[k], m)) {
+            ++skipped;
+            continue;
+        }
+        core::CarPreTick(c[k], ctl[k], cc[k], m, w ? &w[k] : nullptr);
+    }
+    return skipped;
+}
+
+// The game's axis quantization over n inputs.
+void rgt_quantize_axis(const float* x, float* out, long n) {
+    for (long i = 0; i < n; ++i) out[i] = core::QuantizeAxisInput(x[i]);
+}
+
+// The default car parameters in the oracle's rgref_car_params order.
+int rgt_car_params(float* out) {
+    const core::CarParams m{};
+    const float v[] = {m.tick_dt, m.frame_frac, m.car_mass, m.car_max_speed, m.car_max_ang_speed,
+                       m.supersonic_start_speed, m.supersonic_maintain_min_speed, m.supersonic_maintain_max_time,
+                       m.jump_immediate_speed, m.jump_min_time, m.jump_reset_time_pad, m.jump_max_time,
+                       m.doublejump_max_delay, m.flip_torque_time, m.flip_initial_vel_scale,
+                       m.flip_forward_impulse_max_speed_scale, m.flip_side_impulse_max_speed_scale,
+                       m.flip_backward_impulse_max_speed_scale, m.flip_backward_impulse_scale_x, m.boost_spawn_amount,
+                       static_cast<float>(m.unlimited_flips), static_cast<float>(m.unlimited_double_jumps)};
+    std::memcpy(out, v, sizeof(v));
+    return static_cast<int>(sizeof(v) / sizeof(float));
 }
 
 // The physics event stream in host buffers (capacity records per env, one window per env); null windows: off.

@@ -1438,6 +1438,14 @@ int64_t rg_state_layouts_json(char* buf, size_t capacity) {
         j += tmp;
         first_field = false;
     };
+    // A 2-D field (an array of vectors): shape [n, m].
+    const auto field2 = [&](const char* name, const char* type, size_t off, int n, int m) {
+        char tmp[192];
+        std::snprintf(tmp, sizeof(tmp), "%s[\"%s\", \"%s\", %zu, [%d, %d]]", first_field ? "" : ", ", name, type, off,
+                      n, m);
+        j += tmp;
+        first_field = false;
+    };
 #define RG_F(T, f, type, n) field(#f, type, offsetof(T, f), n)
 #define RG_SUB(T, m, S, f, type) field(#m "." #f, type, offsetof(T, m) + offsetof(T::S, f), 0)
     begin("ball_states", sizeof(rg::BallState), true);
@@ -1486,16 +1494,136 @@ int64_t rg_state_layouts_json(char* buf, size_t capacity) {
     RG_F(rg::EnvState, goal_latched, "|u1", 0);
     RG_F(rg::EnvState, respawn_match_order, "|u1", 0);
     begin("car_states", sizeof(rg::CarState), false);
-    RG_F(rg::CarState, pos, "<f4", 3);
-    RG_F(rg::CarState, rot, "<f4", 4);
-    RG_F(rg::CarState, vel, "<f4", 3);
-    RG_F(rg::CarState, ang_vel, "<f4", 3);
-    RG_F(rg::CarState, boost, "<f4", 0);
-    RG_F(rg::CarState, is_on_ground, "|u1", 0);
-    RG_F(rg::CarState, is_demoed, "|u1", 0);
-    RG_F(rg::CarState, pos_bt, "<f4", 3);
-    RG_F(rg::CarState, vel_bt, "<f4", 3);
-    RG_F(rg::CarState, tick_count_since_update, "<u8", 0);
+field("pos", "<f4", offsetof(rg::CarState, pos), 3);
+    field("rot", "<f4", offsetof(rg::CarState, rot), 4);
+    field("vel", "<f4", offsetof(rg::CarState, vel), 3);
+    field("ang_vel", "<f4", offsetof(rg::CarState, ang_vel), 3);
+    field("flip_rel_torque", "<f4", offsetof(rg::CarState, flip_rel_torque), 3);
+    field("jump_time", "<f4", offsetof(rg::CarState, jump_time), 0);
+    field("flip_time", "<f4", offsetof(rg::CarState, flip_time), 0);
+    field("air_time", "<f4", offsetof(rg::CarState, air_time), 0);
+    field("air_time_since_jump", "<f4", offsetof(rg::CarState, air_time_since_jump), 0);
+    field("boost", "<f4", offsetof(rg::CarState, boost), 0);
+    field("time_since_boosted", "<f4", offsetof(rg::CarState, time_since_boosted), 0);
+    field("boosting_time", "<f4", offsetof(rg::CarState, boosting_time), 0);
+    field("supersonic_time", "<f4", offsetof(rg::CarState, supersonic_time), 0);
+    field("handbrake_val", "<f4", offsetof(rg::CarState, handbrake_val), 0);
+    field("auto_flip_timer", "<f4", offsetof(rg::CarState, auto_flip_timer), 0);
+    field("boost_amount", "<f4", offsetof(rg::CarState, boost_amount), 0);
+    field("demo_respawn_timer", "<f4", offsetof(rg::CarState, demo_respawn_timer), 0);
+    field("tick_count_since_update", "<u8", offsetof(rg::CarState, tick_count_since_update), 0);
+    field("ball_contact_last_tick", "|u1", offsetof(rg::CarState, ball_contact_last_tick), 0);
+    field("ball_contact_this_tick", "|u1", offsetof(rg::CarState, ball_contact_this_tick), 0);
+    field("is_on_ground", "|u1", offsetof(rg::CarState, is_on_ground), 0);
+    field("wheels_with_contact", "|u1", offsetof(rg::CarState, wheels_with_contact), 4);
+    field("has_jumped", "|u1", offsetof(rg::CarState, has_jumped), 0);
+    field("has_double_jumped", "|u1", offsetof(rg::CarState, has_double_jumped), 0);
+    field("has_flipped", "|u1", offsetof(rg::CarState, has_flipped), 0);
+    field("is_flipping", "|u1", offsetof(rg::CarState, is_flipping), 0);
+    field("is_jumping", "|u1", offsetof(rg::CarState, is_jumping), 0);
+    field("is_boosting", "|u1", offsetof(rg::CarState, is_boosting), 0);
+    field("is_supersonic", "|u1", offsetof(rg::CarState, is_supersonic), 0);
+    field("is_auto_flipping", "|u1", offsetof(rg::CarState, is_auto_flipping), 0);
+    field("is_demoed", "|u1", offsetof(rg::CarState, is_demoed), 0);
+    field("just_respawned", "|u1", offsetof(rg::CarState, just_respawned), 0);
+    field("demo_no_contact_response", "|u1", offsetof(rg::CarState, demo_no_contact_response), 0);
+    field("respawn_slot_override_plus1", "|u1", offsetof(rg::CarState, respawn_slot_override_plus1), 0);
+    field("rlbotv5_boost_hold", "|u1", offsetof(rg::CarState, rlbotv5_boost_hold), 0);
+    field("world_contact.has_contact", "|u1", offsetof(rg::CarState, world_contact) + offsetof(decltype(rg::CarState::world_contact), has_contact), 0);
+    field("world_contact.wheel_ball_mask", "|u1", offsetof(rg::CarState, world_contact) + offsetof(decltype(rg::CarState::world_contact), wheel_ball_mask), 0);
+    field("world_contact.reserved1", "|u1", offsetof(rg::CarState, world_contact) + offsetof(decltype(rg::CarState::world_contact), reserved1), 0);
+    field("world_contact.reserved2", "|u1", offsetof(rg::CarState, world_contact) + offsetof(decltype(rg::CarState::world_contact), reserved2), 0);
+    field("world_contact.contact_normal", "<f4", offsetof(rg::CarState, world_contact) + offsetof(decltype(rg::CarState::world_contact), contact_normal), 3);
+    field("car_contact.other_car_id", "<u4", offsetof(rg::CarState, car_contact) + offsetof(decltype(rg::CarState::car_contact), other_car_id), 0);
+    field("car_contact.cooldown_timer", "<f4", offsetof(rg::CarState, car_contact) + offsetof(decltype(rg::CarState::car_contact), cooldown_timer), 0);
+    field("vel_impulse_cache", "<f4", offsetof(rg::CarState, vel_impulse_cache), 3);
+    field("ball_hit_info.is_valid", "|u1", offsetof(rg::CarState, ball_hit_info) + offsetof(rg::BallHitInfo, is_valid), 0);
+    field("ball_hit_info.extra_pending", "|u1", offsetof(rg::CarState, ball_hit_info) + offsetof(rg::BallHitInfo, extra_pending), 0);
+    field("ball_hit_info.extra_tick_is_previous", "|u1", offsetof(rg::CarState, ball_hit_info) + offsetof(rg::BallHitInfo, extra_tick_is_previous), 0);
+    field("ball_hit_info.tick_count_when_hit", "<u8", offsetof(rg::CarState, ball_hit_info) + offsetof(rg::BallHitInfo, tick_count_when_hit), 0);
+    field("ball_hit_info.tick_count_when_extra_impulse_applied", "<u8", offsetof(rg::CarState, ball_hit_info) + offsetof(rg::BallHitInfo, tick_count_when_extra_impulse_applied), 0);
+    field("ball_hit_info.hit_car_id", "<u4", offsetof(rg::CarState, ball_hit_info) + offsetof(rg::BallHitInfo, hit_car_id), 0);
+    field("ball_hit_info.relative_pos_on_ball", "<f4", offsetof(rg::CarState, ball_hit_info) + offsetof(rg::BallHitInfo, relative_pos_on_ball), 3);
+    field("ball_hit_info.ball_pos", "<f4", offsetof(rg::CarState, ball_hit_info) + offsetof(rg::BallHitInfo, ball_pos), 3);
+    field("ball_hit_info.extra_hit_vel", "<f4", offsetof(rg::CarState, ball_hit_info) + offsetof(rg::BallHitInfo, extra_hit_vel), 3);
+    field("last_controls.throttle", "<f4", offsetof(rg::CarState, last_controls) + offsetof(rg::CarControls, throttle), 0);
+    field("last_controls.steer", "<f4", offsetof(rg::CarState, last_controls) + offsetof(rg::CarControls, steer), 0);
+    field("last_controls.pitch", "<f4", offsetof(rg::CarState, last_controls) + offsetof(rg::CarControls, pitch), 0);
+    field("last_controls.yaw", "<f4", offsetof(rg::CarState, last_controls) + offsetof(rg::CarControls, yaw), 0);
+    field("last_controls.roll", "<f4", offsetof(rg::CarState, last_controls) + offsetof(rg::CarControls, roll), 0);
+    field("last_controls.jump", "|u1", offsetof(rg::CarState, last_controls) + offsetof(rg::CarControls, jump), 0);
+    field("last_controls.boost", "|u1", offsetof(rg::CarState, last_controls) + offsetof(rg::CarControls, boost), 0);
+    field("last_controls.handbrake", "|u1", offsetof(rg::CarState, last_controls) + offsetof(rg::CarControls, handbrake), 0);
+    field("pos_bt", "<f4", offsetof(rg::CarState, pos_bt), 3);
+    field("vel_bt", "<f4", offsetof(rg::CarState, vel_bt), 3);
+    field("tick_start_pos_bt", "<f4", offsetof(rg::CarState, tick_start_pos_bt), 3);
+    field("pad_frame_pos_bt", "<f4", offsetof(rg::CarState, pad_frame_pos_bt), 3);
+    field("vel_touched_solve_stage", "|u1", offsetof(rg::CarState, vel_touched_solve_stage), 0);
+    field("basis_authoritative", "|u1", offsetof(rg::CarState, basis_authoritative), 0);
+    field("ext_force_impulse_bt_valid", "|u1", offsetof(rg::CarState, ext_force_impulse_bt_valid), 0);
+    field("pad_frame_pos_valid", "|u1", offsetof(rg::CarState, pad_frame_pos_valid), 0);
+    field("basis_forward", "<f4", offsetof(rg::CarState, basis_forward), 3);
+    field("basis_right", "<f4", offsetof(rg::CarState, basis_right), 3);
+    field("basis_up", "<f4", offsetof(rg::CarState, basis_up), 3);
+    field("ext_force_impulse_bt", "<f4", offsetof(rg::CarState, ext_force_impulse_bt), 3);
+    field("ext_torque_impulse", "<f4", offsetof(rg::CarState, ext_torque_impulse), 3);
+    field("solve_delta_linear_bt", "<f4", offsetof(rg::CarState, solve_delta_linear_bt), 3);
+    field("vel_impulse_cache_bt", "<f4", offsetof(rg::CarState, vel_impulse_cache_bt), 3);
+    field("solve_delta_angular", "<f4", offsetof(rg::CarState, solve_delta_angular), 3);
+    field("bump_event_bumps", "|u1", offsetof(rg::CarState, bump_event_bumps), 7);
+    field("bump_event_demos", "|u1", offsetof(rg::CarState, bump_event_demos), 7);
+    field("boost_pad_touching", "|u1", offsetof(rg::CarState, boost_pad_touching), 0);
+    field("bump_event_reserved1", "|u1", offsetof(rg::CarState, bump_event_reserved1), 0);
+    field("previous_ball_contact_start", "<u8", offsetof(rg::CarState, previous_ball_contact_start), 0);
+    field("last_ball_contact_start", "<u8", offsetof(rg::CarState, last_ball_contact_start), 0);
+    field("last_world_surface_contact", "<u8", offsetof(rg::CarState, last_world_surface_contact), 0);
+    field("first_world_after_previous_ball", "<u8", offsetof(rg::CarState, first_world_after_previous_ball), 0);
+    field("first_world_after_last_ball", "<u8", offsetof(rg::CarState, first_world_after_last_ball), 0);
+    field("last_flip_or_jump_start", "<u8", offsetof(rg::CarState, last_flip_or_jump_start), 0);
+    field("last_directional_flip_start", "<u8", offsetof(rg::CarState, last_directional_flip_start), 0);
+    field("last_ball_started_with_world", "|u1", offsetof(rg::CarState, last_ball_started_with_world), 0);
+    field("flip_or_jump_last_tick", "|u1", offsetof(rg::CarState, flip_or_jump_last_tick), 0);
+    field("internal_jump_time", "<f4", offsetof(rg::CarState, internal_jump_time), 0);
+    field("internal_flip_time", "<f4", offsetof(rg::CarState, internal_flip_time), 0);
+    field("flip_timer_live", "|u1", offsetof(rg::CarState, flip_timer_live), 0);
+    field("air_state", "|u1", offsetof(rg::CarState, air_state), 0);
+    field("driving_disabled", "<u2", offsetof(rg::CarState, driving_disabled), 0);
+    field("dodge_timeout", "<f4", offsetof(rg::CarState, dodge_timeout), 0);
+    field("dodge_elapsed", "<f4", offsetof(rg::CarState, dodge_elapsed), 0);
+    field("dodge_dir_x", "<f4", offsetof(rg::CarState, dodge_dir_x), 0);
+    field("dodge_dir_y", "<f4", offsetof(rg::CarState, dodge_dir_y), 0);
+    field("latest_touch_tick", "<u8", offsetof(rg::CarState, latest_touch_tick), 0);
+    field("latest_touch_location", "<f4", offsetof(rg::CarState, latest_touch_location), 3);
+    field("latest_touch_normal", "<f4", offsetof(rg::CarState, latest_touch_normal), 3);
+    field("internal_vel", "<f4", offsetof(rg::CarState, internal_vel), 3);
+    field("internal_ang_vel", "<f4", offsetof(rg::CarState, internal_ang_vel), 3);
+    field("reported_vel", "<f4", offsetof(rg::CarState, reported_vel), 3);
+    field("reported_ang_vel", "<f4", offsetof(rg::CarState, reported_ang_vel), 3);
+    field("internal_has_jumped", "|u1", offsetof(rg::CarState, internal_has_jumped), 0);
+    field("internal_has_double_jumped", "|u1", offsetof(rg::CarState, internal_has_double_jumped), 0);
+    field("internal_has_flipped", "|u1", offsetof(rg::CarState, internal_has_flipped), 0);
+    field("internal_is_supersonic", "|u1", offsetof(rg::CarState, internal_is_supersonic), 0);
+    field("reported_flags", "|u1", offsetof(rg::CarState, reported_flags), 0);
+    field("game_has_jumped", "|u1", offsetof(rg::CarState, game_has_jumped), 0);
+    field("double_jump_ticks", "|u1", offsetof(rg::CarState, double_jump_ticks), 0);
+    field("jump_hold_broken", "|u1", offsetof(rg::CarState, jump_hold_broken), 0);
+    field("supersonic_set_v3", "|u1", offsetof(rg::CarState, supersonic_set_v3), 0);
+    field("rlbotv5_dodged", "|u1", offsetof(rg::CarState, rlbotv5_dodged), 0);
+    field2("set_wheel_normals", "<f4", offsetof(rg::CarState, set_wheel_normals), 4, 3);
+    field("set_wheel_normals_mask", "|u1", offsetof(rg::CarState, set_wheel_normals_mask), 0);
+    field("cal_pre_flags", "|u1", offsetof(rg::CarState, cal_pre_flags), 0);
+    field("cal_pre_forward_speed_uu", "<f4", offsetof(rg::CarState, cal_pre_forward_speed_uu), 0);
+    field("cal_pre_pitchlock_restart_time", "<f4", offsetof(rg::CarState, cal_pre_pitchlock_restart_time), 0);
+    field("cal_tick_start_ang_vel", "<f4", offsetof(rg::CarState, cal_tick_start_ang_vel), 3);
+    field("tick_start_basis_forward", "<f4", offsetof(rg::CarState, tick_start_basis_forward), 3);
+    field("tick_start_basis_right", "<f4", offsetof(rg::CarState, tick_start_basis_right), 3);
+    field("tick_start_basis_up", "<f4", offsetof(rg::CarState, tick_start_basis_up), 3);
+    field2("restored_inv_inertia_bt", "<f4", offsetof(rg::CarState, restored_inv_inertia_bt), 3, 3);
+    field("restored_inv_inertia_valid", "|u1", offsetof(rg::CarState, restored_inv_inertia_valid), 0);
+    field2("pending_bump_bt", "<f4", offsetof(rg::CarState, pending_bump_bt), 7, 3);
+    field("pending_bump_hitter", "<u4", offsetof(rg::CarState, pending_bump_hitter), 7);
+    field("pending_bump_count", "|u1", offsetof(rg::CarState, pending_bump_count), 0);
+    field("auto_flip_torque_scale", "<f4", offsetof(rg::CarState, auto_flip_torque_scale), 0);
     begin("controls", sizeof(rg::CarControls), false);
     RG_F(rg::CarControls, throttle, "<f4", 0);
     RG_F(rg::CarControls, steer, "<f4", 0);

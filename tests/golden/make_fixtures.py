@@ -174,12 +174,47 @@ def make_env_commands(name, n, seed, scale, fillet, stream_ticks):
     print(f"{name}: {len(ENV_COMMAND_PHASES)} phases -> {os.path.getsize(path)} bytes")
 
 
+CAR_PRETICK_FIXTURES = {
+    # name: (arenas, cars per arena, ticks, seed, rows kept)
+    "car_pretick_1v1_3v3": ((16, 6), (2, 6), 400, 21, 320),
+}
+
+# CarState.cal_pre_flags bits the pre-tick sets (air jump started, dodge started, dodge changed the velocity) and the
+# byte offsets the fixture selection reads.
+CAL_PRE_FLAGS_OFFSET = 661
+
+
+def make_car_pretick(name, arenas, cars_per_env, ticks, seed, keep):
+    """Pre-tick inputs from car arenas stepped by the reference (1v1 and 3v3), and the reference's pre-tick of each:
+    every row whose pre-tick started a jump, a double jump or a dodge, plus a seeded sample of the rest."""
+    parts = [R.car_arena_inputs(n, c, ticks, seed + i) for i, (n, c) in enumerate(zip(arenas, cars_per_env))]
+    cars, wheels, controls, configs = (np.concatenate([p[k] for p in parts]) for k in range(4))
+    out_cars, out_wheels = R.car_pretick(cars, controls, configs, wheels)
+    flags = out_cars[:, CAL_PRE_FLAGS_OFFSET]
+    events = np.flatnonzero((flags & 7) != 0)
+    rng = np.random.default_rng(seed)
+    rest = np.setdiff1d(np.arange(len(cars)), events)
+    pick = np.sort(np.concatenate([events[: keep // 2], rng.choice(rest, keep - min(len(events), keep // 2),
+                                                                   replace=False)]))
+    meta = {"workload": "car pre-tick inputs from 1v1 and 3v3 car arenas with random held controls",
+            "arenas": list(arenas), "cars_per_env": list(cars_per_env), "ticks": ticks, "seed": seed,
+            "rows": int(len(pick)), "generator": "oracle/_ref reference CPU build (StepEnvPreTickCar)"}
+    path = os.path.join(HERE, name + ".npz")
+    np.savez_compressed(path, cars=cars[pick], wheels=wheels[pick], controls=controls[pick], configs=configs[pick],
+                        out_cars=out_cars[pick], out_wheels=out_wheels[pick],
+                        meta=np.frombuffer(json.dumps(meta).encode(), np.uint8))
+    print(f"{name}: {len(pick)} rows ({len(events)} with jump/dodge starts) -> {os.path.getsize(path)} bytes")
+
+
 def main(names):
-    for name in names or list(FIXTURES) + list(BALL_PATH_FIXTURES) + list(ENV_COMMAND_FIXTURES):
+    for name in names or (list(FIXTURES) + list(BALL_PATH_FIXTURES) + list(ENV_COMMAND_FIXTURES) +
+                          list(CAR_PRETICK_FIXTURES)):
         if name in FIXTURES:
             make(name, *FIXTURES[name])
         elif name in ENV_COMMAND_FIXTURES:
             make_env_commands(name, *ENV_COMMAND_FIXTURES[name])
+        elif name in CAR_PRETICK_FIXTURES:
+            make_car_pretick(name, *CAR_PRETICK_FIXTURES[name])
         else:
             make_ball_path(name, *BALL_PATH_FIXTURES[name])
 
