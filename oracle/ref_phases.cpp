@@ -78,3 +78,29 @@ void rgref_quantize_axis(const float* x, float* out, long n) {
 }
 
 }  // extern "C"
+
+extern "C" {
+
+// StepEnvPostCar (env_step.hpp:6902-6926) on n independent cars without a world-side finish (no solver scratch),
+// reporting fields when report is nonzero.  Car k is slot 0 of env k of a default config; wheels may be null.
+int rgref_car_posttick(long n, void* cars, const void* controls, void* wheels, int report) {
+    try {
+        SimConfig cfg{};
+        cfg.env_count = static_cast<int32_t>(n);
+        cfg.max_cars_per_env = 1;
+        std::vector<CarConfig> configs(static_cast<size_t>(n));
+        CarState* c = static_cast<CarState*>(cars);
+        const CarControls* ctl = static_cast<const CarControls*>(controls);
+        detail::WheelRuntimeState* w = static_cast<detail::WheelRuntimeState*>(wheels);
+        for (long k = 0; k < n; ++k) {
+            detail::StepEnvPostCar(static_cast<int>(k), 0, cfg, c, w, nullptr, detail::TickCars{}, configs.data(),
+                                   ctl[k], false, report != 0);
+        }
+        return 0;
+    } catch (const std::exception& e) {
+        g_phase_error = e.what();
+        return 1;
+    }
+}
+
+}  // extern "C"

@@ -60,6 +60,7 @@ def lib():
             ("rgref_download_car_configs", [vp, cp, lg]),
             ("rgref_car_pretick", [lg, cp, cp, cp, cp]),
             ("rgref_car_params", [cp]),
+            ("rgref_car_posttick", [lg, cp, cp, cp, ctypes.c_int]),
         ]:
             f = getattr(L, name)
             f.argtypes = args
@@ -261,6 +262,16 @@ def car_pretick(cars, controls, configs, wheels):
     if lib().rgref_car_pretick(c.shape[0], _ptr(c), _ptr(ctl), _ptr(cfg), _ptr(w)) != 0:
         raise RuntimeError("rgref_car_pretick failed")
     return c, w
+
+
+def car_posttick(cars, controls, wheels, report: bool):
+    """StepEnvPostCar without the world-side finish on independent cars: returns the cars after it."""
+    c = np.ascontiguousarray(cars, dtype=np.uint8).copy()
+    w = np.ascontiguousarray(wheels, dtype=np.uint8).copy()
+    ctl = np.ascontiguousarray(controls, dtype=np.uint8)
+    if lib().rgref_car_posttick(c.shape[0], _ptr(c), _ptr(ctl), _ptr(w), 1 if report else 0) != 0:
+        raise RuntimeError("rgref_car_posttick failed")
+    return c
 
 
 def car_arena_inputs(n_env: int, cars_per_env: int, ticks: int, seed: int, p_jump: float = 0.25):

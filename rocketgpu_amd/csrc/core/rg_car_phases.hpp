@@ -1,6 +1,7 @@
-// Car pre-tick: what happens to one car at the start of a tick, before the vehicle update and the world step.
+// The per-car phases around the physics of a tick: the pre-tick (before the vehicle update and the world step) and
+// the post-tick (after the world step).
 //
-// The reference's StepEnvPreTickCar (include/rocketgpu/detail/step/env_step.hpp:6751-6873):
+// Pre-tick, the reference's StepEnvPreTickCar (include/rocketgpu/detail/step/env_step.hpp:6751-6873):
 //   1. start: restore the working fields the last report replaced, derive the internal jump/flip timers of a car
 //      whose state was just set, count a demoed car's respawn timer down              env_step.hpp:6751-6816
 //   2. tick-start speed caps and the game's replicated-state quantization         raycast_wheels...:3993-4008
@@ -11,6 +12,9 @@
 // square root); the arithmetic contract is the one rg_math.hpp states.  Cars that respawn this tick (a demoed car
 // whose timer runs out) and three-wheel bodies are outside this file: CarPreTickSupported() says which cars it
 // handles, and the caller rejects the rest.
+//
+// Post-tick (CarPostTick): cooldowns, last controls, pending bumps and velocity caches, the tick counter, and the
+// fields the game reports.
 #pragma once
 
 #include "rg_math.hpp"
@@ -757,6 +761,136 @@ RG_HD void CarPreTick(CarState& car, const CarControls& controls, const CarConfi
     if (!car.is_demoed) {
         const bool impulsed = car.vel_bt.x != vel_before.x || car.vel_bt.y != vel_before.y || car.vel_bt.z != vel_before.z;
         if ((BodyDampingClampCar(car, m) & 1) && impulsed) SetCarVelBt(car, car.vel_bt);
+    }
+}
+
+// ------------------------------------------------------------------------------------------------------------
+// The car post-tick (StepEnvPostCar after the world step, env_step.hpp:6902-6926, with the world-side finish done):
+// cooldowns and last controls (PostTickCar, raycast_wheels_and_contact_prep.hpp:4029-4038), the tick's bookkeeping
+// (FinishCarTick, :3961-3991) and the fields the game reports (ReportRLBotCarFields / AdvanceRLBotCarFields,
+// env_step.hpp:5835-5913).
+
+RG_HD bool HasFlipOrJump(const CarState& car) {
+    return car.is_on_ground || (!car.has_flipped && !car.has_double_jumped && car.air_time_since_jump < 1.25f);
+}
+
+RG_HD void FinishCarTick(CarState& car) {
+    car.restored_inv_inertia_valid = 0;
+    if (HasFlipOrJump(car) && !car.flip_or_jump_last_tick) car.last_flip_or_jump_start = car.tick_count_since_update;
+    car.flip_or_jump_last_tick = HasFlipOrJump(car);
+    car.ball_contact_last_tick = car.ball_contact_this_tick;
+    if (car.is_demoed) return;
+    if (car.pending_bump_count > 0) {
+        Vec3 v = car.vel_bt;
+        for (int i = 0; i < car.pending_bump_count; ++i) v = v + car.pending_bump_bt[i];
+        SetCarVelBt(car, v);
+        car.pending_bump_count = 0;
+        car.vel_impulse_cache = Vec3(0.0f, 0.0f, 0.0f);
+        car.vel_impulse_cache_bt = Vec3(0.0f, 0.0f, 0.0f);
+    } else if (LengthSq(car.vel_impulse_cache_bt) > 0.0f) {
+        SetCarVelBt(car, car.vel_bt + car.vel_impulse_cache_bt);
+        car.vel_impulse_cache = Vec3(0.0f, 0.0f, 0.0f);
+        car.vel_impulse_cache_bt = Vec3(0.0f, 0.0f, 0.0f);
+    } else if (LengthSq(car.vel_impulse_cache) > 0.0f) {
+        // SetCarLinearVelocityUu: through bt and back, marking the velocity touched.
+        const Vec3 v = car.vel + car.vel_impulse_cache;
+        car.vel = v;
+        car.vel_bt = v * (1.0f / 50.0f);
+        car.vel_touched_solve_stage = 1;
+        car.vel_impulse_cache = Vec3(0.0f, 0.0f, 0.0f);
+    }
+    car.tick_count_since_update += 1;
+}
+
+// The double-jump tick counter and the game's has_jumped, shared by both report forms.
+RG_HD void AdvanceJumpReportState(CarState& car, const CarParams& m, float dt, bool dj_started) {
+    car.double_jump_ticks = !car.has_double_jumped
+                                ? 255
+                                : (dj_started ? 0 : (car.double_jump_ticks < 255 ? car.double_jump_ticks + 1 : 255));
+    const bool on_ground_state = car.is_on_ground && !car.is_jumping && !car.is_flipping;
+    if (car.is_jumping) {
+        car.game_has_jumped = 1;
+    } else if (on_ground_state &&
+               (!car.has_jumped || car.internal_jump_time - dt > m.jump_min_time + m.jump_reset_time_pad)) {
+        car.game_has_jumped = 0;
+    }
+}
+
+// The state the game would report: working velocities and flags move to the internal_* fields, the public
+// has_jumped / has_double_jumped / has_flipped become the game's, and the RLBot dodge fields are derived.
+RG_HD void ReportCarFields(CarState& car, const WheelRuntimeState* jw, const CarParams& m, float dt) {
+    if (car.is_demoed && car.tick_count_since_update == 0) {
+        car.reported_flags = 0;
+        return;
+    }
+    const bool dj_started = car.has_double_jumped && !car.internal_has_double_jumped;
+    car.internal_vel = car.vel;
+    car.internal_ang_vel = car.ang_vel;
+    car.internal_has_jumped = car.has_jumped;
+    car.internal_has_double_jumped = car.has_double_jumped;
+    car.internal_has_flipped = car.has_flipped;
+    car.internal_is_supersonic = car.is_supersonic;
+    AdvanceJumpReportState(car, m, dt, dj_started);
+    const bool on_ground_state = car.is_on_ground && !car.is_jumping && !car.is_flipping;
+
+    bool game_jumped = jw ? jw->cal_game_jumped != 0 : car.has_jumped != 0;
+    bool game_double_jumped = jw ? jw->cal_game_double_jumped != 0 : (car.has_double_jumped || car.has_flipped);
+    const int wheels = static_cast<int>(car.wheels_with_contact[0] != 0) + static_cast<int>(car.wheels_with_contact[1] != 0) +
+                       static_cast<int>(car.wheels_with_contact[2] != 0) + static_cast<int>(car.wheels_with_contact[3] != 0);
+    if (jw && !jw->cal_frame_open && !jw->cal_cached_ground && wheels >= 3) game_jumped = game_double_jumped = false;
+    car.has_jumped = game_jumped;
+    car.has_double_jumped = game_double_jumped && car.internal_has_double_jumped;
+    car.has_flipped = game_double_jumped && !car.has_double_jumped;
+
+    const uint8_t air_state = car.is_jumping ? 1 : (car.is_flipping ? 3 : (car.is_on_ground ? 0 : (car.double_jump_ticks < 13 ? 2 : 4)));
+    car.air_state = air_state;
+    const bool flipped = car.internal_has_flipped != 0;
+    const bool dodged = (car.rlbotv5_dodged || flipped) && air_state != 0;
+    car.rlbotv5_dodged = dodged ? 1 : 0;
+    const float jump_inactive = jw ? jw->cal_jump_inactive_time : car.air_time_since_jump;
+    const bool dodge_window = !car.is_on_ground && !car.is_jumping && car.game_has_jumped &&
+                              !car.internal_has_double_jumped && !dodged &&
+                              jump_inactive < m.doublejump_max_delay - 0.5f * dt;
+    car.dodge_timeout = dodge_window ? m.doublejump_max_delay - jump_inactive : -1.0f;
+    car.dodge_elapsed = (flipped && !on_ground_state) ? car.internal_flip_time : (dodged ? car.dodge_elapsed + dt : 0.0f);
+    car.dodge_dir_x = car.flip_rel_torque.y;
+    car.dodge_dir_y = -car.flip_rel_torque.x;
+
+    constexpr uint8_t kJumped = 4, kDoubleJumped = 8, kFlipped = 16, kSupersonic = 32, kValid = 128;
+    car.reported_vel = car.vel;
+    car.reported_ang_vel = car.ang_vel;
+    car.reported_flags = static_cast<uint8_t>(kValid | (car.has_jumped ? kJumped : 0) |
+                                              (car.has_double_jumped ? kDoubleJumped : 0) |
+                                              (car.has_flipped ? kFlipped : 0) | (car.is_supersonic ? kSupersonic : 0));
+}
+
+// The in-between ticks of a multi-tick step report nothing; the counters still advance.
+RG_HD void AdvanceCarFields(CarState& car, const WheelRuntimeState* jw, const CarParams& m, float dt) {
+    if (!(LengthSq(car.vel_bt) < 1.0e38f && LengthSq(car.ang_vel) < 1.0e38f)) {
+        ReportCarFields(car, jw, m, dt);
+        return;
+    }
+    car.reported_flags = 0;
+    if (car.is_demoed && car.tick_count_since_update == 0) return;
+    const bool dj_started = car.has_double_jumped && !car.internal_has_double_jumped;
+    car.internal_has_double_jumped = car.has_double_jumped;
+    AdvanceJumpReportState(car, m, dt, dj_started);
+}
+
+RG_HD void CarPostTick(CarState& car, const CarControls& controls, const CarParams& m, const WheelRuntimeState* jw,
+                       bool report_fields) {
+    const float dt = m.tick_dt;
+    if (!car.is_demoed) {
+        if (car.car_contact.cooldown_timer > 0.0f) {
+            car.car_contact.cooldown_timer = Max(0.0f, car.car_contact.cooldown_timer - dt);
+        }
+        car.last_controls = controls;
+    }
+    FinishCarTick(car);
+    if (report_fields) {
+        ReportCarFields(car, jw, m, dt);
+    } else {
+        AdvanceCarFields(car, jw, m, dt);
     }
 }
 

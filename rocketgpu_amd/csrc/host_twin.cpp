@@ -11,7 +11,7 @@
 #include <vector>
 
 #include "batch_common.hpp"
-#include "core/rg_car_pretick.hpp"
+#include "core/rg_car_phases.hpp"
 #include "core/rg_env_commands.hpp"
 
 using namespace rg;
@@ -282,6 +282,15 @@ long rgt_car_pretick(long n, void* cars, const void* controls, const void* confi
         core::CarPreTick(c[k], ctl[k], cc[k], m, w ? &w[k] : nullptr);
     }
     return skipped;
+}
+
+// The car post-tick on n independent cars (wheels may be null), reporting fields when report is nonzero.
+void rgt_car_posttick(long n, void* cars, const void* controls, const void* wheels, int report) {
+    const core::CarParams m{};
+    CarState* c = static_cast<CarState*>(cars);
+    const CarControls* ctl = static_cast<const CarControls*>(controls);
+    const WheelRuntimeState* w = static_cast<const WheelRuntimeState*>(wheels);
+    for (long k = 0; k < n; ++k) core::CarPostTick(c[k], ctl[k], m, w ? &w[k] : nullptr, report != 0);
 }
 
 // The game's axis quantization over n inputs.

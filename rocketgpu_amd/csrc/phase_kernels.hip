@@ -5,7 +5,7 @@
 
 #include <cstdint>
 
-#include "core/rg_car_pretick.hpp"
+#include "core/rg_car_phases.hpp"
 
 using namespace rg;
 
@@ -31,9 +31,32 @@ __global__ void __launch_bounds__(64) CarPreTickKernel(CarState* cars, const Car
     if (wheels) wheels[k] = ws;
 }
 
+// Car k's post-tick (cooldowns, last controls, bumps and velocity caches, tick counter, reported fields).
+__global__ void __launch_bounds__(64) CarPostTickKernel(CarState* cars, const CarControls* controls,
+                                                        const WheelRuntimeState* wheels, int32_t n, core::CarParams m,
+                                                        int32_t report) {
+    const int k = static_cast<int>(blockIdx.x) * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x);
+    if (k >= n) return;
+    CarState car = cars[k];
+    WheelRuntimeState ws;
+    if (wheels) ws = wheels[k];
+    core::CarPostTick(car, controls[k], m, wheels ? &ws : nullptr, report != 0);
+    cars[k] = car;
+}
+
 }  // namespace
 
 extern "C" {
+
+// n cars in place (device pointers; wheels may be null, and are only read).  Returns a hipError_t.
+int rgp_car_posttick(void* cars, const void* controls, const void* wheels, int32_t n, int32_t report, void* stream) {
+    if (n <= 0) return 0;
+    const core::CarParams m{};
+    hipLaunchKernelGGL(CarPostTickKernel, dim3((n + 63) / 64), dim3(64), 0, static_cast<hipStream_t>(stream),
+                       static_cast<CarState*>(cars), static_cast<const CarControls*>(controls),
+                       static_cast<const WheelRuntimeState*>(wheels), n, m, report);
+    return static_cast<int>(hipGetLastError());
+}
 
 // n cars in place (device pointers; wheels may be null), the default car parameters; unsupported: n device bytes.
 // Returns a hipError_t.

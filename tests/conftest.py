@@ -60,6 +60,8 @@ def twin():
     T.rgt_car_params.argtypes = [vp]
     T.rgt_quantize_axis.argtypes = [vp, vp, ctypes.c_long]
     T.rgt_quantize_axis.restype = None
+    T.rgt_car_posttick.argtypes = [ctypes.c_long, vp, vp, vp, ctypes.c_int]
+    T.rgt_car_posttick.restype = None
     return T
 
 

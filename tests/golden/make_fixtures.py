@@ -206,15 +206,69 @@ def make_car_pretick(name, arenas, cars_per_env, ticks, seed, keep):
     print(f"{name}: {len(pick)} rows ({len(events)} with jump/dodge starts) -> {os.path.getsize(path)} bytes")
 
 
+CAR_POSTTICK_FIXTURES = {
+    # name: (arenas, cars per arena, ticks, seed, rows kept)
+    "car_posttick_1v1_3v3": ((8, 4), (2, 6), 200, 31, 256),
+}
+
+
+def perturb_post_inputs(cars, seed):
+    """Post-tick inputs that exercise every branch: on a seeded third of the rows pending bumps (1-3 velocity
+    deltas in bt), on another third a velocity impulse cache (uu, or bt and uu), and a few demoed cars."""
+    import rocketgpu_amd as rg
+    lay = {f[0]: f[2] for f in rg.state_layouts()["car_states"]["fields"]}
+    rng = np.random.default_rng(seed)
+    c = cars.copy()
+    n = len(c)
+    kind = rng.integers(0, 3, n)
+    for i in np.flatnonzero(kind == 1):
+        k = int(rng.integers(1, 4))
+        c[i, lay["pending_bump_count"]] = k
+        bumps = rng.uniform(-3, 3, (7, 3)).astype(np.float32)
+        c[i, lay["pending_bump_bt"]:lay["pending_bump_bt"] + 84] = bumps.view(np.uint8).reshape(-1)
+    for i in np.flatnonzero(kind == 2):
+        uu = rng.uniform(-200, 200, 3).astype(np.float32)
+        c[i, lay["vel_impulse_cache"]:lay["vel_impulse_cache"] + 12] = uu.view(np.uint8)
+        if rng.random() < 0.5:
+            bt = (uu * np.float32(0.02)).astype(np.float32)
+            c[i, lay["vel_impulse_cache_bt"]:lay["vel_impulse_cache_bt"] + 12] = bt.view(np.uint8)
+    demo = rng.random(n) < 0.05
+    c[demo, lay["is_demoed"]] = 1
+    return c
+
+
+def make_car_posttick(name, arenas, cars_per_env, ticks, seed, keep):
+    """Post-tick inputs (the pre-tick's outputs of car-arena states, perturbed) and the reference's post-tick of
+    each, with and without reported fields."""
+    parts = [R.car_arena_inputs(n, c, ticks, seed + i) for i, (n, c) in enumerate(zip(arenas, cars_per_env))]
+    cars, wheels, controls, configs = (np.concatenate([p[k] for p in parts]) for k in range(4))
+    mid, mid_wheels = R.car_pretick(cars, controls, configs, wheels)
+    rng = np.random.default_rng(seed)
+    pick = np.sort(rng.choice(len(cars), keep, replace=False))
+    inputs = perturb_post_inputs(mid[pick], seed)
+    out_report = R.car_posttick(inputs, controls[pick], mid_wheels[pick], True)
+    out_advance = R.car_posttick(inputs, controls[pick], mid_wheels[pick], False)
+    meta = {"workload": "car post-tick inputs: pre-ticked car-arena states with seeded pending bumps, velocity "
+                        "caches and demoed cars", "arenas": list(arenas), "cars_per_env": list(cars_per_env),
+            "ticks": ticks, "seed": seed, "rows": int(keep),
+            "generator": "oracle/_ref reference CPU build (StepEnvPostCar)"}
+    path = os.path.join(HERE, name + ".npz")
+    np.savez_compressed(path, cars=inputs, wheels=mid_wheels[pick], controls=controls[pick], out_report=out_report,
+                        out_advance=out_advance, meta=np.frombuffer(json.dumps(meta).encode(), np.uint8))
+    print(f"{name}: {keep} rows -> {os.path.getsize(path)} bytes")
+
+
 def main(names):
     for name in names or (list(FIXTURES) + list(BALL_PATH_FIXTURES) + list(ENV_COMMAND_FIXTURES) +
-                          list(CAR_PRETICK_FIXTURES)):
+                          list(CAR_PRETICK_FIXTURES) + list(CAR_POSTTICK_FIXTURES)):
         if name in FIXTURES:
             make(name, *FIXTURES[name])
         elif name in ENV_COMMAND_FIXTURES:
             make_env_commands(name, *ENV_COMMAND_FIXTURES[name])
         elif name in CAR_PRETICK_FIXTURES:
             make_car_pretick(name, *CAR_PRETICK_FIXTURES[name])
+        elif name in CAR_POSTTICK_FIXTURES:
+            make_car_posttick(name, *CAR_POSTTICK_FIXTURES[name])
         else:
             make_ball_path(name, *BALL_PATH_FIXTURES[name])
 
