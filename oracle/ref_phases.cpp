@@ -48,30 +48,6 @@ int rgref_car_params(float* out) {
     return static_cast<int>(sizeof(v) / sizeof(float));
 }
 
-// StepEnvPreTickCar (env_step.hpp:6854-6873) on n independent cars, car k as slot 0 of env k of a default config
-// (one car per env; teams Blue), each with its controls, config and wheel state (wheels may be null).
-int rgref_car_pretick(long n, void* cars, const void* controls, const void* configs, void* wheels) {
-    try {
-        SimConfig cfg{};
-        cfg.env_count = static_cast<int32_t>(n);
-        cfg.max_cars_per_env = 1;
-        std::vector<Team> teams(static_cast<size_t>(n), Team::Blue);
-        const EnvState env{};
-        CarState* c = static_cast<CarState*>(cars);
-        const CarControls* ctl = static_cast<const CarControls*>(controls);
-        const CarConfig* cc = static_cast<const CarConfig*>(configs);
-        detail::WheelRuntimeState* w = static_cast<detail::WheelRuntimeState*>(wheels);
-        for (long k = 0; k < n; ++k) {
-            detail::StepEnvPreTickCar(static_cast<int>(k), 0, cfg, teams.data(), c, detail::TickCars{}, env, nullptr, 0,
-                                      ctl[k], cc[k], w ? &w[k] : nullptr);
-        }
-        return 0;
-    } catch (const std::exception& e) {
-        g_phase_error = e.what();
-        return 1;
-    }
-}
-
 // QuantizeAxisInput (raycast_wheels_and_contact_prep.hpp:2977) over n inputs.
 void rgref_quantize_axis(const float* x, float* out, long n) {
     for (long i = 0; i < n; ++i) out[i] = detail::QuantizeAxisInput(x[i]);

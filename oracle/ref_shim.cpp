@@ -16,6 +16,7 @@
 
 #include "rocketgpu/rocketgpu.h"
 #include "rocketgpu/detail/arena/mesh_bvh.hpp"
+#include "rocketgpu/detail/step.hpp"
 #include "rocketgpu_amd/workload.h"
 
 using namespace rocketgpu;
@@ -156,6 +157,9 @@ long rgref_offsetof(int which, const char* field) {
         RGREF_FIELD(CarConfig, front_wheel_radius) RGREF_FIELD(CarConfig, back_wheel_radius)
         RGREF_FIELD(CarConfig, front_suspension_rest) RGREF_FIELD(CarConfig, back_suspension_rest)
         RGREF_FIELD(CarConfig, dodge_deadzone) RGREF_FIELD(CarConfig, three_wheels)
+        RGREF_FIELD(CarConfig, hitbox_published) RGREF_FIELD(CarConfig, body)
+        RGREF_FIELD(CarConfig, steer_angle_curve_deg) RGREF_FIELD(CarConfig, powerslide_steer_angle_curve_deg)
+        RGREF_FIELD(CarConfig, spawn_z)
     }
     if (which == 0) {
         RGREF_FIELD(CarState, pos)
@@ -445,6 +449,31 @@ int rgref_download_wheels(void* p, void* dst, long n) {
     try {
         static_cast<RefSim*>(p)->sim->download_wheel_runtime_states(static_cast<detail::WheelRuntimeState*>(dst),
                                                                    static_cast<size_t>(n));
+        return 0;
+    } catch (const std::exception& e) { return Fail(e); }
+}
+
+// StepEnvPreTickCar (env_step.hpp:6854-6873) on every car slot of the batch's arrays as given (host copies, batch
+// layout: slot k of env e at e * max_cars + k), with this batch's config (arena rules included) and a respawn seed.
+int rgref_car_pretick_batch(void* p, uint64_t seed, void* cars, const void* controls, const void* configs,
+                            void* wheels, const void* envs, const uint8_t* teams) {
+    try {
+        const RefSim& r = *static_cast<RefSim*>(p);
+        const SimConfig& cfg = r.sim->config();
+        std::vector<Team> team(static_cast<size_t>(r.env_count) * r.max_cars);
+        for (size_t i = 0; i < team.size(); ++i) team[i] = teams[i] ? Team::Orange : Team::Blue;
+        CarState* c = static_cast<CarState*>(cars);
+        const CarControls* ctl = static_cast<const CarControls*>(controls);
+        const CarConfig* cc = static_cast<const CarConfig*>(configs);
+        detail::WheelRuntimeState* w = static_cast<detail::WheelRuntimeState*>(wheels);
+        const EnvState* ev = static_cast<const EnvState*>(envs);
+        for (int e = 0; e < r.env_count; ++e) {
+            for (int k = 0; k < r.max_cars; ++k) {
+                const size_t i = static_cast<size_t>(e) * r.max_cars + k;
+                detail::StepEnvPreTickCar(e, k, cfg, team.data(), c, detail::TickCars{}, ev[e], nullptr, seed, ctl[i],
+                                          cc[i], w ? &w[i] : nullptr);
+            }
+        }
         return 0;
     } catch (const std::exception& e) { return Fail(e); }
 }

@@ -58,7 +58,7 @@ def lib():
             ("rgref_set_event_stream", [vp, cp, cp, ctypes.c_uint32, ctypes.c_int]),
             ("rgref_download_wheels", [vp, cp, lg]),
             ("rgref_download_car_configs", [vp, cp, lg]),
-            ("rgref_car_pretick", [lg, cp, cp, cp, cp]),
+            ("rgref_car_pretick_batch", [vp, ctypes.c_uint64, cp, cp, cp, cp, cp, cp]),
             ("rgref_car_params", [cp]),
             ("rgref_car_posttick", [lg, cp, cp, cp, ctypes.c_int]),
         ]:
@@ -253,14 +253,22 @@ def quantize_axis(x) -> np.ndarray:
     return out
 
 
-def car_pretick(cars, controls, configs, wheels):
-    """StepEnvPreTickCar on independent cars: returns (cars, wheels) after the pre-tick (inputs are not changed)."""
+_phase_sims = {}
+
+
+def car_pretick(n_env: int, max_cars: int, seed: int, cars, controls, configs, wheels, envs, teams):
+    """StepEnvPreTickCar on every car slot of a batch (slot k of env e at e * max_cars + k) with the procedural
+    arena's rules and the respawn seed: returns (cars, wheels) after it (inputs are not changed)."""
+    key = (n_env, max_cars)
+    if key not in _phase_sims:
+        _phase_sims[key] = RefSim(n_env, max_cars)
+    sim = _phase_sims[key]
     c = np.ascontiguousarray(cars, dtype=np.uint8).copy()
     w = np.ascontiguousarray(wheels, dtype=np.uint8).copy()
-    ctl = np.ascontiguousarray(controls, dtype=np.uint8)
-    cfg = np.ascontiguousarray(configs, dtype=np.uint8)
-    if lib().rgref_car_pretick(c.shape[0], _ptr(c), _ptr(ctl), _ptr(cfg), _ptr(w)) != 0:
-        raise RuntimeError("rgref_car_pretick failed")
+    args = [np.ascontiguousarray(a, dtype=np.uint8) for a in (controls, configs, envs, teams)]
+    if lib().rgref_car_pretick_batch(sim.h, seed, _ptr(c), _ptr(args[0]), _ptr(args[1]), _ptr(w), _ptr(args[2]),
+                                     _ptr(args[3])) != 0:
+        raise RuntimeError(lib().rgref_last_error().decode())
     return c, w
 
 
@@ -274,22 +282,24 @@ def car_posttick(cars, controls, wheels, report: bool):
     return c
 
 
-def car_arena_inputs(n_env: int, cars_per_env: int, ticks: int, seed: int, p_jump: float = 0.25):
-    """Inputs of the car pre-tick as car arenas meet them: the reference steps n_env arenas of cars_per_env cars
-    (hitbox presets cycling through the six bodies, alternating teams) from a kickoff, with random controls held for
-    1-30 ticks (axes uniform in [-1, 1], jump / boost / handbrake with probability p_jump / 0.3 / 0.05).  Returns
-    the per-tick records before each tick, stacked [ticks * cars]: cars, wheels, controls, configs."""
+def car_arena_run(n_env: int, cars_per_env: int, ticks: int, seed: int, p_jump: float = 0.25) -> dict:
+    """Car arenas as the reference steps them: n_env arenas of cars_per_env cars (hitbox presets cycling through the
+    six bodies, alternating teams) from a kickoff, with random controls held for 1-30 ticks (axes uniform in
+    [-1, 1], jump / boost / handbrake with probability p_jump / 0.3 / 0.05).  Returns the records before each tick:
+    cars [ticks, cap, 880], wheels [ticks, cap, 292], controls [ticks, cap, 24], envs [ticks, n_env, 44], and the
+    batch's configs [cap, 120] and teams [cap]."""
     rng = np.random.default_rng(seed)
     s = RefSim(n_env, cars_per_env)
     cap = n_env * cars_per_env
+    teams = (np.arange(cap) % 2).astype(np.uint8)
     s.set_env_car_count(np.full(n_env, cars_per_env, np.int32))
-    s.set_teams(np.arange(cap) % 2)
+    s.set_teams(teams)
     s.set_car_presets(np.arange(cap) % 6)
     s.reset_to_kickoff(seed)
     configs = s.download_car_configs(cap)
     ctl = np.zeros(cap, CONTROLS_DTYPE)
     left = np.zeros(cap, np.int64)
-    out_cars, out_wheels, out_ctl = [], [], []
+    out = {"cars": [], "wheels": [], "controls": [], "envs": []}
     for _ in range(ticks):
         for i in range(cap):
             if left[i] <= 0:
@@ -301,10 +311,13 @@ def car_arena_inputs(n_env: int, cars_per_env: int, ticks: int, seed: int, p_jum
                 ctl[i]["handbrake"] = rng.random() < 0.05
             left[i] -= 1
         cb = ctl.view(np.uint8).reshape(cap, 24).copy()
-        out_cars.append(s.download(CAR, cap))
-        out_wheels.append(s.download_wheels(cap))
-        out_ctl.append(cb)
+        out["cars"].append(s.download(CAR, cap))
+        out["wheels"].append(s.download_wheels(cap))
+        out["controls"].append(cb)
+        out["envs"].append(s.download(ENV, n_env))
         s.step_with_controls(cb, 1, cap)
     s.close()
-    return (np.concatenate(out_cars), np.concatenate(out_wheels), np.concatenate(out_ctl),
-            np.tile(configs, (ticks, 1)))
+    res = {k: np.stack(v) for k, v in out.items()}
+    res["configs"] = configs
+    res["teams"] = teams
+    return res

@@ -6,6 +6,7 @@
 #include <vector>
 
 #include "core/rg_ball_step.hpp"
+#include "core/rg_car_phases.hpp"
 #include "mesh_build.hpp"
 #include "rocketgpu_amd/workload.h"
 
@@ -40,6 +41,24 @@ inline core::StepParams DefaultStepParams(const rgw::ArenaRulesSpec& rules) {
     p.pad_big_mask = 0;
     for (int k = 0; k < rules.pad_count; ++k) p.pad_big_mask |= static_cast<uint64_t>(rules.pad_big[k] != 0) << k;
     return p;
+}
+
+// The layout's spawn spots as the step reads them (MakeArenaRules, src/rocketgpu.cpp:858-868).
+inline core::CarSpawnRules SpawnRules(const rgw::ArenaRulesSpec& rules) {
+    core::CarSpawnRules r{};
+    for (int t = 0; t < 2; ++t) {
+        for (int k = 0; k < 5; ++k) {
+            const rgw::SpawnSpot& s = rules.kickoff[t][k];
+            r.kickoff_pos[t][k] = Vec3(s.x, s.y, s.z);
+            r.kickoff_yaw[t][k] = s.yaw;
+        }
+        for (int k = 0; k < 4; ++k) {
+            const rgw::SpawnSpot& s = rules.respawn[t][k];
+            r.respawn_pos[t][k] = Vec3(s.x, s.y, s.z);
+            r.respawn_yaw[t][k] = s.yaw;
+        }
+    }
+    return r;
 }
 
 inline ArenaMeshHost BuildProceduralArenaMesh(const rgw::ArenaParams& params) {

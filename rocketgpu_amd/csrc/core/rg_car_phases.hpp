@@ -9,14 +9,14 @@
 //      state machine with its impulses, and the body velocity clamp               env_step.hpp:6818-6852, 2981-3120
 //
 // Every expression follows the reference's float operation order (no contraction, correctly rounded division and
-// square root); the arithmetic contract is the one rg_math.hpp states.  Cars that respawn this tick (a demoed car
-// whose timer runs out) and three-wheel bodies are outside this file: CarPreTickSupported() says which cars it
-// handles, and the caller rejects the rest.
+// square root); the arithmetic contract is the one rg_math.hpp states.  Three-wheel bodies are outside this file:
+// CarPreTickSupported() says which cars it handles, and the caller rejects the rest.
 //
 // Post-tick (CarPostTick): cooldowns, last controls, pending bumps and velocity caches, the tick counter, and the
 // fields the game reports.
 #pragma once
 
+#include "rg_env_commands.hpp"
 #include "rg_math.hpp"
 
 namespace rg {
@@ -681,6 +681,207 @@ RG_HD void PreTickCarPrePhysics(CarState& car, CarControls controls, const CarCo
 }
 
 // ------------------------------------------------------------------------------------------------------------
+// Respawns of demoed cars (arena_ball.hpp:2478-2576, 3092-3166; math.hpp:838-863, 1454-1518): the spot, the spawn
+// transform, and the state reset.
+
+// The arena's spawn spots (ArenaRules, constants.hpp:57-67): position (uu, the car's spawn height added on top) and
+// yaw, per team.
+struct CarSpawnRules {
+    Vec3 kickoff_pos[2][5];
+    float kickoff_yaw[2][5];
+    Vec3 respawn_pos[2][4];
+    float respawn_yaw[2][4];
+};
+
+// Where this tick's respawns come from: the batch seed, the env, its tick, the car's id (env * max_cars + slot + 1)
+// and team.
+struct CarRespawnContext {
+    const CarSpawnRules* rules;
+    uint64_t seed;
+    int env_idx;
+    uint32_t env_tick;
+    uint32_t car_id;
+    int team;  // 0 Blue, 1 Orange
+};
+
+RG_HD float WrapAngle(float a) {
+    while (a > Pi()) a -= 2.0f * Pi();
+    while (a < -Pi()) a += 2.0f * Pi();
+    return a;
+}
+
+// Yaw quaternion with the exact eighth-turn special cases (math.hpp:838-863).
+RG_HD Quat FromYaw(float yaw) {
+    constexpr float kEps = 1e-6f;
+    constexpr float kSin8 = 0.38268343236508977173f, kCos8 = 0.92387953251128675613f;
+    constexpr float kSqrtHalf = 0.7071067811865475244f;
+    if (Abs(yaw - (Pi() * 0.25f)) <= kEps) return Quat(kCos8, 0.0f, 0.0f, kSin8);
+    if (Abs(yaw + (Pi() * 0.25f)) <= kEps) return Quat(kCos8, 0.0f, 0.0f, -kSin8);
+    if (Abs(yaw - (Pi() * 0.75f)) <= kEps) return Quat(kSin8, 0.0f, 0.0f, kCos8);
+    if (Abs(yaw + (Pi() * 0.75f)) <= kEps) return Quat(kSin8, 0.0f, 0.0f, -kCos8);
+    if (Abs(yaw - HalfPi()) <= kEps) return Quat(kSqrtHalf, 0.0f, 0.0f, kSqrtHalf);
+    if (Abs(yaw + HalfPi()) <= kEps) return Quat(kSqrtHalf, 0.0f, 0.0f, -kSqrtHalf);
+    const float h = 0.5f * yaw;
+    return Quat(UcrtCosf(h), 0.0f, 0.0f, UcrtSinf(h));
+}
+
+// A yaw in rotator units as a quaternion built from the game's rotation matrix (math.hpp:1456-1476).
+RG_HD Quat Ue3QuatFromYawRotator(int yaw) {
+    const float sy = Ue3RotatorSin(yaw), cy = Ue3RotatorSin(yaw + 16384);
+    const float cp = Ue3RotatorSin(16384);
+    const float m[3][3] = {{cy * cp, cp * sy, 0.0f}, {-(cp * sy), cy * cp, -0.0f}, {-0.0f, 0.0f, cp * cp}};
+    const float tr = (m[1][1] + m[0][0]) + m[2][2];
+    if (tr > 0.0f) {
+        const float inv_s = 1.0f / sqrtf(tr + 1.0f);
+        const float s = inv_s * 0.5f;
+        return Quat(0.5f / inv_s, (m[1][2] - m[2][1]) * s, (m[2][0] - m[0][2]) * s, (m[0][1] - m[1][0]) * s);
+    }
+    int i = m[0][0] < m[1][1] ? 1 : 0;
+    if (m[i][i] <= m[2][2] && m[2][2] != m[i][i]) i = 2;
+    const int j = i == 2 ? 0 : i + 1, k = j == 2 ? 0 : j + 1;
+    const float inv_s = 1.0f / sqrtf(((m[i][i] - m[j][j]) - m[k][k]) + 1.0f);
+    const float s = inv_s * 0.5f;
+    float v[3];
+    v[i] = 0.5f / inv_s;
+    v[j] = (m[i][j] + m[j][i]) * s;
+    v[k] = (m[i][k] + m[k][i]) * s;
+    return Quat((m[j][k] - m[k][j]) * s, v[0], v[1], v[2]);
+}
+
+RG_HD int SpawnYawRotatorUnits(float yaw) { return static_cast<int>(yaw * (32768.0f / 3.14159274f) + 0.5f); }
+
+// The spawned car's authoritative basis from its yaw in rotator units (half a turn more for Orange).
+RG_HD void SeedCarBasisFromSpawnRotator(CarState& car, int blue_yaw, bool orange) {
+    Quat q = Ue3QuatFromYawRotator(blue_yaw);
+    if (orange) {
+        const Quat h = Ue3QuatFromYawRotator(32768);
+        q = Quat(((h.w * q.w - h.x * q.x) - h.y * q.y) - h.z * q.z, ((h.w * q.x + h.x * q.w) + h.y * q.z) - h.z * q.y,
+                 ((h.w * q.y + h.y * q.w) + h.z * q.x) - h.x * q.z, ((h.w * q.z + h.z * q.w) + h.x * q.y) - h.y * q.x);
+    }
+    GameQuatToColumns(q, car.basis_forward, car.basis_right, car.basis_up);
+    car.basis_authoritative = 1;
+}
+
+// Everything a spawn resets (ResetCarCommonState).
+RG_HD void ResetCarCommonState(CarState& car, float spawn_boost) {
+    car.restored_inv_inertia_valid = 0;
+    SetCarVelBt(car, Vec3(0.0f, 0.0f, 0.0f));
+    car.ang_vel = Vec3(0.0f, 0.0f, 0.0f);
+    car.world_contact.has_contact = 0;
+    car.world_contact.wheel_ball_mask = 0;
+    car.world_contact.reserved1 = 0;
+    car.world_contact.reserved2 = 0;
+    car.world_contact.contact_normal = Vec3(0.0f, 0.0f, 0.0f);
+    car.flip_rel_torque = Vec3(0.0f, 0.0f, 0.0f);
+    car.boost = spawn_boost;
+    car.boost_amount = spawn_boost / 100.0f;
+    car.jump_time = 0.0f;
+    car.flip_time = 0.0f;
+    car.internal_jump_time = 0.0f;
+    car.internal_flip_time = 0.0f;
+    car.flip_timer_live = 1;
+    car.air_time = 0.0f;
+    car.air_time_since_jump = 0.0f;
+    car.boosting_time = 0.0f;
+    car.time_since_boosted = 0.0f;
+    car.supersonic_time = 0.0f;
+    car.handbrake_val = 0.0f;
+    car.auto_flip_timer = 0.0f;
+    car.auto_flip_torque_scale = 0.0f;
+    car.demo_respawn_timer = 0.0f;
+    car.tick_count_since_update = 0;
+    car.is_on_ground = 1;
+    car.air_state = 0;
+    car.rlbotv5_dodged = 0;
+    car.rlbotv5_boost_hold = 0;
+    car.dodge_timeout = -1.0f;
+    car.dodge_elapsed = 0.0f;
+    car.dodge_dir_x = 0.0f;
+    car.dodge_dir_y = 0.0f;
+    for (int w = 0; w < 4; ++w) car.wheels_with_contact[w] = 0;
+    car.has_jumped = 0;
+    car.has_double_jumped = 0;
+    car.has_flipped = 0;
+    car.is_flipping = 0;
+    car.is_jumping = 0;
+    car.jump_hold_broken = 0;
+    car.is_boosting = 0;
+    car.is_supersonic = 0;
+    car.is_auto_flipping = 0;
+    car.is_demoed = 0;
+    car.just_respawned = 0;
+    car.car_contact.other_car_id = 0;
+    car.car_contact.cooldown_timer = 0.0f;
+    car.vel_impulse_cache = Vec3(0.0f, 0.0f, 0.0f);
+    car.vel_impulse_cache_bt = Vec3(0.0f, 0.0f, 0.0f);
+    car.pending_bump_count = 0;
+    car.ball_hit_info = BallHitInfo{};
+    car.last_controls = CarControls{};
+    car.boost_pad_touching = 0;
+    car.driving_disabled = 0;
+    car.respawn_slot_override_plus1 = 0;
+    car.pad_frame_pos_valid = 0;
+}
+
+RG_HD void PlaceCarAtWorldSpawn(CarState& car, const Vec3& pos, float yaw, float z, float boost) {
+    SetCarPosBt(car, Vec3(pos.x, pos.y, pos.z + z) * (1.0f / 50.0f));
+    car.rot = FromYaw(WrapAngle(yaw));
+    SeedCarBasisFromSpawnRotator(car, SpawnYawRotatorUnits(yaw), false);
+    ResetCarCommonState(car, boost);
+}
+
+// Respawn spot `spot` of the team, moved 192 uu in direction dir (0-7, eighths of a turn from the spot's facing)
+// when dir >= 0 (RespawnPlace).
+RG_HD void RespawnPlace(const CarSpawnRules& rules, int team, int spot, int dir, Vec3& ground, float& yaw) {
+    ground = rules.respawn_pos[team][spot & 3];
+    yaw = rules.respawn_yaw[team][spot & 3];
+    const Quat q = FromYaw(WrapAngle(yaw));
+    const float fx = q.w * q.w - q.z * q.z;
+    const float fy = 2.0f * q.w * q.z;
+    if (dir >= 0) {
+        constexpr float k = 0.70710677f;
+        const int d = dir & 7;
+        const float ux = d == 0 ? 1.0f : (d == 1 || d == 7) ? k : (d == 2 || d == 6) ? 0.0f : (d == 3 || d == 5) ? -k : -1.0f;
+        const float uy = (d == 0 || d == 4) ? 0.0f : (d == 1 || d == 3) ? k : d == 2 ? 1.0f : d == 6 ? -1.0f : -k;
+        ground.x += 192.0f * (fx * ux - fy * uy);
+        ground.y += 192.0f * (fx * uy + fy * ux);
+    }
+}
+
+RG_HD void RespawnCar(CarState& car, const CarSpawnRules& rules, int team, int slot, float spawn_z, float boost,
+                      int offset_dir) {
+    Vec3 ground;
+    float yaw;
+    RespawnPlace(rules, team, slot % 4, offset_dir, ground, yaw);
+    PlaceCarAtWorldSpawn(car, ground, yaw, spawn_z, boost);
+    car.vel_impulse_cache = Vec3(0.0f, 0.0f, 0.0f);
+    car.is_on_ground = 0;
+}
+
+RG_HD void KickoffSpawnCar(CarState& car, const CarSpawnRules& rules, int team, int spot, float spawn_z, float boost) {
+    const uint16_t driving_disabled = car.driving_disabled;
+    const int s = spot >= 0 && spot < 4 ? spot : 4;
+    PlaceCarAtWorldSpawn(car, rules.kickoff_pos[team][s], rules.kickoff_yaw[team][s], spawn_z, boost);
+    car.driving_disabled = driving_disabled;
+    car.vel_impulse_cache = Vec3(0.0f, 0.0f, 0.0f);
+}
+
+// The respawn spot a demoed car draws when nothing chose one (RespawnStartIndex, arena_ball.hpp:3092-3096).
+RG_HD int RespawnStartIndex(uint64_t seed, int env_idx, uint64_t tick, uint32_t car_id) {
+    uint32_t state = MixEnvSeed(seed ^ (tick * 0x9e3779b97f4a7c15ull), env_idx, static_cast<int>(car_id * 0x45d9f3bu));
+    state = NextLcg(state);
+    return LcgPick(state, 4);
+}
+
+// The tick's inertia basis of a car that just spawned: its yaw alone, through the game's rotator and matrix forms
+// (SpawnInertiaBasis, math.hpp:1513-1517).
+RG_HD void SpawnInertiaBasis(const Quat& spawn_rot, Vec3& forward, Vec3& right, Vec3& up) {
+    Vec3 f, r, u;
+    UeQuatAxes(Ue3QuatFromYawRotator(Ue3QuatToRotator(spawn_rot).yaw), f, r, u);
+    GameQuatToColumns(GameMatrixToQuat(f, r, u), forward, right, up);
+}
+
+// ------------------------------------------------------------------------------------------------------------
 // The car pre-tick (StepEnvPreTickCar).
 
 // Working fields the last report replaced (CarState::RestoreWorkingFields, rocketgpu.h:398-417), or the first-tick
@@ -720,38 +921,64 @@ RG_HD void RestoreStepCarFields(CarState& car) {
     car.double_jump_ticks = 255;
 }
 
-// Whether this file steps the car's pre-tick: not a demoed car whose respawn falls on this tick (respawn placement
-// is not built) and not a three-wheel body.
+// Whether this file steps the car's pre-tick: not a three-wheel body (its jump torques are not built).
 RG_HD bool CarPreTickSupported(const CarState& car, const CarConfig& cfg, const CarParams& m) {
-    if (cfg.three_wheels) return false;
-    return !(car.is_demoed && !(Max(0.0f, car.demo_respawn_timer - m.tick_dt) > 0.0f));
+    (void)car;
+    (void)m;
+    return cfg.three_wheels == 0;
 }
 
-// One car's pre-tick (StepEnvPreTickCar without the respawn branch).  `ws` may be null (no wheel runtime state).
+// One car's pre-tick (StepEnvPreTickCar).  `ws` may be null (no wheel runtime state).
 RG_HD void CarPreTick(CarState& car, const CarControls& controls, const CarConfig& cfg, const CarParams& m,
-                      WheelRuntimeState* ws) {
+                      WheelRuntimeState* ws, const CarRespawnContext& rc) {
     const float dt = m.tick_dt;
     // 1. start.
     car.ball_contact_this_tick = 0;
     RestoreStepCarFields(car);
     if (car.tick_count_since_update == 0) InternalTimersFromSetState(car, m, dt);
     if (car.just_respawned) car.just_respawned = 0;
+    bool respawned = false;
+    // respawn_slot_override_plus1: 5 + spot is a kickoff re-creation at kickoff spot `spot`, 128 + 8 * spot + dir a
+    // respawn 192 uu off respawn spot `spot` (rocketgpu.h:527-533), any other nonzero value 1 + the respawn spot.
+    constexpr int kKickoffSpawnOverridePlus1 = 5, kRespawnOffsetOverridePlus1 = 128;
     if (car.is_demoed) {
         car.demo_respawn_timer = Max(0.0f, car.demo_respawn_timer - dt);
-        // The respawn itself (timer at zero) is outside this file.
-        // A kickoff re-creation (respawn_slot_override_plus1 = 5 + spot, rocketgpu.h:527-529) keeps its boost.
-        constexpr int kKickoffSpawnOverridePlus1 = 5;
-        if (!(Max(0.0f, car.demo_respawn_timer - dt) > 0.0f) &&
-            !(car.respawn_slot_override_plus1 >= kKickoffSpawnOverridePlus1 &&
-              car.respawn_slot_override_plus1 < kKickoffSpawnOverridePlus1 + 5)) {
+        if (car.demo_respawn_timer <= 0.0f) {
+            int slot = -1, offset_dir = -1;
+            const int ov = car.respawn_slot_override_plus1;
+            if (ov >= kRespawnOffsetOverridePlus1 && ov < kRespawnOffsetOverridePlus1 + 32) {
+                offset_dir = (ov - kRespawnOffsetOverridePlus1) & 7;
+                slot = ((ov - kRespawnOffsetOverridePlus1) >> 3) & 3;
+                car.respawn_slot_override_plus1 = 0;
+            } else if (ov != 0) {
+                slot = ov - 1;
+                car.respawn_slot_override_plus1 = 0;
+            }
+            if (slot < 0) slot = RespawnStartIndex(rc.seed, rc.env_idx, rc.env_tick, rc.car_id);
+            if (offset_dir >= 0) {
+                RespawnCar(car, *rc.rules, rc.team, slot, cfg.spawn_z, m.boost_spawn_amount, offset_dir);
+            } else if (slot >= kKickoffSpawnOverridePlus1 - 1) {
+                KickoffSpawnCar(car, *rc.rules, rc.team, slot - (kKickoffSpawnOverridePlus1 - 1), cfg.spawn_z,
+                                m.boost_spawn_amount);
+            } else {
+                RespawnCar(car, *rc.rules, rc.team, slot, cfg.spawn_z, m.boost_spawn_amount, -1);
+            }
+            respawned = true;
+        } else if (!(Max(0.0f, car.demo_respawn_timer - dt) > 0.0f) &&
+                   !(car.respawn_slot_override_plus1 >= kKickoffSpawnOverridePlus1 &&
+                     car.respawn_slot_override_plus1 < kKickoffSpawnOverridePlus1 + 5)) {
+            // The car respawns next tick (not as a kickoff re-creation): it shows the spawn boost already.
             car.boost = m.boost_spawn_amount;
             car.boost_amount = car.boost / 100.0f;
         }
     }
+    const Quat spawn_rot = car.rot;
     car.tick_start_pos_bt = car.pos_bt;
     // 2. caps and quantization.
     TickStartLimitAndQuantize(car, m);
     // 3. finish.
+    if (respawned) SpawnInertiaBasis(spawn_rot, car.tick_start_basis_forward, car.tick_start_basis_right,
+                                     car.tick_start_basis_up);
     const Quat start_rot = car.rot;
     const Vec3 start_vel_uu = car.vel;
     FlipZDampPrepass(car, m, dt);

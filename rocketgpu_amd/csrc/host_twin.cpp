@@ -265,21 +265,32 @@ int rgt_step_with_commands(void* p, const void* commands, int stride, int ticks)
     return 0;
 }
 
-// The car pre-tick on n independent cars with the default car parameters (wheels may be null).  Returns the number
-// of cars it does not step (respawning or three-wheel); those are left as they were.
-long rgt_car_pretick(long n, void* cars, const void* controls, const void* configs, void* wheels) {
+// The car pre-tick over a batch of n_env envs x max_cars slots on PA(scale, fillet) (car slot k of env e at
+// e * max_cars + k; wheels may be null; envs give the tick, teams one byte per slot), with the respawn seed.  Returns
+// the number of cars it does not step (three-wheel bodies); those are left as they were.
+long rgt_car_pretick(int n_env, int max_cars, double scale, double fillet, uint64_t seed, void* cars,
+                     const void* controls, const void* configs, void* wheels, const void* envs, const uint8_t* teams) {
     const core::CarParams m{};
+    rgw::ArenaParams ap;
+    ap.scale = scale;
+    ap.fillet_radius = fillet;
+    const core::CarSpawnRules rules = SpawnRules(rgw::ProceduralArenaRules(ap));
     CarState* c = static_cast<CarState*>(cars);
     const CarControls* ctl = static_cast<const CarControls*>(controls);
     const CarConfig* cc = static_cast<const CarConfig*>(configs);
     WheelRuntimeState* w = static_cast<WheelRuntimeState*>(wheels);
+    const EnvState* ev = static_cast<const EnvState*>(envs);
     long skipped = 0;
-    for (long k = 0; k < n; ++k) {
-        if (!core::CarPreTickSupported(c[k], cc[k], m)) {
-            ++skipped;
-            continue;
+    for (int e = 0; e < n_env; ++e) {
+        for (int k = 0; k < max_cars; ++k) {
+            const size_t i = static_cast<size_t>(e) * max_cars + k;
+            if (!core::CarPreTickSupported(c[i], cc[i], m)) {
+                ++skipped;
+                continue;
+            }
+            const core::CarRespawnContext rc{&rules, seed, e, ev[e].tick, static_cast<uint32_t>(i + 1), teams[i]};
+            core::CarPreTick(c[i], ctl[i], cc[i], m, w ? &w[i] : nullptr, rc);
         }
-        core::CarPreTick(c[k], ctl[k], cc[k], m, w ? &w[k] : nullptr);
     }
     return skipped;
 }

@@ -5,30 +5,35 @@
 
 #include <cstdint>
 
-#include "core/rg_car_phases.hpp"
+#include "batch_common.hpp"
 
 using namespace rg;
 
 namespace {
 
-// Car k's pre-tick (StepEnvPreTickCar without respawns); unsupported[k] = 1 marks a car it left alone.
+// Car slot k of env e (lane e * max_cars + k): its pre-tick (StepEnvPreTickCar); unsupported[i] = 1 marks a car
+// it left alone.
 __global__ void __launch_bounds__(64) CarPreTickKernel(CarState* cars, const CarControls* controls,
-                                                       const CarConfig* configs, WheelRuntimeState* wheels, int32_t n,
-                                                       core::CarParams m, uint8_t* unsupported) {
-    const int k = static_cast<int>(blockIdx.x) * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x);
-    if (k >= n) return;
-    CarState car = cars[k];
-    const CarConfig cfg = configs[k];
+                                                       const CarConfig* configs, WheelRuntimeState* wheels,
+                                                       const EnvState* envs, const uint8_t* teams, int32_t n_env,
+                                                       int32_t max_cars, uint64_t seed, core::CarParams m,
+                                                       core::CarSpawnRules rules, uint8_t* unsupported) {
+    const int i = static_cast<int>(blockIdx.x) * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x);
+    if (i >= n_env * max_cars) return;
+    const int e = i / max_cars;
+    CarState car = cars[i];
+    const CarConfig cfg = configs[i];
     if (!core::CarPreTickSupported(car, cfg, m)) {
-        unsupported[k] = 1;
+        unsupported[i] = 1;
         return;
     }
-    unsupported[k] = 0;
+    unsupported[i] = 0;
     WheelRuntimeState ws;
-    if (wheels) ws = wheels[k];
-    core::CarPreTick(car, controls[k], cfg, m, wheels ? &ws : nullptr);
-    cars[k] = car;
-    if (wheels) wheels[k] = ws;
+    if (wheels) ws = wheels[i];
+    const core::CarRespawnContext rc{&rules, seed, e, envs[e].tick, static_cast<uint32_t>(i + 1), teams[i]};
+    core::CarPreTick(car, controls[i], cfg, m, wheels ? &ws : nullptr, rc);
+    cars[i] = car;
+    if (wheels) wheels[i] = ws;
 }
 
 // Car k's post-tick (cooldowns, last controls, bumps and velocity caches, tick counter, reported fields).
@@ -58,16 +63,23 @@ int rgp_car_posttick(void* cars, const void* controls, const void* wheels, int32
     return static_cast<int>(hipGetLastError());
 }
 
-// n cars in place (device pointers; wheels may be null), the default car parameters; unsupported: n device bytes.
-// Returns a hipError_t.
-int rgp_car_pretick(void* cars, const void* controls, const void* configs, void* wheels, int32_t n,
+// n_env x max_cars car slots in place on PA(scale, fillet) (device pointers; wheels may be null), the default car
+// parameters; unsupported: one device byte per slot.  Returns a hipError_t.
+int rgp_car_pretick(void* cars, const void* controls, const void* configs, void* wheels, const void* envs,
+                    const void* teams, int32_t n_env, int32_t max_cars, double scale, double fillet, uint64_t seed,
                     void* unsupported, void* stream) {
+    const int32_t n = n_env * max_cars;
     if (n <= 0) return 0;
     const core::CarParams m{};
+    rgw::ArenaParams ap;
+    ap.scale = scale;
+    ap.fillet_radius = fillet;
+    const core::CarSpawnRules rules = SpawnRules(rgw::ProceduralArenaRules(ap));
     hipLaunchKernelGGL(CarPreTickKernel, dim3((n + 63) / 64), dim3(64), 0, static_cast<hipStream_t>(stream),
                        static_cast<CarState*>(cars), static_cast<const CarControls*>(controls),
-                       static_cast<const CarConfig*>(configs), static_cast<WheelRuntimeState*>(wheels), n, m,
-                       static_cast<uint8_t*>(unsupported));
+                       static_cast<const CarConfig*>(configs), static_cast<WheelRuntimeState*>(wheels),
+                       static_cast<const EnvState*>(envs), static_cast<const uint8_t*>(teams), n_env, max_cars, seed,
+                       m, rules, static_cast<uint8_t*>(unsupported));
     return static_cast<int>(hipGetLastError());
 }
 

@@ -1635,6 +1635,7 @@ field("pos", "<f4", offsetof(rg::CarState, pos), 3);
     RG_F(rg::CarControls, handbrake, "|u1", 0);
     begin("car_configs", sizeof(rg::CarConfig), false);
     RG_F(rg::CarConfig, hitbox_size, "<f4", 3);
+    RG_F(rg::CarConfig, hitbox_published, "<f4", 3);
     RG_F(rg::CarConfig, hitbox_offset, "<f4", 3);
     RG_F(rg::CarConfig, front_wheels_offset, "<f4", 3);
     RG_F(rg::CarConfig, back_wheels_offset, "<f4", 3);
@@ -1644,6 +1645,10 @@ field("pos", "<f4", offsetof(rg::CarState, pos), 3);
     RG_F(rg::CarConfig, back_suspension_rest, "<f4", 0);
     RG_F(rg::CarConfig, dodge_deadzone, "<f4", 0);
     RG_F(rg::CarConfig, three_wheels, "|u1", 0);
+    RG_F(rg::CarConfig, body, "<u2", 0);
+    RG_F(rg::CarConfig, steer_angle_curve_deg, "<f4", 6);
+    RG_F(rg::CarConfig, powerslide_steer_angle_curve_deg, "<f4", 2);
+    RG_F(rg::CarConfig, spawn_z, "<f4", 0);
     begin("teams", 1, false);
     field("team", "|u1", 0, 0);
     begin("env_car_count", sizeof(int32_t), false);

@@ -55,7 +55,8 @@ def twin():
     T.rgt_step_with_commands.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int]
     T.rgt_set_event_stream.argtypes = [vp, vp, vp, ctypes.c_uint32, ctypes.c_int]
     T.rgt_set_event_stream.restype = None
-    T.rgt_car_pretick.argtypes = [ctypes.c_long, vp, vp, vp, vp]
+    T.rgt_car_pretick.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_uint64, vp, vp,
+                                  vp, vp, vp, vp]
     T.rgt_car_pretick.restype = ctypes.c_long
     T.rgt_car_params.argtypes = [vp]
     T.rgt_quantize_axis.argtypes = [vp, vp, ctypes.c_long]

@@ -175,35 +175,54 @@ def make_env_commands(name, n, seed, scale, fillet, stream_ticks):
 
 
 CAR_PRETICK_FIXTURES = {
-    # name: (arenas, cars per arena, ticks, seed, rows kept)
-    "car_pretick_1v1_3v3": ((16, 6), (2, 6), 400, 21, 320),
+    # name: (arenas, cars per arena, ticks, seed, batches kept)
+    "car_pretick_3v3": (8, 6, 240, 21, 8),
 }
 
-# CarState.cal_pre_flags bits the pre-tick sets (air jump started, dodge started, dodge changed the velocity) and the
-# byte offsets the fixture selection reads.
-CAL_PRE_FLAGS_OFFSET = 661
+
+def demo_some_cars(cars, seed, lay):
+    """Demoed cars for the respawn branch: a seeded ~20 % of the slots, each with a respawn timer that ends this
+    tick, next tick or later, and a respawn override of none, a respawn spot, a kickoff spot or an offset spot."""
+    rng = np.random.default_rng(seed)
+    c = cars.copy()
+    pick = rng.random(len(c)) < 0.2
+    for i in np.flatnonzero(pick):
+        c[i, lay["is_demoed"]] = 1
+        timer = np.float32(rng.choice([0.004, 0.012, 0.5]))
+        c[i, lay["demo_respawn_timer"]:lay["demo_respawn_timer"] + 4] = np.frombuffer(timer.tobytes(), np.uint8)
+        c[i, lay["respawn_slot_override_plus1"]] = rng.choice([0, 0, 1, 3, 4, 5, 7, 9, 128, 140, 159])
+    return c
 
 
 def make_car_pretick(name, arenas, cars_per_env, ticks, seed, keep):
-    """Pre-tick inputs from car arenas stepped by the reference (1v1 and 3v3), and the reference's pre-tick of each:
-    every row whose pre-tick started a jump, a double jump or a dodge, plus a seeded sample of the rest."""
-    parts = [R.car_arena_inputs(n, c, ticks, seed + i) for i, (n, c) in enumerate(zip(arenas, cars_per_env))]
-    cars, wheels, controls, configs = (np.concatenate([p[k] for p in parts]) for k in range(4))
-    out_cars, out_wheels = R.car_pretick(cars, controls, configs, wheels)
-    flags = out_cars[:, CAL_PRE_FLAGS_OFFSET]
-    events = np.flatnonzero((flags & 7) != 0)
-    rng = np.random.default_rng(seed)
-    rest = np.setdiff1d(np.arange(len(cars)), events)
-    pick = np.sort(np.concatenate([events[: keep // 2], rng.choice(rest, keep - min(len(events), keep // 2),
-                                                                   replace=False)]))
-    meta = {"workload": "car pre-tick inputs from 1v1 and 3v3 car arenas with random held controls",
-            "arenas": list(arenas), "cars_per_env": list(cars_per_env), "ticks": ticks, "seed": seed,
-            "rows": int(len(pick)), "generator": "oracle/_ref reference CPU build (StepEnvPreTickCar)"}
+    """Batches of pre-tick inputs from 3v3 car arenas stepped by the reference, with a seeded fifth of the cars
+    demoed (respawns on and after the tick), and the reference's pre-tick of each batch.  The `keep` ticks kept are
+    those whose pre-tick starts the most jumps, double jumps and dodges."""
+    import rocketgpu_amd as rg
+    lay = {f[0]: f[2] for f in rg.state_layouts()["car_states"]["fields"]}
+    run = R.car_arena_run(arenas, cars_per_env, ticks, seed)
+    score = []
+    for t in range(ticks):
+        c, _ = R.car_pretick(arenas, cars_per_env, seed, run["cars"][t], run["controls"][t], run["configs"],
+                             run["wheels"][t], run["envs"][t], run["teams"])
+        score.append(int(((c[:, lay["cal_pre_flags"]] & 7) != 0).sum()))
+    snaps = sorted(int(t) for t in np.argsort(-np.array(score), kind="stable")[:keep])
+    cars = np.stack([demo_some_cars(run["cars"][t], seed + t, lay) for t in snaps])
+    wheels, controls, envs = run["wheels"][snaps], run["controls"][snaps], run["envs"][snaps]
+    out_cars, out_wheels = [], []
+    for k in range(len(snaps)):
+        c, w = R.car_pretick(arenas, cars_per_env, seed, cars[k], controls[k], run["configs"], wheels[k], envs[k],
+                             run["teams"])
+        out_cars.append(c)
+        out_wheels.append(w)
+    meta = {"workload": "car pre-tick batches from 3v3 car arenas with random held controls, a fifth of the cars "
+                        "demoed", "arenas": arenas, "cars_per_env": cars_per_env, "ticks": ticks, "seed": seed,
+            "snapshots": snaps, "generator": "oracle/_ref reference CPU build (StepEnvPreTickCar)"}
     path = os.path.join(HERE, name + ".npz")
-    np.savez_compressed(path, cars=cars[pick], wheels=wheels[pick], controls=controls[pick], configs=configs[pick],
-                        out_cars=out_cars[pick], out_wheels=out_wheels[pick],
+    np.savez_compressed(path, cars=cars, wheels=wheels, controls=controls, envs=envs, configs=run["configs"],
+                        teams=run["teams"], out_cars=np.stack(out_cars), out_wheels=np.stack(out_wheels),
                         meta=np.frombuffer(json.dumps(meta).encode(), np.uint8))
-    print(f"{name}: {len(pick)} rows ({len(events)} with jump/dodge starts) -> {os.path.getsize(path)} bytes")
+    print(f"{name}: {len(snaps)} batches of {arenas * cars_per_env} cars -> {os.path.getsize(path)} bytes")
 
 
 CAR_POSTTICK_FIXTURES = {
@@ -240,11 +259,18 @@ def perturb_post_inputs(cars, seed):
 def make_car_posttick(name, arenas, cars_per_env, ticks, seed, keep):
     """Post-tick inputs (the pre-tick's outputs of car-arena states, perturbed) and the reference's post-tick of
     each, with and without reported fields."""
-    parts = [R.car_arena_inputs(n, c, ticks, seed + i) for i, (n, c) in enumerate(zip(arenas, cars_per_env))]
-    cars, wheels, controls, configs = (np.concatenate([p[k] for p in parts]) for k in range(4))
-    mid, mid_wheels = R.car_pretick(cars, controls, configs, wheels)
+    cars, wheels, controls = [], [], []
+    for i, (n, c) in enumerate(zip(arenas, cars_per_env)):
+        run = R.car_arena_run(n, c, ticks, seed + i)
+        for t in range(ticks):
+            mc, mw = R.car_pretick(n, c, seed, run["cars"][t], run["controls"][t], run["configs"], run["wheels"][t],
+                                   run["envs"][t], run["teams"])
+            cars.append(mc)
+            wheels.append(mw)
+            controls.append(run["controls"][t])
+    mid, mid_wheels, controls = np.concatenate(cars), np.concatenate(wheels), np.concatenate(controls)
     rng = np.random.default_rng(seed)
-    pick = np.sort(rng.choice(len(cars), keep, replace=False))
+    pick = np.sort(rng.choice(len(mid), keep, replace=False))
     inputs = perturb_post_inputs(mid[pick], seed)
     out_report = R.car_posttick(inputs, controls[pick], mid_wheels[pick], True)
     out_advance = R.car_posttick(inputs, controls[pick], mid_wheels[pick], False)

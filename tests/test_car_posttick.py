@@ -81,14 +81,16 @@ def test_twin_matches_oracle_on_fresh_car_arenas():
     R = oracle_or_skip()
     T = twin()
     from tests.golden.make_fixtures import perturb_post_inputs
-    cars, wheels, controls, configs = R.car_arena_inputs(6, 2, 150, 301)
-    mid, mid_wheels = R.car_pretick(cars, controls, configs, wheels)
-    inputs = perturb_post_inputs(mid, 302)
-    for report in (True, False):
-        want = R.car_posttick(inputs, controls, mid_wheels, report)
-        got = twin_posttick(T, inputs, controls, mid_wheels, report)
-        bad = np.flatnonzero((got != want).any(1))
-        assert bad.size == 0, (report, bad[:10])
+    run = R.car_arena_run(6, 2, 150, 301)
+    for t in range(0, 150, 5):
+        mid, mid_wheels = R.car_pretick(6, 2, 301, run["cars"][t], run["controls"][t], run["configs"],
+                                        run["wheels"][t], run["envs"][t], run["teams"])
+        inputs = perturb_post_inputs(mid, 302 + t)
+        for report in (True, False):
+            want = R.car_posttick(inputs, run["controls"][t], mid_wheels, report)
+            got = twin_posttick(T, inputs, run["controls"][t], mid_wheels, report)
+            bad = np.flatnonzero((got != want).any(1))
+            assert bad.size == 0, (t, report, bad[:10])
 
 
 @pytest.mark.gpu
