@@ -39,6 +39,8 @@ HBM_PEAK_GBS = 8000.0
 def kernel_name(shape):
     """The step kernel a launch shape runs (rocketgpu_amd/csrc/rg_batch.hip)."""
     block, stage, per_wave = shape
+    if stage in (6, 7):
+        return f"StepBallArenasSplitKernel<{32 if stage == 6 else 16}, false>"
     if stage in (4, 5):
         return f"StepBallArenasGroupKernel<{64 // per_wave}, {'true' if stage == 5 else 'false'}, false>"
     return f"StepBallArenasKernel<{stage}, false>"
@@ -61,7 +63,8 @@ def parse():
     p.add_argument("--block", type=int, default=0, help="lanes per workgroup (0: the tuner chooses the shape)")
     p.add_argument("--stage", type=int, default=0,
                    help="state staging with --block: 0 in place, 1 private copy, 2 LDS scratch, 3 LDS state+scratch, "
-                        "4 in place with 64/per-wave lanes per arena sharing its tree walks, 5 as 4 with the scratch in LDS")
+                        "4 in place with 64/per-wave lanes per arena sharing its tree walks, 5 as 4 with the scratch in LDS, "
+                        "6/7 as 5 with 8 lanes per arena and 32/16 for the arenas with the longest contact walks")
     p.add_argument("--per-wave", type=int, default=64, help="arenas per 64-lane wavefront with --block")
     p.add_argument("--no-forms", action="store_true")
     p.add_argument("--no-serve", action="store_true")
@@ -154,7 +157,8 @@ def main():
         key = lambda c: f"block{c[0]}_stage{c[1]}_perwave{c[2]}"
         cands = ([(64, s, w) for s in (0, 1) for w in (64, 32, 16, 8, 4)] + [(128, 0, 64), (256, 0, 64)] +
                  [(64, 2, w) for w in (64, 32, 16, 8)] + [(64, 3, w) for w in (16, 8, 4)] + [(128, 3, 4), (256, 3, 4)] +
-                 [(b, st, w) for st in (4, 5) for b in (64, 128, 256) for w in (16, 8, 4)])
+                 [(b, st, w) for st in (4, 5) for b in (64, 128, 256) for w in (16, 8, 4)] +
+                 [(b, st, 8) for st in (6, 7) for b in (64, 128, 256)])
         for cand in cands:
             sim_b, _, _ = make_batch(rga, np, n, args.max_cars, args.seed)
             try:

@@ -165,11 +165,17 @@ int rg_batch_step_graph(rg_batch* batch, int32_t ticks_per_launch, int32_t launc
 /* Launch shape of the step kernel: lanes per workgroup (64, 128 or 256); staging (0: arena records stepped in place
  * in HBM; 1: records copied into lane-private memory for the launch; 2: records in place, per-tick contact scratch
  * in LDS; 3: records and scratch in LDS; 4: 64/arenas_per_wave lanes share each arena and split its contact walks;
- * 5: as 4, with the per-tick scratch in LDS, one slice per arena); arenas per 64-lane wavefront (64, 32, 16, 8 or
- * 4; stages 4 and 5 take 16, 8 or 4).  A shape whose LDS exceeds the device's per-workgroup limit is rejected
+ * 5: as 4, with the per-tick scratch in LDS, one slice per arena; 6 and 7: as 5 with 8 lanes per arena, and 32 or
+ * 16 for the arenas with the longest contact walks, see rg_batch_set_split); arenas per 64-lane wavefront (64, 32,
+ * 16, 8 or 4; stages 4 and 5 take 16, 8 or 4, stages 6 and 7 take 8).  A shape whose LDS exceeds the device's per-workgroup limit is rejected
  * (RG_EINVAL) and the previous shape kept.  Results are identical for every shape; the benchmark tunes it.
  * Defaults 64, 0, 64. */
 int rg_batch_set_launch_shape(rg_batch* batch, int32_t block, int32_t stage, int32_t arenas_per_wave);
+/* Stages 6 and 7 of the launch shape: before each whole-batch launch the arenas whose last contact walk loaded at
+ * least `rounds` windows (default 26) go to a heavy list of at most env_count / cap_divisor arenas (default 16),
+ * stepped with 32 (stage 6) or 16 (stage 7) lanes each; the rest get 8.  Results are identical for any split.  No
+ * reference counterpart: a tuning knob, like the launch shape. */
+int rg_batch_set_split(rg_batch* batch, int32_t rounds, int32_t cap_divisor);
 
 /* ball_path(samples, ticks, goal_y, out, result, stream)  (rocketgpu.h:1069-1073): every arena's ball stepped in
  * place for samples x ticks ticks; pos and vel after every `ticks` ticks into device_out[env][sample][6] (f32), and

@@ -139,6 +139,7 @@ def lib():
         L.rg_batch_step_graph.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, vp]
         L.rg_batch_ball_path.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_float, vp, vp, vp]
         L.rg_batch_set_launch_shape.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]
+        L.rg_batch_set_split.argtypes = [vp, ctypes.c_int32, ctypes.c_int32]
         L.rg_serve_run.argtypes = [vp, ctypes.POINTER(ServeConfig), ctypes.POINTER(ServeReport), vp]
         L.rg_fold_digests.argtypes = [vp, ctypes.c_int64]
         L.rg_fold_digests.restype = ctypes.c_uint64
@@ -356,6 +357,10 @@ class BatchSim:
         _check(lib().rg_batch_ball_path(self._h, samples, ticks, goal_y, ctypes.c_void_p(out_ptr),
                                         ctypes.c_void_p(result_ptr), _stream_ptr(stream)))
         return ret
+
+    def set_split(self, rounds: int = 26, cap_divisor: int = 16):
+        """Launch stages 6 and 7: the heavy-arena threshold (contact-walk windows) and list capacity divisor."""
+        _check(lib().rg_batch_set_split(self._h, rounds, cap_divisor))
 
     def set_launch_shape(self, block: int = 64, stage: int = 0, arenas_per_wave: int = 64):
         """Lanes per workgroup, state staging (0 in place, 1 lane-private copy), arenas per 64-lane wavefront."""

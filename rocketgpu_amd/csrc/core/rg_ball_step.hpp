@@ -112,7 +112,9 @@ struct BallMeshManifold {
 struct ArenaInternal {
     BroadphaseState bp;
     uint16_t ball_mesh_retained;  // bit m: mesh m's persistent ball manifold may hold points
-    uint16_t pad;
+    // Windows the last tick's contact walk loaded in a group kernel (saturating; 0 after a one-lane kernel).  The
+    // launch uses it to give the arenas with the longest walks more lanes; it does not affect the physics.
+    uint16_t walk_rounds;
     BallMeshManifold ball_mesh[kMaxArenaMeshes];
 };
 
@@ -717,6 +719,7 @@ RG_HD void BallMeshContacts(BallState& ball, const StepParams& p, ArenaInternal&
     const Vec3 qmax_bt = center + Vec3(r_th, r_th, r_th);
     constexpr float kSkip = 0.01f;
     uint32_t next_retained = st.ball_mesh_retained;
+    int walk_rounds = 0;
     for (int mi = 0; mi < mesh.mesh_count; ++mi) {
         const MeshInfo& info = mesh.meshes[mi];
         BallMeshManifold& man = st.ball_mesh[mi];
@@ -758,7 +761,7 @@ RG_HD void BallMeshContacts(BallState& ball, const StepParams& p, ArenaInternal&
             using LeafContact = typename std::conditional<kEvents, LeafContactWithNormal, LeafContactPlain>::type;
             constexpr float kLeafReachMarginBt = 0.01f;
             const SphereNearLeaf near_leaf(info.q, center, r_th + kLeafReachMarginBt);
-            Walk::Run(
+            walk_rounds += Walk::Run(
                 mesh.nodes, info.node_start, info.node_start + info.node_count, qmin, qmax, near_leaf,
                 [&](int32_t local) {
                     BallManifoldPoint np;
@@ -805,6 +808,7 @@ RG_HD void BallMeshContacts(BallState& ball, const StepParams& p, ArenaInternal&
         }
     }
     st.ball_mesh_retained = static_cast<uint16_t>(next_retained);
+    st.walk_rounds = static_cast<uint16_t>(walk_rounds < 65535 ? walk_rounds : 65535);
 
     // Aggregate the contact points in manifold-list order (kind-0 entries), then the rest.
     uint32_t consumed = 0u;
@@ -1119,6 +1123,7 @@ RG_HD void StepBallArena(BallState& ball, EnvState& env, ArenaInternal& st, cons
         any_retained = any_retained || (((st.ball_mesh_retained >> mi) & 1u) != 0u && st.ball_mesh[mi].count != 0);
     }
     const bool dispatched = !sleeping || vehicle_woke || any_retained;
+    st.walk_rounds = 0;
     if (awake_rows && dispatched) {
         BallMeshContacts<ContactWalk, kEvents>(ball, p, st, mesh, t, ext_lin_uu, solver_vel_bt, dt, events);
     }

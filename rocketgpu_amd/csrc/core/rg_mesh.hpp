@@ -130,10 +130,12 @@ struct AcceptAll {
 // commit changes nothing).  Both forms give the same result.  SerialWalk is the one-lane
 // walk; a cooperative device walk lives with the kernels.
 struct SerialWalk {
+    // Returns the walk's dependent memory rounds as a group walk counts them (0: not counted here).
     template <typename Accept, typename SerialLeaf, typename Compute, typename Commit>
-    RG_HD static void Run(const MeshBvhNode* nodes, int begin, int end, const uint16_t* qmin, const uint16_t* qmax,
-                          Accept&& accept, SerialLeaf&& serial_leaf, Compute&&, Commit&&) {
+    RG_HD static int Run(const MeshBvhNode* nodes, int begin, int end, const uint16_t* qmin, const uint16_t* qmax,
+                         Accept&& accept, SerialLeaf&& serial_leaf, Compute&&, Commit&&) {
         WalkMeshBvh(nodes, begin, end, qmin, qmax, serial_leaf, accept);
+        return 0;
     }
 };
 

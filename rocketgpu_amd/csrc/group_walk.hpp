@@ -42,16 +42,19 @@ struct GroupWalk {
         return static_cast<uint32_t>((ballot >> first) & ((1ull << G) - 1ull));
     }
 
+    // Returns the number of windows loaded (the walk's dependent memory rounds).
     template <typename Accept, typename SerialLeaf, typename Compute, typename Commit>
-    __device__ static void Run(const core::MeshBvhNode* nodes, int begin, int end, const uint16_t* qmin,
-                               const uint16_t* qmax, Accept&& accept, SerialLeaf&&, Compute&& compute,
-                               Commit&& commit) {
+    __device__ static int Run(const core::MeshBvhNode* nodes, int begin, int end, const uint16_t* qmin,
+                              const uint16_t* qmax, Accept&& accept, SerialLeaf&&, Compute&& compute,
+                              Commit&& commit) {
         using Result = decltype(compute(int32_t{0}));
         const int lane = static_cast<int>(__lane_id());
         const int k = lane & (G - 1);
         const int first = lane - k;
         int base = begin;
+        int windows = 0;
         while (base < end) {
+            ++windows;
             const int idx = base + k;
             const bool valid = idx < end;
             core::MeshBvhNode node{};
@@ -107,6 +110,7 @@ struct GroupWalk {
             }
             base += far >= 0 ? far_land : pos;
         }
+        return windows;
     }
 };
 

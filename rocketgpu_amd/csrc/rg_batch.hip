@@ -142,16 +142,29 @@ constexpr int kMaxStepBlock = 256;
 //      and split its mesh-tree walks (GroupWalk), which shortens the chain of dependent node loads;
 //   5  as 4, with the arena's per-tick contact scratch in LDS, one slice shared by the arena's lanes (instead of a
 //      private copy on every lane).
+//   6  as 5 with 8 lanes per arena, except that the arenas whose last contact walk loaded the most windows (the
+//      ball in a corner or along a fillet, which set the launch's duration) get 32 lanes, in the launch's first
+//      workgroups; 7  the same with 16 lanes for them.  A whole-batch launch only: a subset launch runs as 5.
 enum StageMode : int {
     kStageInPlace = 0,
     kStagePrivate = 1,
     kStageLdsScratch = 2,
     kStageLds = 3,
     kStageGroup = 4,
-    kStageGroupLds = 5
+    kStageGroupLds = 5,
+    kStageSplit32 = 6,
+    kStageSplit16 = 7
 };
 
-constexpr bool IsGroupStage(int stage) { return stage == kStageGroup || stage == kStageGroupLds; }
+constexpr bool IsSplitStage(int stage) { return stage == kStageSplit32 || stage == kStageSplit16; }
+constexpr bool IsGroupStage(int stage) { return stage == kStageGroup || stage == kStageGroupLds || IsSplitStage(stage); }
+
+// Stages 6 and 7: by default an arena whose last tick's contact walk loaded at least 26 windows (the slowest 0.5 %
+// or so of the benchmark's workload, where the median is 7) is heavy, and at most env_count / 16 of them get the
+// wider groups (rg_batch_set_split tunes both; a sweep of 12-40 windows and env_count / 64 to / 8 on 16,384 arenas
+// put the optimum at 24-28 and / 16).
+constexpr int kSplitRoundsDefault = 26;
+constexpr int kSplitCapDivisorDefault = 16;
 
 struct LdsSlice {
     core::BallTickScratch scratch;
@@ -163,7 +176,9 @@ struct LdsSlice {
 // LDS per arena of a launch (a workgroup holds blockDim / 64 x arenas_per_wave arenas).
 __host__ __device__ constexpr size_t LdsBytesPerLane(int stage) {
     return stage == kStageLds ? sizeof(LdsSlice)
-                              : (stage == kStageLdsScratch || stage == kStageGroupLds ? sizeof(core::BallTickScratch) : 0);
+                              : (stage == kStageLdsScratch || stage == kStageGroupLds || IsSplitStage(stage)
+                                     ? sizeof(core::BallTickScratch)
+                                     : 0);
 }
 
 // One lane per arena; its ticks back to back.  kEvents: the batch has a physics event stream.
@@ -254,8 +269,30 @@ __global__ void __launch_bounds__(64) ResetToKickoffKernel(StepArgs a, uint64_t 
     ClearEnvControls(a, e);
 }
 
-// G lanes per arena (stage 4).  Work item i = thread / G; every lane of the group steps arena e with the same
-// data, so their in-place writes agree.
+template <int G, bool kEvents>
+__device__ inline void StepGroupArenaWith(const StepArgs& a, int e, int n, core::BallTickScratch& scratch) {
+    const core::EventSink events = EnvEvents(a, e, (static_cast<int>(threadIdx.x) % G) == 0);
+    for (int t = 0; t < n; ++t) {
+        // The predictive ray's walk is short and stays serial (every lane walks it, with the same loads).
+        core::StepBallArena<GroupWalk<G>, core::SerialWalk, kEvents>(a.balls[e], a.envs[e], a.internal[e], a.mesh,
+                                                                      a.params, scratch, events);
+    }
+}
+
+// Arena e's n ticks on the G lanes of one group (stages 4-7): every lane steps it with the same data, so their
+// in-place writes agree; the group's first lane writes its events.  kLds: the tick scratch is `slot`, the group's
+// LDS slice; otherwise lane-private.
+template <int G, bool kLds, bool kEvents>
+__device__ inline void StepGroupArena(const StepArgs& a, int e, int n, core::BallTickScratch* slot) {
+    if constexpr (kLds) {
+        StepGroupArenaWith<G, kEvents>(a, e, n, *slot);
+    } else {
+        core::BallTickScratch private_scratch;
+        StepGroupArenaWith<G, kEvents>(a, e, n, private_scratch);
+    }
+}
+
+// G lanes per arena (stages 4 and 5).  Work item i = thread / G.
 // At most 256 registers per lane, so that two waves fit each SIMD (8 arenas per wave fill 8 waves per CU at 16,384
 // arenas).
 template <int G, bool kLdsScratch, bool kEvents>
@@ -268,16 +305,51 @@ StepBallArenasGroupKernel(StepArgs a, int32_t ticks) {
     const int e = a.arena_ids ? a.arena_ids[i] : i;
     if (e < 0 || e >= a.env_count) return;
     const int n = a.arena_ticks ? a.arena_ticks[i] : ticks;
-    core::BallTickScratch private_scratch;
-    core::BallTickScratch& scratch =
-        kLdsScratch ? reinterpret_cast<core::BallTickScratch*>(lds)[static_cast<int>(threadIdx.x) / G] : private_scratch;
-    // The group's first lane writes its events; the others compute the same ones.
-    const core::EventSink events = EnvEvents(a, e, (static_cast<int>(threadIdx.x) % G) == 0);
-    for (int t = 0; t < n; ++t) {
-        // The predictive ray's walk is short and stays serial (every lane walks it, with the same loads).
-        core::StepBallArena<GroupWalk<G>, core::SerialWalk, kEvents>(a.balls[e], a.envs[e], a.internal[e], a.mesh,
-                                                                      a.params, scratch, events);
+    core::BallTickScratch* slot =
+        kLdsScratch ? reinterpret_cast<core::BallTickScratch*>(lds) + static_cast<int>(threadIdx.x) / G : nullptr;
+    StepGroupArena<G, kLdsScratch, kEvents>(a, e, n, slot);
+}
+
+// Stages 6 and 7, whole-batch launches: workgroups [0, heavy_blocks) step the heavy list with GH lanes per arena,
+// the rest the light list with 8.  lists[0..cap) heavy arenas, lists[cap..cap+n) light ones; counts[0] heavy (may
+// exceed cap: those went to the light list), counts[1] light.
+template <int GH, bool kEvents>
+__global__ void __launch_bounds__(kMaxStepBlock) __attribute__((amdgpu_waves_per_eu(2)))
+StepBallArenasSplitKernel(StepArgs a, int32_t ticks, const int32_t* lists, const int32_t* counts, int32_t cap,
+                          int32_t heavy_blocks) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    core::BallTickScratch* const slices = reinterpret_cast<core::BallTickScratch*>(lds);
+    const int b = static_cast<int>(blockIdx.x);
+    if (b < heavy_blocks) {
+        const int i = (b * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x)) / GH;
+        const int heavy = counts[0] < cap ? counts[0] : cap;
+        if (i >= heavy) return;
+        StepGroupArena<GH, true, kEvents>(a, lists[i], ticks, slices + static_cast<int>(threadIdx.x) / GH);
+    } else {
+        const int i = ((b - heavy_blocks) * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x)) / 8;
+        if (i >= counts[1]) return;
+        StepGroupArena<8, true, kEvents>(a, lists[cap + i], ticks, slices + static_cast<int>(threadIdx.x) / 8);
     }
+}
+
+__global__ void __launch_bounds__(64) ClearSplitCountsKernel(int32_t* counts) {
+    if (threadIdx.x < 2) counts[threadIdx.x] = 0;
+}
+
+// Sorts the arenas into the heavy and light lists (any order: arenas step independently).
+__global__ void __launch_bounds__(256) PartitionArenasKernel(const core::ArenaInternal* internal, int32_t n,
+                                                             int32_t rounds, int32_t cap, int32_t* lists,
+                                                             int32_t* counts) {
+    const int e = static_cast<int>(blockIdx.x) * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x);
+    if (e >= n) return;
+    if (internal[e].walk_rounds >= rounds) {
+        const int h = atomicAdd(&counts[0], 1);
+        if (h < cap) {
+            lists[h] = e;
+            return;
+        }
+    }
+    lists[cap + atomicAdd(&counts[1], 1)] = e;
 }
 
 __device__ inline uint64_t Fnv(uint64_t h, const void* data, int n) {
@@ -808,7 +880,7 @@ public:
         block_ = block;
     }
     void SetStage(int stage) {
-        if (stage < kStageInPlace || stage > kStageGroupLds) throw InvalidArgument("stage must be 0 to 5");
+        if (stage < kStageInPlace || stage > kStageSplit16) throw InvalidArgument("stage must be 0 to 7");
         stage_ = stage;
     }
     void SetArenasPerWave(int n) {
@@ -817,10 +889,13 @@ public:
     }
     int LaunchShape() const { return (block_ * 4 + stage_) * 128 + per_wave_; }
 
-    // Rejects a shape whose LDS slices do not fit one workgroup's LDS.
-    void CheckLaunchShape() const {
+    // Rejects a shape whose LDS slices do not fit one workgroup's LDS; allocates what the shape's launches need.
+    void CheckLaunchShape() {
         if (IsGroupStage(stage_) && per_wave_ != 16 && per_wave_ != 8 && per_wave_ != 4) {
             throw InvalidArgument("stages 4 and 5 need 16, 8 or 4 arenas per wave (4, 8 or 16 lanes per arena)");
+        }
+        if (IsSplitStage(stage_) && per_wave_ != 8) {
+            throw InvalidArgument("stages 6 and 7 need 8 arenas per wave (8 lanes per light arena)");
         }
         const size_t lds = LdsBytesPerLane(stage_) * static_cast<size_t>(block_ / 64) * per_wave_;
         int device = 0, limit = 0;
@@ -834,6 +909,21 @@ public:
             AllowLargeLds<false>(limit);
             AllowLargeLds<true>(limit);
         }
+        // The split stages' lists are allocated here, not at launch: a launch may be under graph capture.
+        if (IsSplitStage(stage_)) {
+            const size_t n = static_cast<size_t>(cfg_.env_count);
+            if (split_lists_.size() < n + n) split_lists_.Resize(n + n);
+            if (split_counts_.size() < 2) split_counts_.Resize(2);
+        }
+    }
+    // Stages 6 and 7: the windows a contact walk must load for its arena to count as heavy, and the heavy list's
+    // capacity as a divisor of env_count.
+    void SetSplit(int rounds, int cap_divisor) {
+        if (rounds < 1 || rounds > 65535) throw InvalidArgument("split rounds must be 1 to 65535");
+        if (cap_divisor < 1 || cap_divisor > 65536) throw InvalidArgument("split capacity divisor must be 1 to 65536");
+        split_rounds_ = rounds;
+        split_cap_divisor_ = cap_divisor;
+        DestroyGraph();
     }
     int block() const { return block_; }
     int stage() const { return stage_; }
@@ -1059,21 +1149,52 @@ private:
                                  reinterpret_cast<const void*>(&StepBallArenasKernel<kStageLds, kEvents>),
                                  reinterpret_cast<const void*>(&StepBallArenasGroupKernel<4, true, kEvents>),
                                  reinterpret_cast<const void*>(&StepBallArenasGroupKernel<8, true, kEvents>),
-                                 reinterpret_cast<const void*>(&StepBallArenasGroupKernel<16, true, kEvents>)};
+                                 reinterpret_cast<const void*>(&StepBallArenasGroupKernel<16, true, kEvents>),
+                                 reinterpret_cast<const void*>(&StepBallArenasSplitKernel<32, kEvents>),
+                                 reinterpret_cast<const void*>(&StepBallArenasSplitKernel<16, kEvents>)};
         for (const void* k : kernels) {
             Check(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, limit), "hipFuncSetAttribute");
         }
     }
 
+    // Stages 6 and 7 over the whole batch: clear the list counts, sort the arenas by their last contact walk, and
+    // step both lists in one launch (the heavy arenas' workgroups first, so they start first).
+    template <bool kEvents>
+    void LaunchSplit(const StepArgs& a, int32_t ticks, hipStream_t s) {
+        const int n = cfg_.env_count;
+        // The heavy list holds at most env_count arenas (cap_divisor >= 1): the lists fit the 2 env_count allocation.
+        const int cap = n / split_cap_divisor_ > 1 ? n / split_cap_divisor_ : 1;
+        hipLaunchKernelGGL(ClearSplitCountsKernel, dim3(1), dim3(64), 0, s, split_counts_.data());
+        hipLaunchKernelGGL(PartitionArenasKernel, dim3((n + 255) / 256), dim3(256), 0, s, internal_.data(), n,
+                           split_rounds_, cap, split_lists_.data(), split_counts_.data());
+        const int gh = stage_ == kStageSplit32 ? 32 : 16;
+        const int heavy_blocks = (cap * gh + block_ - 1) / block_;
+        const int light_blocks = static_cast<int>((static_cast<int64_t>(n) * 8 + block_ - 1) / block_);
+        const size_t lds = sizeof(core::BallTickScratch) * static_cast<size_t>(block_ / 8);
+        const dim3 grid(static_cast<unsigned>(heavy_blocks + light_blocks)), block(block_);
+        if (gh == 32) {
+            hipLaunchKernelGGL((StepBallArenasSplitKernel<32, kEvents>), grid, block, lds, s, a, ticks,
+                               split_lists_.data(), split_counts_.data(), cap, heavy_blocks);
+        } else {
+            hipLaunchKernelGGL((StepBallArenasSplitKernel<16, kEvents>), grid, block, lds, s, a, ticks,
+                               split_lists_.data(), split_counts_.data(), cap, heavy_blocks);
+        }
+        Check(hipGetLastError(), "StepBallArenasSplitKernel launch");
+    }
+
     template <bool kEvents>
     void LaunchStep(const StepArgs& a, int32_t count, int32_t ticks, hipStream_t s) {
+        if (IsSplitStage(stage_) && a.arena_ids == nullptr && a.arena_ticks == nullptr) {
+            LaunchSplit<kEvents>(a, ticks, s);
+            return;
+        }
         if (IsGroupStage(stage_)) {
-            // 64 / per_wave lanes per arena.
+            // 64 / per_wave lanes per arena (a subset launch of stage 6 or 7 runs as stage 5).
             const int lanes = 64 / per_wave_;
             const int64_t threads = static_cast<int64_t>(count) * lanes;
             const dim3 grid(static_cast<unsigned>((threads + block_ - 1) / block_)), block(block_);
             const size_t lds = LdsBytesPerLane(stage_) * static_cast<size_t>(block_ / lanes);
-            const bool in_lds = stage_ == kStageGroupLds;
+            const bool in_lds = stage_ == kStageGroupLds || IsSplitStage(stage_);
             switch (lanes) {
             case 4:
                 if (in_lds) hipLaunchKernelGGL((StepBallArenasGroupKernel<4, true, kEvents>), grid, block, lds, s, a, ticks);
@@ -1142,6 +1263,10 @@ private:
     bool settled_ready_ = false;
     EventStream events_{};
     DeviceArray<uint64_t> digests_;
+    DeviceArray<int32_t> split_lists_;   // stages 6 and 7: heavy then light arena ids
+    DeviceArray<int32_t> split_counts_;
+    int split_rounds_ = kSplitRoundsDefault;
+    int split_cap_divisor_ = kSplitCapDivisorDefault;
     int block_ = 64;
     int stage_ = kStageInPlace;
     int per_wave_ = 64;
@@ -1300,6 +1425,10 @@ int rg_batch_set_car_configs(rg_batch* b, const void* host_configs, size_t count
 
 int rg_batch_set_kickoff_settled(rg_batch* b, int32_t on) {
     return Guard([&] { b->sim->SetKickoffSettled(on != 0); });
+}
+
+int rg_batch_set_split(rg_batch* b, int32_t rounds, int32_t cap_divisor) {
+    return Guard([&] { b->sim->SetSplit(rounds, cap_divisor); });
 }
 
 int rg_batch_set_physics_event_stream(rg_batch* b, void* device_records, void* device_windows,

@@ -101,6 +101,35 @@ def test_launch_shapes_match_reference(block, stage, per_wave):
     assert np.array_equal(envs, f["final_env"])
 
 
+@pytest.mark.parametrize("block,stage", [(64, 6), (128, 6), (256, 6), (64, 7), (256, 7)])
+def test_split_launches_match_reference(block, stage):
+    """Stages 6 and 7 sort the arenas into heavy and light lists before every launch (by the windows their last
+    contact walk loaded) and give the heavy ones 32 or 16 lanes: 375 launches of 8 ticks give the reference's result,
+    and so does a subset launch (which runs as stage 5)."""
+    _torch()
+    f = load_fixture("ball_arena_1024_t3000")
+    sim = make_sim(f)
+    sim.set_launch_shape(block, stage, 8)
+    for _ in range(375):
+        sim.step(8)
+    balls = sim.download_state_array("ball_states")
+    envs = sim.download_state_array("env_states")
+    sim.close()
+    assert np.array_equal(balls, f["final_ball"])
+    assert np.array_equal(envs, f["final_env"])
+
+
+def test_split_stage_rejects_other_group_sizes():
+    import rocketgpu_amd as rga
+    f = load_fixture("ball_arena_96_t1000")
+    sim = make_sim(f)
+    with pytest.raises(rga.RocketGpuError, match="stages 6 and 7 need 8 arenas per wave"):
+        sim.set_launch_shape(64, 6, 4)
+    with pytest.raises(rga.RocketGpuError, match="stage must be 0 to 7"):
+        sim.set_launch_shape(64, 8, 8)
+    sim.close()
+
+
 def test_serving_results_same_alone_and_in_the_mix():
     """Each client's arenas after the served mix equal the same arenas stepped alone by the same tick count."""
     torch = _torch()
