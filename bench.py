@@ -66,6 +66,9 @@ def parse():
                         "4 in place with 64/per-wave lanes per arena sharing its tree walks, 5 as 4 with the scratch in LDS, "
                         "6/7 as 5 with 8 lanes per arena and 32/16 for the arenas with the longest contact walks")
     p.add_argument("--per-wave", type=int, default=64, help="arenas per 64-lane wavefront with --block")
+    p.add_argument("--split-sweep", action="store_true",
+                   help="time launch stages 6 and 7 over a grid of split thresholds and capacities (block 64) "
+                        "instead of the benchmark, one JSON line")
     p.add_argument("--no-forms", action="store_true")
     p.add_argument("--no-serve", action="store_true")
     p.add_argument("--serve-arenas", type=int, default=262144)
@@ -125,6 +128,35 @@ def cpu_baseline(args, init_ball, init_env, start_tick, ticks):
                       f"{dt:.2f} s, ROCKETGPU_CPU_THREADS={cores}"}
 
 
+def split_sweep(args, rga, np, torch, stream, n, T):
+    """Stages 6 and 7 at block 64 over heavy-arena thresholds (contact-walk windows) and list capacities: a fresh
+    batch per point, the warm-up window, then args.steps launches of T ticks timed with HIP events."""
+    def point(stage, rounds, cap_divisor):
+        sim, _, _ = make_batch(rga, np, n, args.max_cars, args.seed)
+        sim.set_launch_shape(64, stage, 8)
+        sim.set_split(rounds, cap_divisor)
+        for _ in range(args.warmup):
+            sim.step(T, stream=stream)
+        ms = timed(torch, stream, lambda: sim.step(T, stream=stream), args.steps)
+        sim.close()
+        return ms
+    out = {"stage5_ms": None, "points": []}
+    sim, _, _ = make_batch(rga, np, n, args.max_cars, args.seed)
+    sim.set_launch_shape(64, 5, 8)
+    for _ in range(args.warmup):
+        sim.step(T, stream=stream)
+    out["stage5_ms"] = timed(torch, stream, lambda: sim.step(T, stream=stream), args.steps)
+    sim.close()
+    for stage in (6, 7):
+        for rounds in (16, 20, 24, 28, 32, 40):
+            for cap_divisor in (32, 16, 8):
+                out["points"].append({"stage": stage, "rounds": rounds, "cap_divisor": cap_divisor,
+                                      "ms": point(stage, rounds, cap_divisor)})
+                print(json.dumps(out["points"][-1]), file=sys.stderr, flush=True)
+    print(json.dumps({"split_sweep": out, "arenas": n, "ticks_per_launch": T, "steps": args.steps,
+                      "warmup": args.warmup}))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -146,6 +178,9 @@ def main():
         torch.cuda.synchronize()
         del src, dst
     stream = torch.cuda.Stream()
+    if args.split_sweep:
+        split_sweep(args, rga, np, torch, stream, n, T)
+        return
     g = torch.Generator(device="cuda")
     g.manual_seed(args.seed)
 
