@@ -32,7 +32,9 @@ void rgref_default_wheel_state(void* out) {
 // supersonic_maintain_max_time, jump_immediate_speed, jump_min_time, jump_reset_time_pad, jump_max_time,
 // doublejump_max_delay, flip_torque_time, flip_initial_vel_scale, flip_forward_impulse_max_speed_scale,
 // flip_side_impulse_max_speed_scale, flip_backward_impulse_max_speed_scale, flip_backward_impulse_scale_x,
-// boost_spawn_amount, unlimited_flips, unlimited_double_jumps.  Returns the count written.
+// boost_spawn_amount, unlimited_flips, unlimited_double_jumps, jump_accel, boost_max, boost_used_per_second,
+// boost_min_time, boost_accel_ground, recharge_boost_per_second, recharge_boost_delay, recharge_boost_enabled.
+// Returns the count written.
 int rgref_car_params(float* out) {
     const SimConfig cfg{};
     const MutatorConfig& m = cfg.mutator;
@@ -43,7 +45,9 @@ int rgref_car_params(float* out) {
                        m.flip_forward_impulse_max_speed_scale, m.flip_side_impulse_max_speed_scale,
                        m.flip_backward_impulse_max_speed_scale, m.flip_backward_impulse_scale_x,
                        m.boost_spawn_amount, static_cast<float>(m.unlimited_flips),
-                       static_cast<float>(m.unlimited_double_jumps)};
+                       static_cast<float>(m.unlimited_double_jumps), m.jump_accel, m.boost_max,
+                       m.boost_used_per_second, m.boost_min_time, m.boost_accel_ground, m.recharge_boost_per_second,
+                       m.recharge_boost_delay, static_cast<float>(m.recharge_boost_enabled)};
     std::memcpy(out, v, sizeof(v));
     return static_cast<int>(sizeof(v) / sizeof(float));
 }
@@ -56,6 +60,88 @@ void rgref_quantize_axis(const float* x, float* out, long n) {
 }  // extern "C"
 
 extern "C" {
+
+// A car's force queue as the caller hands it over: the frame's queued sums, jump force and counts, then the first four
+// forces and torques (host_validation_and_contact_debug.hpp:184-212), as plain floats and ints.
+struct ForceQueueRecord {
+    float force[3], ang_accel[3], torque_bt[3], jump_force[3];
+    int32_t has_jump_force, n_force, n_torque;
+    float force_items[4][3], torque_items[4][3];
+};
+static_assert(sizeof(ForceQueueRecord) == 156, "force queue record");
+
+static Vec3 V(const float* f) { return Vec3(f[0], f[1], f[2]); }
+static void Put(float* f, const Vec3& v) {
+    f[0] = v.x;
+    f[1] = v.y;
+    f[2] = v.z;
+}
+
+// UpdateJumpState (phase 0, raycast_wheels_and_contact_prep.hpp:3629-3652) or ApplyBoost (phase 1, :3252-3300) on n
+// independent cars, as the controls step calls them: with a frame, and with the wheel runtime state (so the frame
+// queues the forces) when queues is not null, else without it (the force changes the velocity).  params: tick_dt,
+// car_mass, jump_min_time, jump_reset_time_pad, jump_max_time, jump_accel, boost_max, boost_used_per_second,
+// boost_min_time, boost_accel_ground, recharge_boost_per_second, recharge_boost_delay, recharge_boost_enabled.
+int rgref_car_force_phase(long n, void* cars, const void* controls, void* queues, int phase, const float* params) {
+    if (phase != 0 && phase != 1) return 1;
+    MutatorConfig m = SimConfig{}.mutator;
+    const float dt = params[0];
+    m.car_mass = params[1];
+    m.jump_min_time = params[2];
+    m.jump_reset_time_pad = params[3];
+    m.jump_max_time = params[4];
+    m.jump_accel = params[5];
+    m.boost_max = params[6];
+    m.boost_used_per_second = params[7];
+    m.boost_min_time = params[8];
+    m.boost_accel_ground = params[9];
+    m.recharge_boost_per_second = params[10];
+    m.recharge_boost_delay = params[11];
+    m.recharge_boost_enabled = params[12] != 0.0f ? 1 : 0;
+    CarState* c = static_cast<CarState*>(cars);
+    const CarControls* ctl = static_cast<const CarControls*>(controls);
+    ForceQueueRecord* q = static_cast<ForceQueueRecord*>(queues);
+    for (long k = 0; k < n; ++k) {
+        detail::CarFrame frame{};
+        detail::WheelRuntimeState ws{};
+        if (q) {
+            const ForceQueueRecord& r = q[k];
+            frame.queue_forces = true;
+            frame.queued_force = V(r.force);
+            frame.queued_ang_accel = V(r.ang_accel);
+            frame.queued_torque_bt = V(r.torque_bt);
+            frame.queued_jump_force = V(r.jump_force);
+            frame.has_queued_jump_force = r.has_jump_force != 0;
+            frame.n_force_items = r.n_force;
+            frame.n_torque_items = r.n_torque;
+            for (int i = 0; i < 4; ++i) {
+                frame.force_items[i] = V(r.force_items[i]);
+                frame.torque_items[i] = V(r.torque_items[i]);
+            }
+        }
+        detail::WheelRuntimeState* wsp = q ? &ws : nullptr;
+        if (phase == 0) {
+            detail::UpdateJumpState(c[k], ctl[k], m, dt, wsp, &frame);
+        } else {
+            detail::ApplyBoost(c[k], ctl[k], m, dt, wsp, &frame);
+        }
+        if (q) {
+            ForceQueueRecord& r = q[k];
+            Put(r.force, frame.queued_force);
+            Put(r.ang_accel, frame.queued_ang_accel);
+            Put(r.torque_bt, frame.queued_torque_bt);
+            Put(r.jump_force, frame.queued_jump_force);
+            r.has_jump_force = frame.has_queued_jump_force ? 1 : 0;
+            r.n_force = frame.n_force_items;
+            r.n_torque = frame.n_torque_items;
+            for (int i = 0; i < 4; ++i) {
+                Put(r.force_items[i], frame.force_items[i]);
+                Put(r.torque_items[i], frame.torque_items[i]);
+            }
+        }
+    }
+    return 0;
+}
 
 // StepEnvPostCar (env_step.hpp:6902-6926) on n independent cars without a world-side finish (no solver scratch),
 // reporting fields when report is nonzero.  Car k is slot 0 of env k of a default config; wheels may be null.

@@ -61,6 +61,7 @@ def lib():
             ("rgref_car_pretick_batch", [vp, ctypes.c_uint64, cp, cp, cp, cp, cp, cp]),
             ("rgref_car_params", [cp]),
             ("rgref_car_posttick", [lg, cp, cp, cp, ctypes.c_int]),
+            ("rgref_car_force_phase", [lg, cp, cp, cp, ctypes.c_int, cp]),
         ]:
             f = getattr(L, name)
             f.argtypes = args
@@ -280,6 +281,46 @@ def car_posttick(cars, controls, wheels, report: bool):
     if lib().rgref_car_posttick(c.shape[0], _ptr(c), _ptr(ctl), _ptr(w), 1 if report else 0) != 0:
         raise RuntimeError("rgref_car_posttick failed")
     return c
+
+
+FORCE_QUEUE_BYTES = 156
+# tick_dt, car_mass, jump_min_time, jump_reset_time_pad, jump_max_time, jump_accel, boost_max, boost_used_per_second,
+# boost_min_time, boost_accel_ground, recharge_boost_per_second, recharge_boost_delay, recharge_boost_enabled.
+FORCE_PARAM_NAMES = ("tick_dt", "car_mass", "jump_min_time", "jump_reset_time_pad", "jump_max_time", "jump_accel",
+                     "boost_max", "boost_used_per_second", "boost_min_time", "boost_accel_ground",
+                     "recharge_boost_per_second", "recharge_boost_delay", "recharge_boost_enabled")
+
+
+# The order of car_params() (ref_phases.cpp rgref_car_params).
+CAR_PARAM_NAMES = ("tick_dt", "boost_pad_frame_frac", "car_mass", "car_max_speed", "car_max_ang_speed",
+                   "supersonic_start_speed", "supersonic_maintain_min_speed", "supersonic_maintain_max_time",
+                   "jump_immediate_speed", "jump_min_time", "jump_reset_time_pad", "jump_max_time",
+                   "doublejump_max_delay", "flip_torque_time", "flip_initial_vel_scale",
+                   "flip_forward_impulse_max_speed_scale", "flip_side_impulse_max_speed_scale",
+                   "flip_backward_impulse_max_speed_scale", "flip_backward_impulse_scale_x", "boost_spawn_amount",
+                   "unlimited_flips", "unlimited_double_jumps", "jump_accel", "boost_max", "boost_used_per_second",
+                   "boost_min_time", "boost_accel_ground", "recharge_boost_per_second", "recharge_boost_delay",
+                   "recharge_boost_enabled")
+
+
+def force_params(**override) -> np.ndarray:
+    """A force-phase parameter list: the reference's defaults with the named fields replaced."""
+    d = dict(zip(CAR_PARAM_NAMES, (float(v) for v in car_params())))
+    d.update(override)
+    return np.array([d[k] for k in FORCE_PARAM_NAMES], np.float32)
+
+
+def car_force_phase(phase: int, cars, controls, queues, params):
+    """UpdateJumpState (phase 0) or ApplyBoost (phase 1) on independent cars, queueing on the given force queues
+    (None: the form that changes the velocity).  Returns (cars, queues) after it."""
+    c = np.ascontiguousarray(cars, dtype=np.uint8).copy()
+    q = None if queues is None else np.ascontiguousarray(queues, dtype=np.uint8).copy()
+    ctl = np.ascontiguousarray(controls, dtype=np.uint8)
+    pr = np.ascontiguousarray(params, dtype=np.float32)
+    if lib().rgref_car_force_phase(c.shape[0], _ptr(c), _ptr(ctl), None if q is None else _ptr(q), phase,
+                                   _ptr(pr)) != 0:
+        raise RuntimeError("rgref_car_force_phase failed")
+    return c, q
 
 
 def car_arena_run(n_env: int, cars_per_env: int, ticks: int, seed: int, p_jump: float = 0.25) -> dict:

@@ -61,6 +61,31 @@ inline core::CarSpawnRules SpawnRules(const rgw::ArenaRulesSpec& rules) {
     return r;
 }
 
+// The number of floats in a force-phase parameter list.
+constexpr int kForcePhaseParamCount = 13;
+
+// Car parameters from a force-phase list, in this order: tick_dt, car_mass, jump_min_time, jump_reset_time_pad,
+// jump_max_time, jump_accel, boost_max, boost_used_per_second, boost_min_time, boost_accel_ground,
+// recharge_boost_per_second, recharge_boost_delay, recharge_boost_enabled (nonzero: on).  The other fields keep
+// their defaults.
+inline core::CarParams ForcePhaseParams(const float* p) {
+    core::CarParams m{};
+    m.tick_dt = p[0];
+    m.car_mass = p[1];
+    m.jump_min_time = p[2];
+    m.jump_reset_time_pad = p[3];
+    m.jump_max_time = p[4];
+    m.jump_accel = p[5];
+    m.boost_max = p[6];
+    m.boost_used_per_second = p[7];
+    m.boost_min_time = p[8];
+    m.boost_accel_ground = p[9];
+    m.recharge_boost_per_second = p[10];
+    m.recharge_boost_delay = p[11];
+    m.recharge_boost_enabled = p[12] != 0.0f ? 1 : 0;
+    return m;
+}
+
 inline ArenaMeshHost BuildProceduralArenaMesh(const rgw::ArenaParams& params) {
     const rgw::ArenaGeometry g = rgw::BuildProceduralArena(params);
     std::vector<std::vector<Vec3>> verts(1);

@@ -14,6 +14,9 @@
 //
 // Post-tick (CarPostTick): cooldowns, last controls, pending bumps and velocity caches, the tick counter, and the
 // fields the game reports.
+//
+// Force phases of the controls step (CarUpdateJumpState, CarApplyBoost): the jump hold and boost, queueing their
+// forces for the solver.  The wheel rays, suspension, air control and the rest of that step are not here.
 #pragma once
 
 #include "rg_env_commands.hpp"
@@ -22,7 +25,7 @@
 namespace rg {
 namespace core {
 
-// The MutatorConfig fields the car pre-tick reads (rocketgpu.h:708-815), with the reference's defaults.
+// The MutatorConfig fields the car phases read (rocketgpu.h:708-815), with the reference's defaults.
 struct CarParams {
     float tick_dt = 1.0f / 120.0f;
     float frame_frac = 0.5f;  // boost_pad_frame_frac: negative on ticks inside a render frame
@@ -46,6 +49,15 @@ struct CarParams {
     float boost_spawn_amount = 33.3f;
     uint8_t unlimited_flips = 0;
     uint8_t unlimited_double_jumps = 0;
+    // Read by the force phases (jump hold and boost).
+    float jump_accel = 4375.0f / 3.0f;
+    float boost_max = 100.0f;
+    float boost_used_per_second = 33.3f;
+    float boost_min_time = 0.10f;
+    float boost_accel_ground = 2975.0f / 3.0f;
+    float recharge_boost_per_second = 10.0f;
+    float recharge_boost_delay = 0.25f;
+    uint8_t recharge_boost_enabled = 0;
 };
 
 // ------------------------------------------------------------------------------------------------------------
@@ -1118,6 +1130,140 @@ RG_HD void CarPostTick(CarState& car, const CarControls& controls, const CarPara
         ReportCarFields(car, jw, m, dt);
     } else {
         AdvanceCarFields(car, jw, m, dt);
+    }
+}
+
+// ------------------------------------------------------------------------------------------------------------
+// Force phases of the car's controls step: the jump hold (UpdateJumpState, raycast_wheels_and_contact_prep.hpp:
+// 3399-3431, :3629-3652) and boost (ApplyBoost, :3242-3300).  With the wheel runtime state present, as in every batch
+// step, a phase leaves the velocity alone and queues its force on the car's frame for the solver
+// (host_validation_and_contact_debug.hpp:184-212, :726-745); the caller passes that queue.  Without it (a null queue)
+// the force changes the linear velocity at once (ApplyCentralForce, :136-138).
+
+// The frame's force queue: the sums, the number of forces and torques queued with the first four of each in order,
+// and the jump force, which a jump replaces rather than adds.
+struct CarForceQueue {
+    Vec3 force;
+    Vec3 ang_accel;
+    Vec3 torque_bt;
+    Vec3 jump_force;
+    int32_t has_jump_force;
+    int32_t n_force;
+    int32_t n_torque;
+    Vec3 force_items[4];
+    Vec3 torque_items[4];
+};
+static_assert(sizeof(CarForceQueue) == 12 * sizeof(Vec3) + 12, "force queue records are packed");
+
+RG_HD void QueueCarForce(CarForceQueue& q, const Vec3& force_bt) {
+    q.force = q.force + force_bt;
+    if (q.n_force < 4) q.force_items[q.n_force] = force_bt;
+    ++q.n_force;
+}
+
+// v += F dt / m in game units, with the Bullet-unit copy kept in step (arena_geometry.hpp:1816-1821).
+RG_HD void ApplyCarCentralForce(CarState& car, float mass, const Vec3& force_uu, float dt) {
+    const Vec3 v = car.vel + force_uu * (dt / mass);
+    car.vel = v;
+    car.vel_bt = v * (1.0f / 50.0f);
+    car.vel_touched_solve_stage = 1;
+}
+
+// The car's up axis scaled by scale_uu, in Bullet units (host_validation_and_contact_debug.hpp:505-509).
+RG_HD Vec3 GameJumpUpBt(const CarState& car, float scale_uu) {
+    Vec3 f, r, u;
+    GameQuatAxes(car.rot, f, r, u);
+    return Vec3((scale_uu * u.x) * 0.02f, (scale_uu * u.y) * 0.02f, (scale_uu * u.z) * 0.02f);
+}
+
+// Jump hold: a grounded car that is not jumping clears its jump once the reset pad has passed; a jump ends when the
+// minimum time is over and the button is released (or its hold broken) or the maximum time is reached; a jump that
+// goes on pushes along the up axis and counts its time.
+RG_HD void CarUpdateJumpState(CarState& car, const CarControls& controls, const CarParams& m, CarForceQueue* q) {
+    const float dt = m.tick_dt;
+    if (car.is_on_ground && !car.is_jumping &&
+        !(car.has_jumped && car.internal_jump_time < m.jump_min_time + m.jump_reset_time_pad)) {
+        car.has_jumped = 0;
+        car.internal_jump_time = 0.0f;
+    }
+    const bool held = car.internal_jump_time < m.jump_min_time ||
+                      (controls.jump && !car.jump_hold_broken && car.internal_jump_time < m.jump_max_time);
+    if (car.is_jumping && !held) car.is_jumping = 0;
+    if (car.is_jumping) {
+        car.has_jumped = 1;
+        const Vec3 f = GameJumpUpBt(car, m.jump_accel * m.car_mass);
+        if (q) {
+            q->jump_force = f;
+            q->has_jump_force = 1;
+        } else {
+            ApplyCarCentralForce(car, m.car_mass, f * 50.0f, dt);
+        }
+    }
+    if (car.is_jumping || car.has_jumped) car.internal_jump_time += dt;
+}
+
+// The boost fraction kept beside the 0..100 amount: trusted while the two agree (raycast_wheels_and_contact_prep.hpp:
+// 3242-3250).
+RG_HD float CarBoostFraction(const CarState& car) {
+    return car.boost_amount * 100.0f == car.boost ? car.boost_amount : car.boost / 100.0f;
+}
+RG_HD void SetCarBoostFraction(CarState& car, float fraction) {
+    car.boost_amount = fraction;
+    car.boost = fraction * 100.0f;
+}
+
+// Boost: a boosting car spends boost, and keeps boosting while it has boost and either holds the button or has not
+// boosted for the minimum time; a car that pushes accelerates along its forward axis.  Otherwise the boost timers
+// reset and an enabled recharge refills after its delay.  The amount ends clamped to 0..boost_max.  A demoed car
+// (which the tick itself skips) only updates its latch and timers.
+RG_HD void CarApplyBoost(CarState& car, const CarControls& controls, const CarParams& m, CarForceQueue* q) {
+    const float dt = m.tick_dt;
+    car.rlbotv5_boost_hold = 0;
+    if (car.is_demoed) {
+        if (car.boost > 0.0f) {
+            if (car.is_boosting) {
+                car.is_boosting = (controls.boost || car.boosting_time < m.boost_min_time) ? 1 : 0;
+            } else if (controls.boost) {
+                car.is_boosting = 1;
+            }
+        } else {
+            car.is_boosting = 0;
+        }
+        car.boosting_time = 0.0f;
+        car.time_since_boosted += dt;
+        if (m.recharge_boost_enabled && car.time_since_boosted >= m.recharge_boost_delay) {
+            car.boost = Min(m.boost_max, car.boost + m.recharge_boost_per_second * dt);
+        }
+        car.boost = Min(car.boost, m.boost_max);
+        return;
+    }
+    bool push;
+    if (car.is_boosting) {
+        SetCarBoostFraction(car, Max(0.0f, CarBoostFraction(car) - (m.boost_used_per_second / 100.0f) * dt));
+        push = !(car.boost == 0.0f || (!controls.boost && car.boosting_time >= m.boost_min_time));
+    } else {
+        push = controls.boost && car.boost > 0.0f;
+    }
+    car.is_boosting = push ? 1 : 0;
+    if (push) {
+        const Vec3 dir = GameQuatRotate(car.rot, Vec3(1.0f, 0.0f, 0.0f));
+        const Vec3 f = ((m.boost_accel_ground * m.car_mass) * dir) * (1.0f / 50.0f);
+        if (q) {
+            QueueCarForce(*q, f);
+        } else {
+            ApplyCarCentralForce(car, m.car_mass, f * 50.0f, dt);
+        }
+        car.boosting_time += dt;
+        car.time_since_boosted = 0.0f;
+    } else {
+        car.boosting_time = 0.0f;
+        car.time_since_boosted += dt;
+        if (m.recharge_boost_enabled && car.time_since_boosted >= m.recharge_boost_delay) {
+            SetCarBoostFraction(car, CarBoostFraction(car) + (m.recharge_boost_per_second / 100.0f) * dt);
+        }
+    }
+    if (car.boost < 0.0f || car.boost > m.boost_max) {
+        SetCarBoostFraction(car, Min(Max(car.boost, 0.0f), m.boost_max) / 100.0f);
     }
 }
 

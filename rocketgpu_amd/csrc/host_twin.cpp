@@ -304,6 +304,25 @@ void rgt_car_posttick(long n, void* cars, const void* controls, const void* whee
     for (long k = 0; k < n; ++k) core::CarPostTick(c[k], ctl[k], m, w ? &w[k] : nullptr, report != 0);
 }
 
+// One force phase (0: jump hold, 1: boost) on n independent cars, with the parameters of a force-phase list;
+// queues: one force queue per car, or null for the form that changes the velocity at once.  Returns 0, or 1 for an
+// unknown phase.
+int rgt_car_force_phase(long n, void* cars, const void* controls, void* queues, int phase, const float* params) {
+    if (phase != 0 && phase != 1) return 1;
+    const core::CarParams m = ForcePhaseParams(params);
+    CarState* c = static_cast<CarState*>(cars);
+    const CarControls* ctl = static_cast<const CarControls*>(controls);
+    core::CarForceQueue* q = static_cast<core::CarForceQueue*>(queues);
+    for (long k = 0; k < n; ++k) {
+        if (phase == 0) {
+            core::CarUpdateJumpState(c[k], ctl[k], m, q ? &q[k] : nullptr);
+        } else {
+            core::CarApplyBoost(c[k], ctl[k], m, q ? &q[k] : nullptr);
+        }
+    }
+    return 0;
+}
+
 // The game's axis quantization over n inputs.
 void rgt_quantize_axis(const float* x, float* out, long n) {
     for (long i = 0; i < n; ++i) out[i] = core::QuantizeAxisInput(x[i]);
@@ -318,7 +337,10 @@ int rgt_car_params(float* out) {
                        m.doublejump_max_delay, m.flip_torque_time, m.flip_initial_vel_scale,
                        m.flip_forward_impulse_max_speed_scale, m.flip_side_impulse_max_speed_scale,
                        m.flip_backward_impulse_max_speed_scale, m.flip_backward_impulse_scale_x, m.boost_spawn_amount,
-                       static_cast<float>(m.unlimited_flips), static_cast<float>(m.unlimited_double_jumps)};
+                       static_cast<float>(m.unlimited_flips), static_cast<float>(m.unlimited_double_jumps),
+                       m.jump_accel, m.boost_max, m.boost_used_per_second, m.boost_min_time, m.boost_accel_ground,
+                       m.recharge_boost_per_second, m.recharge_boost_delay,
+                       static_cast<float>(m.recharge_boost_enabled)};
     std::memcpy(out, v, sizeof(v));
     return static_cast<int>(sizeof(v) / sizeof(float));
 }

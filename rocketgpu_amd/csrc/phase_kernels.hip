@@ -49,9 +49,41 @@ __global__ void __launch_bounds__(64) CarPostTickKernel(CarState* cars, const Ca
     cars[k] = car;
 }
 
+// Car k's force phase (0: jump hold, 1: boost); queues may be null.
+__global__ void __launch_bounds__(64) CarForcePhaseKernel(CarState* cars, const CarControls* controls,
+                                                          core::CarForceQueue* queues, int32_t n, int32_t phase,
+                                                          core::CarParams m) {
+    const int k = static_cast<int>(blockIdx.x) * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x);
+    if (k >= n) return;
+    CarState car = cars[k];
+    core::CarForceQueue q;
+    if (queues) q = queues[k];
+    if (phase == 0) {
+        core::CarUpdateJumpState(car, controls[k], m, queues ? &q : nullptr);
+    } else {
+        core::CarApplyBoost(car, controls[k], m, queues ? &q : nullptr);
+    }
+    cars[k] = car;
+    if (queues) queues[k] = q;
+}
+
 }  // namespace
 
 extern "C" {
+
+// One force phase (0: jump hold, 1: boost) on n cars in place (device pointers; queues: one force queue per car, or
+// null); params: a host list of the force-phase parameters.  Returns a hipError_t, or hipErrorInvalidValue for an
+// unknown phase or a null parameter list.
+int rgp_car_force_phase(void* cars, const void* controls, void* queues, int32_t n, int32_t phase,
+                        const float* params, void* stream) {
+    if ((phase != 0 && phase != 1) || params == nullptr) return static_cast<int>(hipErrorInvalidValue);
+    if (n <= 0) return 0;
+    const core::CarParams m = ForcePhaseParams(params);
+    hipLaunchKernelGGL(CarForcePhaseKernel, dim3((n + 63) / 64), dim3(64), 0, static_cast<hipStream_t>(stream),
+                       static_cast<CarState*>(cars), static_cast<const CarControls*>(controls),
+                       static_cast<core::CarForceQueue*>(queues), n, phase, m);
+    return static_cast<int>(hipGetLastError());
+}
 
 // n cars in place (device pointers; wheels may be null, and are only read).  Returns a hipError_t.
 int rgp_car_posttick(void* cars, const void* controls, const void* wheels, int32_t n, int32_t report, void* stream) {

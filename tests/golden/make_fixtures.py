@@ -284,9 +284,95 @@ def make_car_posttick(name, arenas, cars_per_env, ticks, seed, keep):
     print(f"{name}: {keep} rows -> {os.path.getsize(path)} bytes")
 
 
+CAR_FORCE_FIXTURES = {
+    # name: (arenas, cars per arena, ticks, seed, rows kept)
+    "car_force_phases": (6, 4, 160, 41, 192),
+}
+
+# (phase, queued, parameter set) of each output: phase 0 the jump hold, 1 boost; parameter set 0 the defaults,
+# 1 the defaults with boost recharge on.
+FORCE_CASES = ((0, 1, 0), (0, 0, 0), (1, 1, 0), (1, 0, 0), (1, 1, 1), (1, 0, 1))
+
+
+def force_param_sets():
+    return np.stack([R.force_params(), R.force_params(recharge_boost_enabled=1.0)])
+
+
+def perturb_force_inputs(cars, controls, seed):
+    """Force-phase inputs that reach every branch: boost amounts at and past the limits (0, tiny, spawn, full,
+    over full, negative) with the kept fraction agreeing or not, boost timers on both sides of the minimum time and
+    the recharge delay, jump timers on both sides of the reset pad, minimum and maximum times, random jump / boost
+    buttons and flags, a few demoed cars, and force queues already holding 0-5 forces.  Returns (cars, controls,
+    queues)."""
+    import rocketgpu_amd as rg
+    lay = {f[0]: f[2] for f in rg.state_layouts()["car_states"]["fields"]}
+    rng = np.random.default_rng(seed)
+    c = cars.copy()
+    k = controls.copy()
+    n = len(c)
+
+    def put(i, name, v):
+        c[i, lay[name]:lay[name] + 4] = np.frombuffer(np.float32(v).tobytes(), np.uint8)
+
+    for i in range(n):
+        boost = np.float32(rng.choice([0.0, 0.2, 33.3, rng.uniform(0, 100), 100.0, 100.5, 120.0, -3.0]))
+        put(i, "boost", boost)
+        put(i, "boost_amount", np.float32(boost / np.float32(100)) if rng.random() < 0.7 else rng.uniform(0, 1))
+        put(i, "boosting_time", rng.choice([0.0, 0.05, 0.0999, 0.1, 0.1001, 0.3]))
+        put(i, "time_since_boosted", rng.choice([0.0, 0.1, 0.2416, 0.2417, 0.25, 0.3, 1.0]))
+        put(i, "internal_jump_time", rng.choice([0.0, 0.01, 0.0249, 0.03, 0.0499, 0.05, 0.06, 0.19, 0.2, 0.25, 0.5]))
+        for name, p in (("is_boosting", 0.5), ("is_on_ground", 0.5), ("is_jumping", 0.4), ("has_jumped", 0.5),
+                        ("jump_hold_broken", 0.2), ("is_demoed", 0.08)):
+            c[i, lay[name]] = 1 if rng.random() < p else 0
+    kc = k.view(R.CONTROLS_DTYPE).reshape(-1)
+    kc["jump"] = rng.random(n) < 0.5
+    kc["boost"] = rng.random(n) < 0.5
+    q = np.zeros((n, R.FORCE_QUEUE_BYTES), np.uint8)
+    qf = q.view(np.float32)
+    qi = q.view(np.int32)
+    qf[:, 0:12] = rng.uniform(-40, 40, (n, 12))
+    qi[:, 12] = rng.random(n) < 0.3
+    qi[:, 13] = rng.integers(0, 6, n)
+    qi[:, 14] = rng.integers(0, 6, n)
+    qf[:, 15:39] = rng.uniform(-40, 40, (n, 24))
+    return c, k, q
+
+
+def make_car_force_phases(name, arenas, cars_per_env, ticks, seed, keep):
+    """Force-phase inputs (the pre-tick's outputs of car-arena states, perturbed) and the reference's jump hold and
+    boost on each, with and without a force queue, under the default parameters and with boost recharge on."""
+    run = R.car_arena_run(arenas, cars_per_env, ticks, seed)
+    cars, controls = [], []
+    for t in range(ticks):
+        mc, _ = R.car_pretick(arenas, cars_per_env, seed, run["cars"][t], run["controls"][t], run["configs"],
+                              run["wheels"][t], run["envs"][t], run["teams"])
+        cars.append(mc)
+        controls.append(run["controls"][t])
+    cars, controls = np.concatenate(cars), np.concatenate(controls)
+    rng = np.random.default_rng(seed)
+    pick = np.sort(rng.choice(len(cars), keep, replace=False))
+    c, k, q = perturb_force_inputs(cars[pick], controls[pick], seed)
+    params = force_param_sets()
+    out_cars, out_queues = [], []
+    for phase, queued, ps in FORCE_CASES:
+        oc, oq = R.car_force_phase(phase, c, k, q if queued else None, params[ps])
+        out_cars.append(oc)
+        out_queues.append(oq if queued else np.zeros_like(q))
+    meta = {"workload": "car force-phase inputs: pre-ticked car-arena states with seeded boost and jump timers, "
+                        "flags, buttons and queued forces", "arenas": arenas, "cars_per_env": cars_per_env,
+            "ticks": ticks, "seed": seed, "rows": int(keep),
+            "generator": "oracle/_ref reference CPU build (UpdateJumpState, ApplyBoost)"}
+    path = os.path.join(HERE, name + ".npz")
+    # Outputs as the XOR with the inputs (mostly zero bytes, so the fixture stays small).
+    np.savez_compressed(path, cars=c, controls=k, queues=q, params=params, cases=np.array(FORCE_CASES, np.int32),
+                        out_cars_xor=np.stack(out_cars) ^ c[None], out_queues=np.stack(out_queues),
+                        meta=np.frombuffer(json.dumps(meta).encode(), np.uint8))
+    print(f"{name}: {keep} rows x {len(FORCE_CASES)} cases -> {os.path.getsize(path)} bytes")
+
+
 def main(names):
     for name in names or (list(FIXTURES) + list(BALL_PATH_FIXTURES) + list(ENV_COMMAND_FIXTURES) +
-                          list(CAR_PRETICK_FIXTURES) + list(CAR_POSTTICK_FIXTURES)):
+                          list(CAR_PRETICK_FIXTURES) + list(CAR_POSTTICK_FIXTURES) + list(CAR_FORCE_FIXTURES)):
         if name in FIXTURES:
             make(name, *FIXTURES[name])
         elif name in ENV_COMMAND_FIXTURES:
@@ -295,6 +381,8 @@ def main(names):
             make_car_pretick(name, *CAR_PRETICK_FIXTURES[name])
         elif name in CAR_POSTTICK_FIXTURES:
             make_car_posttick(name, *CAR_POSTTICK_FIXTURES[name])
+        elif name in CAR_FORCE_FIXTURES:
+            make_car_force_phases(name, *CAR_FORCE_FIXTURES[name])
         else:
             make_ball_path(name, *BALL_PATH_FIXTURES[name])
 

@@ -1,0 +1,271 @@
+"""Car force phases of the controls step: the jump hold (UpdateJumpState, include/rocketgpu/detail/step/
+raycast_wheels_and_contact_prep.hpp:3629-3652 with ContinueJump :3399-3431) and boost (ApplyBoost, :3252-3300), with
+the forces queued on the car's frame (the batch form) or applied to the velocity at once.  Checked byte for byte
+against the reference's own functions: the committed fixture (six cases: each phase with and without a queue, boost
+also with recharge on), the live oracle, the host twin on the CPU and the gfx950 kernel on the GPU.  The forces are
+also pinned against the textbook rotation matrix of the car's quaternion, and the boost bookkeeping against its
+closed forms."""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+from conftest import REPO, load_fixture, oracle_or_skip, twin
+
+FIX = "car_force_phases"
+PHASES_PATH = os.path.join(REPO, "rocketgpu_amd", "_build", "librg_phases.so")
+QUEUE_BYTES = 156
+
+
+def layout():
+    import rocketgpu_amd as rg
+    return {f[0]: f[2] for f in rg.state_layouts()["car_states"]["fields"]}
+
+
+def fixture():
+    f = load_fixture(FIX)
+    out_cars = f["out_cars_xor"] ^ f["cars"][None]
+    return f, out_cars
+
+
+def f32(c, off, n=1):
+    return c[:, off:off + 4 * n].view(np.float32)
+
+
+def queue_parts(q):
+    qf, qi = q.view(np.float32), q.view(np.int32)
+    return {"force": qf[:, 0:3], "jump_force": qf[:, 9:12], "has_jump": qi[:, 12], "n_force": qi[:, 13],
+            "n_torque": qi[:, 14], "force_items": qf[:, 15:27].reshape(-1, 4, 3)}
+
+
+def rotation_matrix(rot):
+    """Rows of the rotation matrices of unit quaternions (w, x, y, z), in float64 (the textbook form)."""
+    w, x, y, z = (rot[:, i].astype(np.float64) for i in range(4))
+    return np.stack([
+        np.stack([1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)], 1),
+        np.stack([2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)], 1),
+        np.stack([2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)], 1)], 1)
+
+
+def twin_phase(T, phase, cars, controls, queues, params):
+    c = np.ascontiguousarray(cars).copy()
+    q = None if queues is None else np.ascontiguousarray(queues).copy()
+    k = np.ascontiguousarray(controls)
+    p = np.ascontiguousarray(params, dtype=np.float32)
+    assert T.rgt_car_force_phase(len(c), c.ctypes.data, k.ctypes.data, None if q is None else q.ctypes.data, phase,
+                                 p.ctypes.data) == 0
+    return c, q
+
+
+def test_fixture_covers_the_branches():
+    f, out = fixture()
+    lay = layout()
+    c = f["cars"]
+    cases = [tuple(x) for x in f["cases"]]
+    alive = c[:, lay["is_demoed"]] == 0
+    # Jump hold (queued): jumps that go on, jumps that end, grounded cars whose jump is cleared.
+    j = out[cases.index((0, 1, 0))]
+    jq = queue_parts(f["out_queues"][cases.index((0, 1, 0))])
+    was = c[:, lay["is_jumping"]] == 1
+    assert ((jq["has_jump"] == 1) & (f["queues"].view(np.int32)[:, 12] == 0)).sum() >= 10
+    assert (was & (j[:, lay["is_jumping"]] == 0)).sum() >= 10
+    assert ((c[:, lay["has_jumped"]] == 1) & (j[:, lay["has_jumped"]] == 0)).sum() >= 3
+    # Boost (queued, defaults): pushes, latches that expire, tanks that run dry, clamps.
+    b = out[cases.index((1, 1, 0))]
+    bq = queue_parts(f["out_queues"][cases.index((1, 1, 0))])
+    pushed = bq["n_force"] > f["queues"].view(np.int32)[:, 13]
+    assert (pushed & alive).sum() >= 30
+    assert ((c[:, lay["is_boosting"]] == 1) & (b[:, lay["is_boosting"]] == 0) & alive).sum() >= 10
+    assert ((f32(c, lay["boost"])[:, 0] > 100) & alive).sum() >= 5
+    assert ((f32(c, lay["boost"])[:, 0] < 0) & alive).sum() >= 5
+    # Recharge on: some idle cars refill.
+    r = out[cases.index((1, 1, 1))]
+    assert ((f32(r, lay["boost"])[:, 0] > f32(b, lay["boost"])[:, 0]) & alive).sum() >= 10
+    assert (~alive).sum() >= 5
+
+
+def test_oracle_reproduces_fixture():
+    R = oracle_or_skip()
+    f, out = fixture()
+    for i, (phase, queued, ps) in enumerate(f["cases"]):
+        c, q = R.car_force_phase(int(phase), f["cars"], f["controls"], f["queues"] if queued else None,
+                                 f["params"][ps])
+        assert np.array_equal(c, out[i]), i
+        if queued:
+            assert np.array_equal(q, f["out_queues"][i]), i
+
+
+def test_oracle_default_parameters_match_the_twin():
+    R = oracle_or_skip()
+    f = load_fixture(FIX)
+    assert np.array_equal(R.force_params(), f["params"][0])
+
+
+@pytest.mark.parametrize("case", range(6))
+def test_twin_matches_fixture(case):
+    T = twin()
+    f, out = fixture()
+    phase, queued, ps = (int(x) for x in f["cases"][case])
+    c, q = twin_phase(T, phase, f["cars"], f["controls"], f["queues"] if queued else None, f["params"][ps])
+    bad = np.flatnonzero((c != out[case]).any(1))
+    assert bad.size == 0, bad[:10]
+    if queued:
+        bad = np.flatnonzero((q != f["out_queues"][case]).any(1))
+        assert bad.size == 0, bad[:10]
+
+
+def test_boost_force_is_the_forward_axis():
+    """A car that boosts queues mass * boost_accel_ground / 50 along the first column of its rotation matrix."""
+    f, out = fixture()
+    lay = layout()
+    cases = [tuple(x) for x in f["cases"]]
+    p = dict(zip(("tick_dt", "car_mass", "jump_min_time", "jump_reset_time_pad", "jump_max_time", "jump_accel",
+                  "boost_max", "boost_used_per_second", "boost_min_time", "boost_accel_ground"), f["params"][0]))
+    q0 = queue_parts(f["queues"])
+    q1 = queue_parts(f["out_queues"][cases.index((1, 1, 0))])
+    rows = np.flatnonzero(q1["n_force"] == q0["n_force"] + 1)
+    assert rows.size >= 30
+    delta = q1["force"][rows].astype(np.float64) - q0["force"][rows]
+    rot = f32(f["cars"], lay["rot"], 4)[rows]
+    want = rotation_matrix(rot)[:, :, 0] * (p["boost_accel_ground"] * p["car_mass"] / 50.0)
+    assert np.allclose(delta, want, rtol=0, atol=2e-3)
+    # The item slot takes the force while the queue holds fewer than four.
+    few = q0["n_force"][rows] < 4
+    items = q1["force_items"][rows[few], q0["n_force"][rows[few]]]
+    # float32 rounding is relative to the whole vector (|F| ~ 3570), so the tolerance is absolute.
+    assert np.allclose(items, want[few], rtol=0, atol=2e-3)
+
+
+def test_jump_force_is_the_up_axis():
+    """A jump that goes on sets the queued jump force to mass * jump_accel / 50 along the third column."""
+    f, out = fixture()
+    lay = layout()
+    cases = [tuple(x) for x in f["cases"]]
+    p = f["params"][0]
+    j = out[cases.index((0, 1, 0))]
+    q1 = queue_parts(f["out_queues"][cases.index((0, 1, 0))])
+    rows = np.flatnonzero(j[:, lay["is_jumping"]] == 1)
+    assert rows.size >= 10
+    assert (q1["has_jump"][rows] == 1).all()
+    rot = f32(f["cars"], lay["rot"], 4)[rows]
+    want = rotation_matrix(rot)[:, :, 2] * (p[5] * p[1] / 50.0)
+    assert np.allclose(q1["jump_force"][rows], want, rtol=0, atol=3e-3)  # |F| ~ 5250
+
+
+def test_unqueued_boost_changes_the_velocity():
+    """Without a queue a boosting car's velocity gains boost_accel_ground * dt along its forward axis (game units),
+    its Bullet-unit copy follows, and cars that do not push keep their velocity."""
+    f, out = fixture()
+    lay = layout()
+    cases = [tuple(x) for x in f["cases"]]
+    p = f["params"][0]
+    b = out[cases.index((1, 0, 0))]
+    bq = queue_parts(f["out_queues"][cases.index((1, 1, 0))])
+    pushed = (bq["n_force"] > queue_parts(f["queues"])["n_force"]) & (f["cars"][:, lay["is_demoed"]] == 0)
+    v0 = f32(f["cars"], lay["vel"], 3).astype(np.float64)
+    v1 = f32(b, lay["vel"], 3)
+    fwd = rotation_matrix(f32(f["cars"], lay["rot"], 4))[:, :, 0]
+    assert np.allclose(v1[pushed] - v0[pushed], fwd[pushed] * (p[9] * p[0]), rtol=0, atol=2e-3)
+    assert np.array_equal(f32(b, lay["vel_bt"], 3)[pushed], (v1[pushed] * np.float32(0.02)).astype(np.float32))
+    assert np.array_equal(v1[~pushed], v0[~pushed].astype(np.float32))
+
+
+def test_boost_spend_and_recharge():
+    """A car that keeps boosting spends boost_used_per_second * dt; with recharge on an idle car past the delay
+    gains recharge_boost_per_second * dt; every live car ends within 0..boost_max."""
+    f, out = fixture()
+    lay = layout()
+    cases = [tuple(x) for x in f["cases"]]
+    p = f["params"][1]
+    dt = float(p[0])
+    c = f["cars"]
+    alive = c[:, lay["is_demoed"]] == 0
+    b0 = f32(c, lay["boost"])[:, 0].astype(np.float64)
+    frac_ok = f32(c, lay["boost_amount"])[:, 0] * np.float32(100) == f32(c, lay["boost"])[:, 0]
+    for case in ((1, 1, 0), (1, 1, 1)):
+        b1 = f32(out[cases.index(case)], lay["boost"])[:, 0]
+        assert ((b1[alive] >= 0) & (b1[alive] <= 100)).all()
+    b1 = f32(out[cases.index((1, 1, 1))], lay["boost"])[:, 0].astype(np.float64)
+    keep = alive & frac_ok & (c[:, lay["is_boosting"]] == 1) & (b0 > 1) & (b0 <= 100) & \
+        (out[cases.index((1, 1, 1))][:, lay["is_boosting"]] == 1)
+    assert keep.sum() >= 5
+    assert np.allclose(b1[keep], b0[keep] - p[7] * dt, rtol=0, atol=2e-4)
+    t1 = f32(out[cases.index((1, 1, 1))], lay["time_since_boosted"])[:, 0]
+    idle = alive & frac_ok & (c[:, lay["is_boosting"]] == 0) & (t1 >= p[11]) & (b0 >= 0) & (b0 < 99)
+    idle &= out[cases.index((1, 1, 1))][:, lay["is_boosting"]] == 0
+    assert idle.sum() >= 5
+    assert np.allclose(b1[idle], b0[idle] + p[10] * dt, rtol=0, atol=2e-4)
+
+
+def test_twin_matches_oracle_on_fresh_inputs():
+    """Fresh perturbations of live car-arena states, under the defaults, with recharge on and with other masses,
+    accelerations and timings, every phase with and without a queue."""
+    R = oracle_or_skip()
+    T = twin()
+    from tests.golden.make_fixtures import perturb_force_inputs
+    run = R.car_arena_run(4, 4, 60, 411)
+    sets = [R.force_params(), R.force_params(recharge_boost_enabled=1.0, recharge_boost_delay=0.1),
+            R.force_params(car_mass=200.0, boost_accel_ground=1000.0, jump_accel=1500.0, boost_min_time=0.2,
+                           jump_max_time=0.1, boost_max=80.0, recharge_boost_enabled=1.0)]
+    for t in range(0, 60, 6):
+        mid, _ = R.car_pretick(4, 4, 411, run["cars"][t], run["controls"][t], run["configs"], run["wheels"][t],
+                               run["envs"][t], run["teams"])
+        c, k, q = perturb_force_inputs(mid, run["controls"][t], 500 + t)
+        for ps in sets:
+            for phase in (0, 1):
+                for queues in (q, None):
+                    wc, wq = R.car_force_phase(phase, c, k, queues, ps)
+                    gc, gq = twin_phase(T, phase, c, k, queues, ps)
+                    bad = np.flatnonzero((gc != wc).any(1))
+                    assert bad.size == 0, (t, phase, queues is None, bad[:10])
+                    if queues is not None:
+                        assert np.array_equal(gq, wq), (t, phase)
+
+
+def test_twin_rejects_an_unknown_phase():
+    T = twin()
+    f = load_fixture(FIX)
+    c = f["cars"][:1].copy()
+    p = f["params"][0].copy()
+    assert T.rgt_car_force_phase(1, c.ctypes.data, f["controls"][:1].ctypes.data, None, 2, p.ctypes.data) == 1
+    assert np.array_equal(c, f["cars"][:1])
+
+
+def gpu_lib():
+    if not os.path.exists(PHASES_PATH):
+        pytest.fail("librg_phases.so is not built")
+    P = ctypes.CDLL(PHASES_PATH)
+    vp = ctypes.c_void_p
+    P.rgp_car_force_phase.argtypes = [vp, vp, vp, ctypes.c_int32, ctypes.c_int32, vp, vp]
+    return P
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", range(6))
+def test_gpu_matches_fixture(case):
+    import torch
+    P = gpu_lib()
+    f, out = fixture()
+    phase, queued, ps = (int(x) for x in f["cases"][case])
+    n = len(f["cars"])
+    cars = torch.from_numpy(f["cars"].copy()).to("cuda")
+    controls = torch.from_numpy(f["controls"].copy()).to("cuda")
+    queues = torch.from_numpy(f["queues"].copy()).to("cuda") if queued else None
+    params = np.ascontiguousarray(f["params"][ps], dtype=np.float32)
+    assert P.rgp_car_force_phase(cars.data_ptr(), controls.data_ptr(), None if queues is None else queues.data_ptr(),
+                                 n, phase, params.ctypes.data, None) == 0
+    torch.cuda.synchronize()
+    bad = np.flatnonzero((cars.cpu().numpy() != out[case]).any(1))
+    assert bad.size == 0, bad[:10]
+    if queued:
+        bad = np.flatnonzero((queues.cpu().numpy() != f["out_queues"][case]).any(1))
+        assert bad.size == 0, bad[:10]
+
+
+@pytest.mark.gpu
+def test_gpu_rejects_an_unknown_phase():
+    P = gpu_lib()
+    p = np.zeros(13, np.float32)
+    assert P.rgp_car_force_phase(None, None, None, 1, 2, p.ctypes.data, None) != 0
+    assert P.rgp_car_force_phase(None, None, None, 1, 0, None, None) != 0
