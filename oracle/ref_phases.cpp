@@ -33,8 +33,9 @@ void rgref_default_wheel_state(void* out) {
 // doublejump_max_delay, flip_torque_time, flip_initial_vel_scale, flip_forward_impulse_max_speed_scale,
 // flip_side_impulse_max_speed_scale, flip_backward_impulse_max_speed_scale, flip_backward_impulse_scale_x,
 // boost_spawn_amount, unlimited_flips, unlimited_double_jumps, jump_accel, boost_max, boost_used_per_second,
-// boost_min_time, boost_accel_ground, recharge_boost_per_second, recharge_boost_delay, recharge_boost_enabled.
-// Returns the count written.
+// boost_min_time, boost_accel_ground, recharge_boost_per_second, recharge_boost_delay, recharge_boost_enabled,
+// car_air_throttle_accel, flip_pitchlock_extra_time, car_air_control_pitch, _yaw, _roll, car_air_damp_pitch, _yaw,
+// _roll.  Returns the count written.
 int rgref_car_params(float* out) {
     const SimConfig cfg{};
     const MutatorConfig& m = cfg.mutator;
@@ -47,7 +48,10 @@ int rgref_car_params(float* out) {
                        m.boost_spawn_amount, static_cast<float>(m.unlimited_flips),
                        static_cast<float>(m.unlimited_double_jumps), m.jump_accel, m.boost_max,
                        m.boost_used_per_second, m.boost_min_time, m.boost_accel_ground, m.recharge_boost_per_second,
-                       m.recharge_boost_delay, static_cast<float>(m.recharge_boost_enabled)};
+                       m.recharge_boost_delay, static_cast<float>(m.recharge_boost_enabled),
+                       m.car_air_throttle_accel, m.flip_pitchlock_extra_time, m.car_air_control_pitch,
+                       m.car_air_control_yaw, m.car_air_control_roll, m.car_air_damp_pitch, m.car_air_damp_yaw,
+                       m.car_air_damp_roll};
     std::memcpy(out, v, sizeof(v));
     return static_cast<int>(sizeof(v) / sizeof(float));
 }
@@ -77,13 +81,19 @@ static void Put(float* f, const Vec3& v) {
     f[2] = v.z;
 }
 
-// UpdateJumpState (phase 0, raycast_wheels_and_contact_prep.hpp:3629-3652) or ApplyBoost (phase 1, :3252-3300) on n
-// independent cars, as the controls step calls them: with a frame, and with the wheel runtime state (so the frame
-// queues the forces) when queues is not null, else without it (the force changes the velocity).  params: tick_dt,
-// car_mass, jump_min_time, jump_reset_time_pad, jump_max_time, jump_accel, boost_max, boost_used_per_second,
-// boost_min_time, boost_accel_ground, recharge_boost_per_second, recharge_boost_delay, recharge_boost_enabled.
-int rgref_car_force_phase(long n, void* cars, const void* controls, void* queues, int phase, const float* params) {
-    if (phase != 0 && phase != 1) return 1;
+// UpdateJumpState (phase 0, raycast_wheels_and_contact_prep.hpp:3629-3652), ApplyBoost (phase 1, :3252-3300) or
+// ApplyAirControl (phase 2, :3048-3241) on n independent cars, as the controls step calls them (env_step.hpp:
+// 3189-3200, :3309-3331, :3418): with the car's frame (MakeCarFrame), and with the wheel runtime state (so the frame
+// queues the forces) when queues is not null, else without it (the force changes the velocity).  Air control reads
+// configs (one CarConfig per car) and flags (bit 0: update air control, bit 1: air throttle), with the tick's
+// pitch-lock restart time and damping angular velocity taken from the car.  params: tick_dt, car_mass,
+// jump_min_time, jump_reset_time_pad, jump_max_time, jump_accel, boost_max, boost_used_per_second, boost_min_time,
+// boost_accel_ground, recharge_boost_per_second, recharge_boost_delay, recharge_boost_enabled,
+// car_air_throttle_accel, flip_torque_time, flip_pitchlock_extra_time, car_air_control_pitch, _yaw, _roll,
+// car_air_damp_pitch, _yaw, _roll.
+int rgref_car_force_phase(long n, void* cars, const void* controls, const void* configs, void* queues, int phase,
+                          int flags, const float* params) {
+    if (phase < 0 || phase > 2 || (phase == 2 && configs == nullptr)) return 1;
     MutatorConfig m = SimConfig{}.mutator;
     const float dt = params[0];
     m.car_mass = params[1];
@@ -98,11 +108,21 @@ int rgref_car_force_phase(long n, void* cars, const void* controls, void* queues
     m.recharge_boost_per_second = params[10];
     m.recharge_boost_delay = params[11];
     m.recharge_boost_enabled = params[12] != 0.0f ? 1 : 0;
+    m.car_air_throttle_accel = params[13];
+    m.flip_torque_time = params[14];
+    m.flip_pitchlock_extra_time = params[15];
+    m.car_air_control_pitch = params[16];
+    m.car_air_control_yaw = params[17];
+    m.car_air_control_roll = params[18];
+    m.car_air_damp_pitch = params[19];
+    m.car_air_damp_yaw = params[20];
+    m.car_air_damp_roll = params[21];
+    const CarConfig* cfg = static_cast<const CarConfig*>(configs);
     CarState* c = static_cast<CarState*>(cars);
     const CarControls* ctl = static_cast<const CarControls*>(controls);
     ForceQueueRecord* q = static_cast<ForceQueueRecord*>(queues);
     for (long k = 0; k < n; ++k) {
-        detail::CarFrame frame{};
+        detail::CarFrame frame = phase == 2 ? detail::MakeCarFrame(c[k], cfg[k], m.car_mass) : detail::CarFrame{};
         detail::WheelRuntimeState ws{};
         if (q) {
             const ForceQueueRecord& r = q[k];
@@ -122,8 +142,11 @@ int rgref_car_force_phase(long n, void* cars, const void* controls, void* queues
         detail::WheelRuntimeState* wsp = q ? &ws : nullptr;
         if (phase == 0) {
             detail::UpdateJumpState(c[k], ctl[k], m, dt, wsp, &frame);
-        } else {
+        } else if (phase == 1) {
             detail::ApplyBoost(c[k], ctl[k], m, dt, wsp, &frame);
+        } else {
+            detail::ApplyAirControl(c[k], ctl[k], cfg[k], m, dt, (flags & 1) != 0, wsp, (flags & 2) != 0,
+                                    c[k].cal_pre_pitchlock_restart_time, &c[k].cal_tick_start_ang_vel, &frame);
         }
         if (q) {
             ForceQueueRecord& r = q[k];

@@ -61,7 +61,7 @@ def lib():
             ("rgref_car_pretick_batch", [vp, ctypes.c_uint64, cp, cp, cp, cp, cp, cp]),
             ("rgref_car_params", [cp]),
             ("rgref_car_posttick", [lg, cp, cp, cp, ctypes.c_int]),
-            ("rgref_car_force_phase", [lg, cp, cp, cp, ctypes.c_int, cp]),
+            ("rgref_car_force_phase", [lg, cp, cp, cp, cp, ctypes.c_int, ctypes.c_int, cp]),
         ]:
             f = getattr(L, name)
             f.argtypes = args
@@ -284,11 +284,13 @@ def car_posttick(cars, controls, wheels, report: bool):
 
 
 FORCE_QUEUE_BYTES = 156
-# tick_dt, car_mass, jump_min_time, jump_reset_time_pad, jump_max_time, jump_accel, boost_max, boost_used_per_second,
-# boost_min_time, boost_accel_ground, recharge_boost_per_second, recharge_boost_delay, recharge_boost_enabled.
+# The order of a force-phase parameter list (ref_phases.cpp rgref_car_force_phase).
 FORCE_PARAM_NAMES = ("tick_dt", "car_mass", "jump_min_time", "jump_reset_time_pad", "jump_max_time", "jump_accel",
                      "boost_max", "boost_used_per_second", "boost_min_time", "boost_accel_ground",
-                     "recharge_boost_per_second", "recharge_boost_delay", "recharge_boost_enabled")
+                     "recharge_boost_per_second", "recharge_boost_delay", "recharge_boost_enabled",
+                     "car_air_throttle_accel", "flip_torque_time", "flip_pitchlock_extra_time",
+                     "car_air_control_pitch", "car_air_control_yaw", "car_air_control_roll", "car_air_damp_pitch",
+                     "car_air_damp_yaw", "car_air_damp_roll")
 
 
 # The order of car_params() (ref_phases.cpp rgref_car_params).
@@ -300,7 +302,9 @@ CAR_PARAM_NAMES = ("tick_dt", "boost_pad_frame_frac", "car_mass", "car_max_speed
                    "flip_backward_impulse_max_speed_scale", "flip_backward_impulse_scale_x", "boost_spawn_amount",
                    "unlimited_flips", "unlimited_double_jumps", "jump_accel", "boost_max", "boost_used_per_second",
                    "boost_min_time", "boost_accel_ground", "recharge_boost_per_second", "recharge_boost_delay",
-                   "recharge_boost_enabled")
+                   "recharge_boost_enabled", "car_air_throttle_accel", "flip_pitchlock_extra_time",
+                   "car_air_control_pitch", "car_air_control_yaw", "car_air_control_roll", "car_air_damp_pitch",
+                   "car_air_damp_yaw", "car_air_damp_roll")
 
 
 def force_params(**override) -> np.ndarray:
@@ -310,15 +314,17 @@ def force_params(**override) -> np.ndarray:
     return np.array([d[k] for k in FORCE_PARAM_NAMES], np.float32)
 
 
-def car_force_phase(phase: int, cars, controls, queues, params):
-    """UpdateJumpState (phase 0) or ApplyBoost (phase 1) on independent cars, queueing on the given force queues
-    (None: the form that changes the velocity).  Returns (cars, queues) after it."""
+def car_force_phase(phase: int, cars, controls, queues, params, configs=None, flags: int = 3):
+    """UpdateJumpState (phase 0), ApplyBoost (phase 1) or ApplyAirControl (phase 2, with one CarConfig per car and
+    flags bit 0 update / bit 1 throttle) on independent cars, queueing on the given force queues (None: the form
+    that changes the velocity).  Returns (cars, queues) after it."""
     c = np.ascontiguousarray(cars, dtype=np.uint8).copy()
     q = None if queues is None else np.ascontiguousarray(queues, dtype=np.uint8).copy()
     ctl = np.ascontiguousarray(controls, dtype=np.uint8)
+    cfg = None if configs is None else np.ascontiguousarray(configs, dtype=np.uint8)
     pr = np.ascontiguousarray(params, dtype=np.float32)
-    if lib().rgref_car_force_phase(c.shape[0], _ptr(c), _ptr(ctl), None if q is None else _ptr(q), phase,
-                                   _ptr(pr)) != 0:
+    if lib().rgref_car_force_phase(c.shape[0], _ptr(c), _ptr(ctl), None if cfg is None else _ptr(cfg),
+                                   None if q is None else _ptr(q), phase, flags, _ptr(pr)) != 0:
         raise RuntimeError("rgref_car_force_phase failed")
     return c, q
 

@@ -62,12 +62,13 @@ inline core::CarSpawnRules SpawnRules(const rgw::ArenaRulesSpec& rules) {
 }
 
 // The number of floats in a force-phase parameter list.
-constexpr int kForcePhaseParamCount = 13;
+constexpr int kForcePhaseParamCount = 22;
 
 // Car parameters from a force-phase list, in this order: tick_dt, car_mass, jump_min_time, jump_reset_time_pad,
 // jump_max_time, jump_accel, boost_max, boost_used_per_second, boost_min_time, boost_accel_ground,
-// recharge_boost_per_second, recharge_boost_delay, recharge_boost_enabled (nonzero: on).  The other fields keep
-// their defaults.
+// recharge_boost_per_second, recharge_boost_delay, recharge_boost_enabled (nonzero: on), car_air_throttle_accel,
+// flip_torque_time, flip_pitchlock_extra_time, car_air_control_pitch, _yaw, _roll, car_air_damp_pitch, _yaw, _roll.
+// The other fields keep their defaults.
 inline core::CarParams ForcePhaseParams(const float* p) {
     core::CarParams m{};
     m.tick_dt = p[0];
@@ -83,6 +84,15 @@ inline core::CarParams ForcePhaseParams(const float* p) {
     m.recharge_boost_per_second = p[10];
     m.recharge_boost_delay = p[11];
     m.recharge_boost_enabled = p[12] != 0.0f ? 1 : 0;
+    m.car_air_throttle_accel = p[13];
+    m.flip_torque_time = p[14];
+    m.flip_pitchlock_extra_time = p[15];
+    m.car_air_control_pitch = p[16];
+    m.car_air_control_yaw = p[17];
+    m.car_air_control_roll = p[18];
+    m.car_air_damp_pitch = p[19];
+    m.car_air_damp_yaw = p[20];
+    m.car_air_damp_roll = p[21];
     return m;
 }
 

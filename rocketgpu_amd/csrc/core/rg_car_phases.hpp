@@ -15,8 +15,9 @@
 // Post-tick (CarPostTick): cooldowns, last controls, pending bumps and velocity caches, the tick counter, and the
 // fields the game reports.
 //
-// Force phases of the controls step (CarUpdateJumpState, CarApplyBoost): the jump hold and boost, queueing their
-// forces for the solver.  The wheel rays, suspension, air control and the rest of that step are not here.
+// Force phases of the controls step (CarUpdateJumpState, CarApplyBoost, CarApplyAirControl): the jump hold, boost
+// and air control, queueing their forces and torques for the solver.  The wheel rays, suspension, friction, flip
+// torque and auto-roll of that step are not here.
 #pragma once
 
 #include "rg_env_commands.hpp"
@@ -58,6 +59,15 @@ struct CarParams {
     float recharge_boost_per_second = 10.0f;
     float recharge_boost_delay = 0.25f;
     uint8_t recharge_boost_enabled = 0;
+    // Read by air control.
+    float car_air_throttle_accel = 200.0f / 3.0f;
+    float flip_pitchlock_extra_time = 0.30f;
+    float car_air_control_pitch = 130.0f;
+    float car_air_control_yaw = 95.0f;
+    float car_air_control_roll = 400.0f;
+    float car_air_damp_pitch = 30.0f;
+    float car_air_damp_yaw = 20.0f;
+    float car_air_damp_roll = 50.0f;
 };
 
 // ------------------------------------------------------------------------------------------------------------
@@ -110,6 +120,19 @@ RG_HD void UeQuatAxes(const Quat& q, Vec3& fwd, Vec3& right, Vec3& up) {
     fwd = Vec3(1.0f - (z2 * q.z + y2 * q.y), z2 * q.w + y2 * q.x, z2 * q.x - y2 * q.w);
     right = Vec3(y2 * q.x - z2 * q.w, 1.0f - (z2 * q.z + x2x), x2w + z2 * q.y);
     up = Vec3(y2 * q.w + z2 * q.x, z2 * q.y - x2w, 1.0f - (y2 * q.y + x2x));
+}
+
+// v rotated by a unit quaternion in the game's form (w^2 - |q|^2) v + 2 (q.v) q + 2w (q x v), summed per component
+// in the game's order (math.hpp:1141-1148).  Zero components of v are multiplied, not skipped, so signed zeros come
+// out as the game's.
+RG_HD Vec3 UeQuatRotate(const Quat& q, const Vec3& v) {
+    const float qq = (q.y * q.y + q.x * q.x) + q.z * q.z;
+    const float d = q.w * q.w - qq;
+    const float w2 = q.w + q.w;
+    const float dot = (v.x * q.x + v.y * q.y) + v.z * q.z;
+    const float t = dot + dot;
+    const float cx = v.z * q.y - v.y * q.z, cy = v.x * q.z - v.z * q.x, cz = v.y * q.x - v.x * q.y;
+    return Vec3((cx * w2 + d * v.x) + t * q.x, (cy * w2 + d * v.y) + t * q.y, (cz * w2 + d * v.z) + t * q.z);
 }
 
 // UE3 rotator units: 65536 per turn; the sine table has 16384 entries (math.hpp:1442-1453).
@@ -208,6 +231,37 @@ RG_HD void InverseRows(const Vec3& a, const Vec3& b, const Vec3& c, Vec3& o0, Ve
 
 RG_HD Vec3 MulRows(const Vec3& r0, const Vec3& r1, const Vec3& r2, const Vec3& v) {
     return Vec3(Dot(r0, v), Dot(r1, v), Dot(r2, v));
+}
+
+// Inverse diagonal inertia in game units: the hitbox's Bullet half extents with the collision margin taken off and
+// put back, scaled to uu (host_validation_and_contact_debug.hpp:43-54, :60-71, :81-87; math.hpp:1554-1560).
+RG_HD Vec3 InvInertiaLocalUu(const CarConfig& cfg, float mass) {
+    const Vec3 sized = cfg.hitbox_published + Vec3(2.5f, 2.5f, 2.5f);
+    const bool has_published = sized.x == cfg.hitbox_size.x && sized.y == cfg.hitbox_size.y && sized.z == cfg.hitbox_size.z;
+    const Vec3 published = has_published ? cfg.hitbox_published : cfg.hitbox_size - Vec3(2.5f, 2.5f, 2.5f);
+    const Vec3 size = (((published * 0.02f) * 0.5f + Vec3(0.025f, 0.025f, 0.025f)) * 50.0f) * 2.0f;
+    const Vec3 half_bt = (size * (1.0f / 50.0f)) / 2.0f;
+    const float safe = 0.1f * Min(half_bt.x, Min(half_bt.y, half_bt.z));
+    const float margin = safe < 0.04f ? safe : 0.04f;
+    const Vec3 half = ((half_bt - Vec3(margin, margin, margin)) + Vec3(margin, margin, margin)) * 50.0f;
+    const float x = half.x * 2.0f, y = half.y * 2.0f, z = half.z * 2.0f;
+    const float x2 = x * x, y2 = y * y, z2 = z * z;
+    const float scale = mass / 12.0f;
+    const Vec3 i(scale * (y2 + z2), scale * (x2 + z2), scale * (x2 + y2));
+    return Vec3(i.x != 0.0f ? 1.0f / i.x : 0.0f, i.y != 0.0f ? 1.0f / i.y : 0.0f, i.z != 0.0f ? 1.0f / i.z : 0.0f);
+}
+
+// World inverse inertia in game units from the tick-start basis, or the restored Bullet rows scaled by 1/50^2
+// (host_validation_and_contact_debug.hpp:89-131).
+RG_HD void InvInertiaWorldRowsUu(const CarState& car, const CarConfig& cfg, float mass, Vec3& r0, Vec3& r1, Vec3& r2) {
+    if (car.restored_inv_inertia_valid) {
+        r0 = car.restored_inv_inertia_bt[0] * 0.0004f;
+        r1 = car.restored_inv_inertia_bt[1] * 0.0004f;
+        r2 = car.restored_inv_inertia_bt[2] * 0.0004f;
+        return;
+    }
+    InvInertiaWorldRows(car.tick_start_basis_forward, car.tick_start_basis_right, car.tick_start_basis_up,
+                        InvInertiaLocalUu(cfg, mass), r0, r1, r2);
 }
 
 // The per-tick car frame the reference builds when the jump logic needs it (MakeCarFrame,
@@ -1161,6 +1215,12 @@ RG_HD void QueueCarForce(CarForceQueue& q, const Vec3& force_bt) {
     ++q.n_force;
 }
 
+RG_HD void QueueCarTorque(CarForceQueue& q, const Vec3& torque_bt) {
+    q.torque_bt = q.torque_bt + torque_bt;
+    if (q.n_torque < 4) q.torque_items[q.n_torque] = torque_bt;
+    ++q.n_torque;
+}
+
 // v += F dt / m in game units, with the Bullet-unit copy kept in step (arena_geometry.hpp:1816-1821).
 RG_HD void ApplyCarCentralForce(CarState& car, float mass, const Vec3& force_uu, float dt) {
     const Vec3 v = car.vel + force_uu * (dt / mass);
@@ -1172,7 +1232,7 @@ RG_HD void ApplyCarCentralForce(CarState& car, float mass, const Vec3& force_uu,
 // The car's up axis scaled by scale_uu, in Bullet units (host_validation_and_contact_debug.hpp:505-509).
 RG_HD Vec3 GameJumpUpBt(const CarState& car, float scale_uu) {
     Vec3 f, r, u;
-    GameQuatAxes(car.rot, f, r, u);
+    UeQuatAxes(car.rot, f, r, u);
     return Vec3((scale_uu * u.x) * 0.02f, (scale_uu * u.y) * 0.02f, (scale_uu * u.z) * 0.02f);
 }
 
@@ -1246,7 +1306,7 @@ RG_HD void CarApplyBoost(CarState& car, const CarControls& controls, const CarPa
     }
     car.is_boosting = push ? 1 : 0;
     if (push) {
-        const Vec3 dir = GameQuatRotate(car.rot, Vec3(1.0f, 0.0f, 0.0f));
+        const Vec3 dir = UeQuatRotate(car.rot, Vec3(1.0f, 0.0f, 0.0f));
         const Vec3 f = ((m.boost_accel_ground * m.car_mass) * dir) * (1.0f / 50.0f);
         if (q) {
             QueueCarForce(*q, f);
@@ -1264,6 +1324,96 @@ RG_HD void CarApplyBoost(CarState& car, const CarControls& controls, const CarPa
     }
     if (car.boost < 0.0f || car.boost > m.boost_max) {
         SetCarBoostFraction(car, Min(Max(car.boost, 0.0f), m.boost_max) / 100.0f);
+    }
+}
+
+// Air control (ApplyAirControl, raycast_wheels_and_contact_prep.hpp:3048-3241) with the tick's inputs: the pitch-lock
+// restart time and the damping angular velocity are the car's own tick-start fields.  Pitch input is locked out while
+// a flip runs and for flip_pitchlock_extra_time after it.  Part one: the input torque about the frame's axes minus
+// the damping, as an angular acceleration (through the game-unit inertia when queued; skipped when queued with an
+// authoritative basis).  Part two, queued only: the same input and damping about the game's quaternion axes in rotator
+// units, each through the inverse of the frame's Bullet rows when one of its components reaches 1e-4.  Then the air
+// throttle (full while boosting) along the game's forward axis.
+RG_HD void CarApplyAirControl(CarState& car, const CarControls& c, const CarConfig& cfg, const CarParams& m,
+                              const CarFrame& frame, bool update_air_control, bool air_throttle, CarForceQueue* q) {
+    if (!update_air_control && !air_throttle) return;
+    const float dt = m.tick_dt;
+    constexpr float kRot = 2.0f * Pi() / 65536.0f;
+    constexpr float kTorqueScale = (2.0f * Pi() / 65536.0f) * 1000.0f;
+    if (update_air_control) {
+        float pitch_scale = 1.0f;
+        if (car.is_flipping) {
+            pitch_scale = 0.0f;
+        } else if (car.has_flipped && car.internal_flip_time < (m.flip_torque_time + m.flip_pitchlock_extra_time)) {
+            pitch_scale = 0.0f;
+        } else if (car.cal_pre_pitchlock_restart_time >= 0.0f &&
+                   car.cal_pre_pitchlock_restart_time < m.flip_pitchlock_extra_time) {
+            pitch_scale = 0.0f;
+        }
+        const Vec3 damp_w = car.cal_tick_start_ang_vel;
+        if (!q || !car.basis_authoritative) {
+            const Vec3 pitch_dir = -frame.right;
+            const Vec3 yaw_dir = frame.up;
+            const Vec3 roll_dir = -frame.fwd;
+            Vec3 torque(0.0f, 0.0f, 0.0f);
+            if (c.pitch != 0.0f || c.yaw != 0.0f || c.roll != 0.0f) {
+                torque = ((c.pitch * pitch_dir * pitch_scale * m.car_air_control_pitch) +
+                          (c.yaw * yaw_dir * m.car_air_control_yaw)) +
+                         (c.roll * roll_dir * m.car_air_control_roll);
+            }
+            const float damp_p = Dot(pitch_dir, damp_w) * m.car_air_damp_pitch *
+                                 (1.0f - Min(Abs(c.pitch * pitch_scale), 1.0f));
+            const float damp_y = Dot(yaw_dir, damp_w) * m.car_air_damp_yaw * (1.0f - Min(Abs(c.yaw), 1.0f));
+            const float damp_r = Dot(roll_dir, damp_w) * m.car_air_damp_roll;
+            const Vec3 damping = (yaw_dir * damp_y + pitch_dir * damp_p) + roll_dir * damp_r;
+            const Vec3 accel = (torque - damping) * kTorqueScale;
+            if (q) {
+                Vec3 i0, i1, i2, w0, w1, w2;
+                InvInertiaWorldRowsUu(car, cfg, m.car_mass, i0, i1, i2);
+                InverseRows(i0, i1, i2, w0, w1, w2);
+                q->ang_accel = q->ang_accel + MulRows(w0, w1, w2, accel);
+            } else {
+                car.ang_vel += accel * dt;
+            }
+        }
+        if (q) {
+            Vec3 f, r, u;
+            UeQuatAxes(car.rot, f, r, u);
+            const float in_pitch = c.pitch * pitch_scale;
+            const float a_p = (m.car_air_control_pitch * -in_pitch) * kRot;
+            const float a_y = (m.car_air_control_yaw * c.yaw) * kRot;
+            const float a_r = (-m.car_air_control_roll * c.roll) * kRot;
+            const Vec3 t_in((((a_p * r.x + 0.0f) + a_y * u.x) + a_r * f.x) * 1000.0f,
+                            (((a_p * r.y + 0.0f) + a_y * u.y) + a_r * f.y) * 1000.0f,
+                            (((a_p * r.z + 0.0f) + a_y * u.z) + a_r * f.z) * 1000.0f);
+            const float d_p = (1.0f - Abs(in_pitch)) * (Dot(damp_w, r) * (m.car_air_damp_pitch * kRot));
+            const float d_y = (1.0f - Abs(c.yaw)) * (Dot(damp_w, u) * (m.car_air_damp_yaw * kRot));
+            const float d_r = (1.0f * Dot(damp_w, f)) * (m.car_air_damp_roll * kRot);
+            const Vec3 t_damp((((0.0f - d_p * r.x) - d_y * u.x) - d_r * f.x) * 1000.0f,
+                              (((0.0f - d_p * r.y) - d_y * u.y) - d_r * f.y) * 1000.0f,
+                              (((0.0f - d_p * r.z) - d_y * u.z) - d_r * f.z) * 1000.0f);
+            constexpr float kGate = 1.0e-4f;
+            const bool apply_in = Abs(t_in.x) >= kGate || Abs(t_in.y) >= kGate || Abs(t_in.z) >= kGate;
+            const bool apply_damp = Abs(t_damp.x) >= kGate || Abs(t_damp.y) >= kGate || Abs(t_damp.z) >= kGate;
+            if (apply_in || apply_damp) {
+                Vec3 w0, w1, w2;
+                InverseRows(frame.inv_row0, frame.inv_row1, frame.inv_row2, w0, w1, w2);
+                if (apply_in) QueueCarTorque(*q, MulRows(w0, w1, w2, t_in));
+                if (apply_damp) QueueCarTorque(*q, MulRows(w0, w1, w2, t_damp));
+            }
+        }
+    }
+    const float throttle = car.is_boosting ? 1.0f : c.throttle;
+    if (throttle != 0.0f && air_throttle) {
+        Vec3 f, r, u;
+        UeQuatAxes(car.rot, f, r, u);
+        const float force = m.car_air_throttle_accel * m.car_mass;
+        const Vec3 force_bt = ((f * force) * throttle) * (1.0f / 50.0f);
+        if (q) {
+            QueueCarForce(*q, force_bt);
+        } else {
+            ApplyCarCentralForce(car, m.car_mass, force_bt * 50.0f, dt);
+        }
     }
 }
 

@@ -470,28 +470,5 @@ RG_HD Vec3 QuatRotate(const Quat& q, const Vec3& v) {
                 rw * cz + rz * cw + rx * cy - ry * cx);
 }
 
-// The game's rotation-matrix columns of a unit quaternion (forward, right, up), with doubled y and z as the game
-// forms them (math.hpp:1133-1139).  No normalisation: the quaternion is taken as unit.
-RG_HD void GameQuatAxes(const Quat& q, Vec3& fwd, Vec3& right, Vec3& up) {
-    const float z2 = q.z + q.z, y2 = q.y + q.y;
-    const float xx2 = (q.x + q.x) * q.x, xw2 = (q.x + q.x) * q.w;
-    fwd = Vec3(1.0f - (z2 * q.z + y2 * q.y), z2 * q.w + y2 * q.x, z2 * q.x - y2 * q.w);
-    right = Vec3(y2 * q.x - z2 * q.w, 1.0f - (z2 * q.z + xx2), xw2 + z2 * q.y);
-    up = Vec3(y2 * q.w + z2 * q.x, z2 * q.y - xw2, 1.0f - (y2 * q.y + xx2));
-}
-
-// v rotated by a unit quaternion in the game's form (w^2 - |q|^2) v + 2 (q.v) q + 2w (q x v), summed per component
-// in the game's order (math.hpp:1141-1148).  Zero components of v are multiplied, not skipped, so signed zeros come
-// out as the game's.
-RG_HD Vec3 GameQuatRotate(const Quat& q, const Vec3& v) {
-    const float qq = (q.y * q.y + q.x * q.x) + q.z * q.z;
-    const float d = q.w * q.w - qq;
-    const float w2 = q.w + q.w;
-    const float dot = (v.x * q.x + v.y * q.y) + v.z * q.z;
-    const float t = dot + dot;
-    const float cx = v.z * q.y - v.y * q.z, cy = v.x * q.z - v.z * q.x, cz = v.y * q.x - v.x * q.y;
-    return Vec3((cx * w2 + d * v.x) + t * q.x, (cy * w2 + d * v.y) + t * q.y, (cz * w2 + d * v.z) + t * q.z);
-}
-
 }  // namespace core
 }  // namespace rg

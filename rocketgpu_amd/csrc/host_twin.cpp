@@ -304,20 +304,27 @@ void rgt_car_posttick(long n, void* cars, const void* controls, const void* whee
     for (long k = 0; k < n; ++k) core::CarPostTick(c[k], ctl[k], m, w ? &w[k] : nullptr, report != 0);
 }
 
-// One force phase (0: jump hold, 1: boost) on n independent cars, with the parameters of a force-phase list;
-// queues: one force queue per car, or null for the form that changes the velocity at once.  Returns 0, or 1 for an
-// unknown phase.
-int rgt_car_force_phase(long n, void* cars, const void* controls, void* queues, int phase, const float* params) {
-    if (phase != 0 && phase != 1) return 1;
+// One force phase (0: jump hold, 1: boost, 2: air control) on n independent cars, with the parameters of a
+// force-phase list; configs: one CarConfig per car (read by air control only, may be null otherwise); flags (air
+// control): bit 0 updates air control, bit 1 applies the air throttle; queues: one force queue per car, or null for
+// the form that changes the velocity at once.  Returns 0, or 1 for an unknown phase or missing configs.
+int rgt_car_force_phase(long n, void* cars, const void* controls, const void* configs, void* queues, int phase,
+                        int flags, const float* params) {
+    if (phase < 0 || phase > 2 || (phase == 2 && configs == nullptr)) return 1;
     const core::CarParams m = ForcePhaseParams(params);
     CarState* c = static_cast<CarState*>(cars);
     const CarControls* ctl = static_cast<const CarControls*>(controls);
+    const CarConfig* cfg = static_cast<const CarConfig*>(configs);
     core::CarForceQueue* q = static_cast<core::CarForceQueue*>(queues);
     for (long k = 0; k < n; ++k) {
+        core::CarForceQueue* qk = q ? &q[k] : nullptr;
         if (phase == 0) {
-            core::CarUpdateJumpState(c[k], ctl[k], m, q ? &q[k] : nullptr);
+            core::CarUpdateJumpState(c[k], ctl[k], m, qk);
+        } else if (phase == 1) {
+            core::CarApplyBoost(c[k], ctl[k], m, qk);
         } else {
-            core::CarApplyBoost(c[k], ctl[k], m, q ? &q[k] : nullptr);
+            const core::CarFrame frame = core::MakeCarFrame(c[k], cfg[k], m.car_mass);
+            core::CarApplyAirControl(c[k], ctl[k], cfg[k], m, frame, (flags & 1) != 0, (flags & 2) != 0, qk);
         }
     }
     return 0;
@@ -340,7 +347,9 @@ int rgt_car_params(float* out) {
                        static_cast<float>(m.unlimited_flips), static_cast<float>(m.unlimited_double_jumps),
                        m.jump_accel, m.boost_max, m.boost_used_per_second, m.boost_min_time, m.boost_accel_ground,
                        m.recharge_boost_per_second, m.recharge_boost_delay,
-                       static_cast<float>(m.recharge_boost_enabled)};
+                       static_cast<float>(m.recharge_boost_enabled), m.car_air_throttle_accel,
+                       m.flip_pitchlock_extra_time, m.car_air_control_pitch, m.car_air_control_yaw,
+                       m.car_air_control_roll, m.car_air_damp_pitch, m.car_air_damp_yaw, m.car_air_damp_roll};
     std::memcpy(out, v, sizeof(v));
     return static_cast<int>(sizeof(v) / sizeof(float));
 }

@@ -49,19 +49,24 @@ __global__ void __launch_bounds__(64) CarPostTickKernel(CarState* cars, const Ca
     cars[k] = car;
 }
 
-// Car k's force phase (0: jump hold, 1: boost); queues may be null.
+// Car k's force phase (0: jump hold, 1: boost, 2: air control); queues may be null.
 __global__ void __launch_bounds__(64) CarForcePhaseKernel(CarState* cars, const CarControls* controls,
-                                                          core::CarForceQueue* queues, int32_t n, int32_t phase,
-                                                          core::CarParams m) {
+                                                          const CarConfig* configs, core::CarForceQueue* queues,
+                                                          int32_t n, int32_t phase, int32_t flags, core::CarParams m) {
     const int k = static_cast<int>(blockIdx.x) * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x);
     if (k >= n) return;
     CarState car = cars[k];
     core::CarForceQueue q;
     if (queues) q = queues[k];
+    core::CarForceQueue* qp = queues ? &q : nullptr;
     if (phase == 0) {
-        core::CarUpdateJumpState(car, controls[k], m, queues ? &q : nullptr);
+        core::CarUpdateJumpState(car, controls[k], m, qp);
+    } else if (phase == 1) {
+        core::CarApplyBoost(car, controls[k], m, qp);
     } else {
-        core::CarApplyBoost(car, controls[k], m, queues ? &q : nullptr);
+        const CarConfig cfg = configs[k];
+        const core::CarFrame frame = core::MakeCarFrame(car, cfg, m.car_mass);
+        core::CarApplyAirControl(car, controls[k], cfg, m, frame, (flags & 1) != 0, (flags & 2) != 0, qp);
     }
     cars[k] = car;
     if (queues) queues[k] = q;
@@ -71,47 +76,21 @@ __global__ void __launch_bounds__(64) CarForcePhaseKernel(CarState* cars, const 
 
 extern "C" {
 
-// One force phase (0: jump hold, 1: boost) on n cars in place (device pointers; queues: one force queue per car, or
-// null); params: a host list of the force-phase parameters.  Returns a hipError_t, or hipErrorInvalidValue for an
-// unknown phase or a null parameter list.
-int rgp_car_force_phase(void* cars, const void* controls, void* queues, int32_t n, int32_t phase,
-                        const float* params, void* stream) {
-    if ((phase != 0 && phase != 1) || params == nullptr) return static_cast<int>(hipErrorInvalidValue);
+// One force phase (0: jump hold, 1: boost, 2: air control) on n cars in place (device pointers; configs: one
+// CarConfig per car, read by air control only; queues: one force queue per car, or null); flags (air control): bit 0
+// updates air control, bit 1 applies the air throttle; params: a host list of the force-phase parameters.  Returns a
+// hipError_t, or hipErrorInvalidValue for an unknown phase, missing configs or a null parameter list.
+int rgp_car_force_phase(void* cars, const void* controls, const void* configs, void* queues, int32_t n,
+                        int32_t phase, int32_t flags, const float* params, void* stream) {
+    if (phase < 0 || phase > 2 || (phase == 2 && configs == nullptr) || params == nullptr) {
+        return static_cast<int>(hipErrorInvalidValue);
+    }
     if (n <= 0) return 0;
     const core::CarParams m = ForcePhaseParams(params);
     hipLaunchKernelGGL(CarForcePhaseKernel, dim3((n + 63) / 64), dim3(64), 0, static_cast<hipStream_t>(stream),
                        static_cast<CarState*>(cars), static_cast<const CarControls*>(controls),
-                       static_cast<core::CarForceQueue*>(queues), n, phase, m);
-    return static_cast<int>(hipGetLastError());
-}
-
-// n cars in place (device pointers; wheels may be null, and are only read).  Returns a hipError_t.
-int rgp_car_posttick(void* cars, const void* controls, const void* wheels, int32_t n, int32_t report, void* stream) {
-    if (n <= 0) return 0;
-    const core::CarParams m{};
-    hipLaunchKernelGGL(CarPostTickKernel, dim3((n + 63) / 64), dim3(64), 0, static_cast<hipStream_t>(stream),
-                       static_cast<CarState*>(cars), static_cast<const CarControls*>(controls),
-                       static_cast<const WheelRuntimeState*>(wheels), n, m, report);
-    return static_cast<int>(hipGetLastError());
-}
-
-// n_env x max_cars car slots in place on PA(scale, fillet) (device pointers; wheels may be null), the default car
-// parameters; unsupported: one device byte per slot.  Returns a hipError_t.
-int rgp_car_pretick(void* cars, const void* controls, const void* configs, void* wheels, const void* envs,
-                    const void* teams, int32_t n_env, int32_t max_cars, double scale, double fillet, uint64_t seed,
-                    void* unsupported, void* stream) {
-    const int32_t n = n_env * max_cars;
-    if (n <= 0) return 0;
-    const core::CarParams m{};
-    rgw::ArenaParams ap;
-    ap.scale = scale;
-    ap.fillet_radius = fillet;
-    const core::CarSpawnRules rules = SpawnRules(rgw::ProceduralArenaRules(ap));
-    hipLaunchKernelGGL(CarPreTickKernel, dim3((n + 63) / 64), dim3(64), 0, static_cast<hipStream_t>(stream),
-                       static_cast<CarState*>(cars), static_cast<const CarControls*>(controls),
-                       static_cast<const CarConfig*>(configs), static_cast<WheelRuntimeState*>(wheels),
-                       static_cast<const EnvState*>(envs), static_cast<const uint8_t*>(teams), n_env, max_cars, seed,
-                       m, rules, static_cast<uint8_t*>(unsupported));
+                       static_cast<const CarConfig*>(configs), static_cast<core::CarForceQueue*>(queues), n, phase,
+                       flags, m);
     return static_cast<int>(hipGetLastError());
 }
 

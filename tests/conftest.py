@@ -63,7 +63,7 @@ def twin():
     T.rgt_quantize_axis.restype = None
     T.rgt_car_posttick.argtypes = [ctypes.c_long, vp, vp, vp, ctypes.c_int]
     T.rgt_car_posttick.restype = None
-    T.rgt_car_force_phase.argtypes = [ctypes.c_long, vp, vp, vp, ctypes.c_int, vp]
+    T.rgt_car_force_phase.argtypes = [ctypes.c_long, vp, vp, vp, vp, ctypes.c_int, ctypes.c_int, vp]
     return T
 
 

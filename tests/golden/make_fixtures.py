@@ -289,9 +289,11 @@ CAR_FORCE_FIXTURES = {
     "car_force_phases": (6, 4, 160, 41, 192),
 }
 
-# (phase, queued, parameter set) of each output: phase 0 the jump hold, 1 boost; parameter set 0 the defaults,
-# 1 the defaults with boost recharge on.
-FORCE_CASES = ((0, 1, 0), (0, 0, 0), (1, 1, 0), (1, 0, 0), (1, 1, 1), (1, 0, 1))
+# (phase, queued, parameter set, flags) of each output: phase 0 the jump hold, 1 boost, 2 air control (flags bit 0
+# updates air control, bit 1 applies the air throttle); parameter set 0 the defaults, 1 the defaults with boost
+# recharge on.
+FORCE_CASES = ((0, 1, 0, 3), (0, 0, 0, 3), (1, 1, 0, 3), (1, 0, 0, 3), (1, 1, 1, 3), (1, 0, 1, 3),
+               (2, 1, 0, 3), (2, 0, 0, 3), (2, 1, 0, 1), (2, 1, 0, 2))
 
 
 def force_param_sets():
@@ -302,8 +304,9 @@ def perturb_force_inputs(cars, controls, seed):
     """Force-phase inputs that reach every branch: boost amounts at and past the limits (0, tiny, spawn, full,
     over full, negative) with the kept fraction agreeing or not, boost timers on both sides of the minimum time and
     the recharge delay, jump timers on both sides of the reset pad, minimum and maximum times, random jump / boost
-    buttons and flags, a few demoed cars, and force queues already holding 0-5 forces.  Returns (cars, controls,
-    queues)."""
+    buttons and flags, a few demoed cars, and force queues already holding 0-5 forces.  For air control: flip
+    flags and flip timers on both sides of the pitch lock, pitch-lock restart times, tick-start angular velocities,
+    authoritative bases, restored inverse-inertia rows and stick axes at zero.  Returns (cars, controls, queues)."""
     import rocketgpu_amd as rg
     lay = {f[0]: f[2] for f in rg.state_layouts()["car_states"]["fields"]}
     rng = np.random.default_rng(seed)
@@ -322,11 +325,23 @@ def perturb_force_inputs(cars, controls, seed):
         put(i, "time_since_boosted", rng.choice([0.0, 0.1, 0.2416, 0.2417, 0.25, 0.3, 1.0]))
         put(i, "internal_jump_time", rng.choice([0.0, 0.01, 0.0249, 0.03, 0.0499, 0.05, 0.06, 0.19, 0.2, 0.25, 0.5]))
         for name, p in (("is_boosting", 0.5), ("is_on_ground", 0.5), ("is_jumping", 0.4), ("has_jumped", 0.5),
-                        ("jump_hold_broken", 0.2), ("is_demoed", 0.08)):
+                        ("jump_hold_broken", 0.2), ("is_demoed", 0.08), ("is_flipping", 0.15), ("has_flipped", 0.4),
+                        ("basis_authoritative", 0.3)):
             c[i, lay[name]] = 1 if rng.random() < p else 0
+        put(i, "internal_flip_time", rng.choice([0.0, 0.5, 0.94, 0.95, 0.96, 1.5]))
+        put(i, "cal_pre_pitchlock_restart_time", rng.choice([-1.0, 0.0, 0.1, 0.29, 0.3, 0.5]))
+        w = rng.uniform(-5, 5, 3).astype(np.float32)
+        c[i, lay["cal_tick_start_ang_vel"]:lay["cal_tick_start_ang_vel"] + 12] = w.view(np.uint8)
+        if rng.random() < 0.1:
+            c[i, lay["restored_inv_inertia_valid"]] = 1
+            rows = rng.uniform(0, 0.3, 9).astype(np.float32)
+            c[i, lay["restored_inv_inertia_bt"]:lay["restored_inv_inertia_bt"] + 36] = rows.view(np.uint8)
     kc = k.view(R.CONTROLS_DTYPE).reshape(-1)
     kc["jump"] = rng.random(n) < 0.5
     kc["boost"] = rng.random(n) < 0.5
+    still = rng.random(n) < 0.15
+    for axis in ("pitch", "yaw", "roll"):
+        kc[axis][still] = 0.0
     q = np.zeros((n, R.FORCE_QUEUE_BYTES), np.uint8)
     qf = q.view(np.float32)
     qi = q.view(np.int32)
@@ -339,8 +354,9 @@ def perturb_force_inputs(cars, controls, seed):
 
 
 def make_car_force_phases(name, arenas, cars_per_env, ticks, seed, keep):
-    """Force-phase inputs (the pre-tick's outputs of car-arena states, perturbed) and the reference's jump hold and
-    boost on each, with and without a force queue, under the default parameters and with boost recharge on."""
+    """Force-phase inputs (the pre-tick's outputs of car-arena states, perturbed, with each car's config) and the
+    reference's jump hold, boost and air control on each, with and without a force queue, under the default
+    parameters and with boost recharge on."""
     run = R.car_arena_run(arenas, cars_per_env, ticks, seed)
     cars, controls = [], []
     for t in range(ticks):
@@ -349,22 +365,25 @@ def make_car_force_phases(name, arenas, cars_per_env, ticks, seed, keep):
         cars.append(mc)
         controls.append(run["controls"][t])
     cars, controls = np.concatenate(cars), np.concatenate(controls)
+    configs = np.tile(run["configs"], (ticks, 1))
     rng = np.random.default_rng(seed)
     pick = np.sort(rng.choice(len(cars), keep, replace=False))
     c, k, q = perturb_force_inputs(cars[pick], controls[pick], seed)
+    cfg = configs[pick]
     params = force_param_sets()
     out_cars, out_queues = [], []
-    for phase, queued, ps in FORCE_CASES:
-        oc, oq = R.car_force_phase(phase, c, k, q if queued else None, params[ps])
+    for phase, queued, ps, flags in FORCE_CASES:
+        oc, oq = R.car_force_phase(phase, c, k, q if queued else None, params[ps], cfg, flags)
         out_cars.append(oc)
         out_queues.append(oq if queued else np.zeros_like(q))
     meta = {"workload": "car force-phase inputs: pre-ticked car-arena states with seeded boost and jump timers, "
                         "flags, buttons and queued forces", "arenas": arenas, "cars_per_env": cars_per_env,
             "ticks": ticks, "seed": seed, "rows": int(keep),
-            "generator": "oracle/_ref reference CPU build (UpdateJumpState, ApplyBoost)"}
+            "generator": "oracle/_ref reference CPU build (UpdateJumpState, ApplyBoost, ApplyAirControl)"}
     path = os.path.join(HERE, name + ".npz")
     # Outputs as the XOR with the inputs (mostly zero bytes, so the fixture stays small).
-    np.savez_compressed(path, cars=c, controls=k, queues=q, params=params, cases=np.array(FORCE_CASES, np.int32),
+    np.savez_compressed(path, cars=c, controls=k, configs=cfg, queues=q, params=params,
+                        cases=np.array(FORCE_CASES, np.int32),
                         out_cars_xor=np.stack(out_cars) ^ c[None], out_queues=np.stack(out_queues),
                         meta=np.frombuffer(json.dumps(meta).encode(), np.uint8))
     print(f"{name}: {keep} rows x {len(FORCE_CASES)} cases -> {os.path.getsize(path)} bytes")
