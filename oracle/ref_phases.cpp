@@ -35,7 +35,7 @@ void rgref_default_wheel_state(void* out) {
 // boost_spawn_amount, unlimited_flips, unlimited_double_jumps, jump_accel, boost_max, boost_used_per_second,
 // boost_min_time, boost_accel_ground, recharge_boost_per_second, recharge_boost_delay, recharge_boost_enabled,
 // car_air_throttle_accel, flip_pitchlock_extra_time, car_air_control_pitch, _yaw, _roll, car_air_damp_pitch, _yaw,
-// _roll.  Returns the count written.
+// _roll, flip_torque_x, flip_torque_y.  Returns the count written.
 int rgref_car_params(float* out) {
     const SimConfig cfg{};
     const MutatorConfig& m = cfg.mutator;
@@ -51,7 +51,7 @@ int rgref_car_params(float* out) {
                        m.recharge_boost_delay, static_cast<float>(m.recharge_boost_enabled),
                        m.car_air_throttle_accel, m.flip_pitchlock_extra_time, m.car_air_control_pitch,
                        m.car_air_control_yaw, m.car_air_control_roll, m.car_air_damp_pitch, m.car_air_damp_yaw,
-                       m.car_air_damp_roll};
+                       m.car_air_damp_roll, m.flip_torque_x, m.flip_torque_y};
     std::memcpy(out, v, sizeof(v));
     return static_cast<int>(sizeof(v) / sizeof(float));
 }
@@ -81,8 +81,9 @@ static void Put(float* f, const Vec3& v) {
     f[2] = v.z;
 }
 
-// UpdateJumpState (phase 0, raycast_wheels_and_contact_prep.hpp:3629-3652), ApplyBoost (phase 1, :3252-3300) or
-// ApplyAirControl (phase 2, :3048-3241) on n independent cars, as the controls step calls them (env_step.hpp:
+// UpdateJumpState (phase 0, raycast_wheels_and_contact_prep.hpp:3629-3652), ApplyBoost (phase 1, :3252-3300),
+// ApplyAirControl (phase 2, :3048-3241) or ApplyFlipTorque (phase 3, :3432-3484; flags bit 0 passes the control's
+// pitch as the raw pitch, else 2) on n independent cars, as the controls step calls them (env_step.hpp:
 // 3189-3200, :3309-3331, :3418): with the car's frame (MakeCarFrame), and with the wheel runtime state (so the frame
 // queues the forces) when queues is not null, else without it (the force changes the velocity).  Air control reads
 // configs (one CarConfig per car) and flags (bit 0: update air control, bit 1: air throttle), with the tick's
@@ -90,10 +91,10 @@ static void Put(float* f, const Vec3& v) {
 // jump_min_time, jump_reset_time_pad, jump_max_time, jump_accel, boost_max, boost_used_per_second, boost_min_time,
 // boost_accel_ground, recharge_boost_per_second, recharge_boost_delay, recharge_boost_enabled,
 // car_air_throttle_accel, flip_torque_time, flip_pitchlock_extra_time, car_air_control_pitch, _yaw, _roll,
-// car_air_damp_pitch, _yaw, _roll.
+// car_air_damp_pitch, _yaw, _roll, flip_torque_x, flip_torque_y.
 int rgref_car_force_phase(long n, void* cars, const void* controls, const void* configs, void* queues, int phase,
                           int flags, const float* params) {
-    if (phase < 0 || phase > 2 || (phase == 2 && configs == nullptr)) return 1;
+    if (phase < 0 || phase > 3 || (phase >= 2 && configs == nullptr)) return 1;
     MutatorConfig m = SimConfig{}.mutator;
     const float dt = params[0];
     m.car_mass = params[1];
@@ -117,12 +118,14 @@ int rgref_car_force_phase(long n, void* cars, const void* controls, const void* 
     m.car_air_damp_pitch = params[19];
     m.car_air_damp_yaw = params[20];
     m.car_air_damp_roll = params[21];
+    m.flip_torque_x = params[22];
+    m.flip_torque_y = params[23];
     const CarConfig* cfg = static_cast<const CarConfig*>(configs);
     CarState* c = static_cast<CarState*>(cars);
     const CarControls* ctl = static_cast<const CarControls*>(controls);
     ForceQueueRecord* q = static_cast<ForceQueueRecord*>(queues);
     for (long k = 0; k < n; ++k) {
-        detail::CarFrame frame = phase == 2 ? detail::MakeCarFrame(c[k], cfg[k], m.car_mass) : detail::CarFrame{};
+        detail::CarFrame frame = phase >= 2 ? detail::MakeCarFrame(c[k], cfg[k], m.car_mass) : detail::CarFrame{};
         detail::WheelRuntimeState ws{};
         if (q) {
             const ForceQueueRecord& r = q[k];
@@ -144,9 +147,17 @@ int rgref_car_force_phase(long n, void* cars, const void* controls, const void* 
             detail::UpdateJumpState(c[k], ctl[k], m, dt, wsp, &frame);
         } else if (phase == 1) {
             detail::ApplyBoost(c[k], ctl[k], m, dt, wsp, &frame);
-        } else {
+        } else if (phase == 2) {
             detail::ApplyAirControl(c[k], ctl[k], cfg[k], m, dt, (flags & 1) != 0, wsp, (flags & 2) != 0,
                                     c[k].cal_pre_pitchlock_restart_time, &c[k].cal_tick_start_ang_vel, &frame);
+        } else {
+            // The step defers the queued flip torque past air control (env_step.hpp:3311-3320); queued here at once.
+            Vec3 deferred(0.0f, 0.0f, 0.0f);
+            detail::ApplyFlipTorque(c[k], ctl[k], cfg[k], m, dt, wsp, (flags & 1) != 0 ? ctl[k].pitch : 2.0f, &frame,
+                                    &deferred);
+            if (deferred.x != 0.0f || deferred.y != 0.0f || deferred.z != 0.0f) {
+                detail::QueueOfficialTorqueBt(wsp, deferred, &frame);
+            }
         }
         if (q) {
             ForceQueueRecord& r = q[k];

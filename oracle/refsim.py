@@ -290,7 +290,7 @@ FORCE_PARAM_NAMES = ("tick_dt", "car_mass", "jump_min_time", "jump_reset_time_pa
                      "recharge_boost_per_second", "recharge_boost_delay", "recharge_boost_enabled",
                      "car_air_throttle_accel", "flip_torque_time", "flip_pitchlock_extra_time",
                      "car_air_control_pitch", "car_air_control_yaw", "car_air_control_roll", "car_air_damp_pitch",
-                     "car_air_damp_yaw", "car_air_damp_roll")
+                     "car_air_damp_yaw", "car_air_damp_roll", "flip_torque_x", "flip_torque_y")
 
 
 # The order of car_params() (ref_phases.cpp rgref_car_params).
@@ -304,7 +304,7 @@ CAR_PARAM_NAMES = ("tick_dt", "boost_pad_frame_frac", "car_mass", "car_max_speed
                    "boost_min_time", "boost_accel_ground", "recharge_boost_per_second", "recharge_boost_delay",
                    "recharge_boost_enabled", "car_air_throttle_accel", "flip_pitchlock_extra_time",
                    "car_air_control_pitch", "car_air_control_yaw", "car_air_control_roll", "car_air_damp_pitch",
-                   "car_air_damp_yaw", "car_air_damp_roll")
+                   "car_air_damp_yaw", "car_air_damp_roll", "flip_torque_x", "flip_torque_y")
 
 
 def force_params(**override) -> np.ndarray:

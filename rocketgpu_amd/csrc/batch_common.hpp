@@ -62,13 +62,13 @@ inline core::CarSpawnRules SpawnRules(const rgw::ArenaRulesSpec& rules) {
 }
 
 // The number of floats in a force-phase parameter list.
-constexpr int kForcePhaseParamCount = 22;
+constexpr int kForcePhaseParamCount = 24;
 
 // Car parameters from a force-phase list, in this order: tick_dt, car_mass, jump_min_time, jump_reset_time_pad,
 // jump_max_time, jump_accel, boost_max, boost_used_per_second, boost_min_time, boost_accel_ground,
 // recharge_boost_per_second, recharge_boost_delay, recharge_boost_enabled (nonzero: on), car_air_throttle_accel,
-// flip_torque_time, flip_pitchlock_extra_time, car_air_control_pitch, _yaw, _roll, car_air_damp_pitch, _yaw, _roll.
-// The other fields keep their defaults.
+// flip_torque_time, flip_pitchlock_extra_time, car_air_control_pitch, _yaw, _roll, car_air_damp_pitch, _yaw, _roll,
+// flip_torque_x, flip_torque_y.  The other fields keep their defaults.
 inline core::CarParams ForcePhaseParams(const float* p) {
     core::CarParams m{};
     m.tick_dt = p[0];
@@ -93,6 +93,8 @@ inline core::CarParams ForcePhaseParams(const float* p) {
     m.car_air_damp_pitch = p[19];
     m.car_air_damp_yaw = p[20];
     m.car_air_damp_roll = p[21];
+    m.flip_torque_x = p[22];
+    m.flip_torque_y = p[23];
     return m;
 }
 

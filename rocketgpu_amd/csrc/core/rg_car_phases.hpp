@@ -15,9 +15,9 @@
 // Post-tick (CarPostTick): cooldowns, last controls, pending bumps and velocity caches, the tick counter, and the
 // fields the game reports.
 //
-// Force phases of the controls step (CarUpdateJumpState, CarApplyBoost, CarApplyAirControl): the jump hold, boost
-// and air control, queueing their forces and torques for the solver.  The wheel rays, suspension, friction, flip
-// torque and auto-roll of that step are not here.
+// Force phases of the controls step (CarUpdateJumpState, CarApplyBoost, CarApplyAirControl, CarApplyFlipTorque):
+// the jump hold, boost, air control and flip torque, queueing their forces and torques for the solver.  The wheel
+// rays, suspension, friction and auto-roll of that step are not here.
 #pragma once
 
 #include "rg_env_commands.hpp"
@@ -68,6 +68,9 @@ struct CarParams {
     float car_air_damp_pitch = 30.0f;
     float car_air_damp_yaw = 20.0f;
     float car_air_damp_roll = 50.0f;
+    // Read by the flip torque.
+    float flip_torque_x = 260.0f;
+    float flip_torque_y = 224.0f;
 };
 
 // ------------------------------------------------------------------------------------------------------------
@@ -1414,6 +1417,83 @@ RG_HD void CarApplyAirControl(CarState& car, const CarControls& c, const CarConf
         } else {
             ApplyCarCentralForce(car, m.car_mass, force_bt * 50.0f, dt);
         }
+    }
+}
+
+// Hamilton product in the reference's term order (math.hpp:763-769).
+RG_HD Quat QuatProduct(const Quat& a, const Quat& b) {
+    return Quat(a.w * b.w - a.x * b.x - a.y * b.y - a.z * b.z, a.w * b.x + a.x * b.w + a.y * b.z - a.z * b.y,
+                a.w * b.y - a.x * b.z + a.y * b.w + a.z * b.x, a.w * b.z + a.x * b.y - a.y * b.x + a.z * b.w);
+}
+
+// v rotated as the vector part of q (0, v) q^-1, with q^-1 the conjugate over |q|^2 (identity below 1e-12)
+// (math.hpp:775-786, :866-870).
+RG_HD Vec3 QuatSandwichRotate(const Quat& q, const Vec3& v) {
+    const float len_sq = q.w * q.w + q.x * q.x + q.y * q.y + q.z * q.z;
+    Quat inv(1.0f, 0.0f, 0.0f, 0.0f);
+    if (!(len_sq <= 1e-12f)) {
+        const float s = 1.0f / len_sq;
+        inv = Quat(q.w * s, -q.x * s, -q.y * s, -q.z * s);
+    }
+    const Quat r = QuatProduct(QuatProduct(q, Quat(0.0f, v.x, v.y, v.z)), inv);
+    return Vec3(r.x, r.y, r.z);
+}
+
+// A car-local vector in the world: through the basis columns when they are authoritative, else the quaternion
+// (math.hpp:984-986, :1402-1405).
+RG_HD Vec3 RotateCarVector(const CarState& car, const Vec3& v) {
+    if (!car.basis_authoritative) return QuatSandwichRotate(car.rot, v);
+    return (car.basis_forward * v.x + car.basis_right * v.y) + car.basis_up * v.z;
+}
+
+RG_HD float SignOf(float v) { return static_cast<float>((v > 0.0f) - (v < 0.0f)); }
+
+// The pitch that cancels a dodge: the raw pitch when the caller has one (at most 1), else the control's, once the
+// flip has run 0.041 s, quantized toward the game's axis steps (raycast_wheels_and_contact_prep.hpp:3032-3046).
+RG_HD float DodgeCancelPitch(const CarState& car, const CarControls& c, float raw_pitch) {
+    const float pitch = raw_pitch <= 1.0f ? raw_pitch : c.pitch;
+    const float p = car.internal_flip_time >= 0.041f ? pitch : 0.0f;
+    return -QuantizeAxisInput(-p);
+}
+
+// Flip torque (ApplyFlipTorque, raycast_wheels_and_contact_prep.hpp:3432-3484): a flip ends when its torque time is
+// over; a running flip turns the car by its relative torque (rounded to whole units, pitch reduced by a cancelling
+// pitch input) as an angular acceleration, and, queued, as a torque about the game's quaternion axes through the
+// inverse of the frame's Bullet rows when a component reaches 1e-4.  The step defers that torque until after air
+// control; this phase queues it at once.  raw_pitch: the control's pitch before quantization, or 2 for none.
+RG_HD void CarApplyFlipTorque(CarState& car, const CarControls& c, const CarConfig& cfg, const CarParams& m,
+                              const CarFrame& frame, float raw_pitch, CarForceQueue* q) {
+    if (car.is_flipping) car.is_flipping = (car.has_flipped && car.internal_flip_time < m.flip_torque_time) ? 1 : 0;
+    if (!car.is_flipping) return;
+    const Vec3 rel = car.flip_rel_torque;
+    float torque_y = rintf(rel.y * m.flip_torque_y);
+    const float cancel = DodgeCancelPitch(car, c, raw_pitch);
+    if (rel.y != 0.0f && cancel != 0.0f && SignOf(rel.y) == SignOf(cancel)) {
+        torque_y *= (1.0f - Min(Abs(cancel), 1.0f));
+    }
+    const Vec3 local(rintf(rel.x * m.flip_torque_x), torque_y, 0.0f);
+    const Vec3 world = RotateCarVector(car, local);
+    if (!q) {
+        car.ang_vel += world * m.tick_dt;
+        return;
+    }
+    if (!car.basis_authoritative) {
+        Vec3 i0, i1, i2, w0, w1, w2;
+        InvInertiaWorldRowsUu(car, cfg, m.car_mass, i0, i1, i2);
+        InverseRows(i0, i1, i2, w0, w1, w2);
+        q->ang_accel = q->ang_accel + MulRows(w0, w1, w2, world);
+    }
+    Vec3 f, r, u;
+    UeQuatAxes(car.rot, f, r, u);
+    const Vec3 tw(((local.y * r.x + local.x * f.x) + local.z * u.x) + 0.0f * 0.0f,
+                  ((local.y * r.y + local.x * f.y) + local.z * u.y) + 0.0f * 0.0f,
+                  ((local.y * r.z + local.x * f.z) + local.z * u.z) + 0.0f * 0.0f);
+    constexpr float kGate = 1.0e-4f;
+    if (Abs(tw.x) >= kGate || Abs(tw.y) >= kGate || Abs(tw.z) >= kGate) {
+        Vec3 w0, w1, w2;
+        InverseRows(frame.inv_row0, frame.inv_row1, frame.inv_row2, w0, w1, w2);
+        const Vec3 t = MulRows(w0, w1, w2, tw);
+        if (t.x != 0.0f || t.y != 0.0f || t.z != 0.0f) QueueCarTorque(*q, t);
     }
 }
 

@@ -304,13 +304,14 @@ void rgt_car_posttick(long n, void* cars, const void* controls, const void* whee
     for (long k = 0; k < n; ++k) core::CarPostTick(c[k], ctl[k], m, w ? &w[k] : nullptr, report != 0);
 }
 
-// One force phase (0: jump hold, 1: boost, 2: air control) on n independent cars, with the parameters of a
-// force-phase list; configs: one CarConfig per car (read by air control only, may be null otherwise); flags (air
-// control): bit 0 updates air control, bit 1 applies the air throttle; queues: one force queue per car, or null for
-// the form that changes the velocity at once.  Returns 0, or 1 for an unknown phase or missing configs.
+// One force phase (0: jump hold, 1: boost, 2: air control, 3: flip torque) on n independent cars, with the
+// parameters of a force-phase list; configs: one CarConfig per car (read by phases 2 and 3, may be null otherwise);
+// flags: for air control bit 0 updates air control and bit 1 applies the air throttle, for the flip torque bit 0
+// passes the control's pitch as the raw pitch (else none); queues: one force queue per car, or null for the form
+// that changes the velocity at once.  Returns 0, or 1 for an unknown phase or missing configs.
 int rgt_car_force_phase(long n, void* cars, const void* controls, const void* configs, void* queues, int phase,
                         int flags, const float* params) {
-    if (phase < 0 || phase > 2 || (phase == 2 && configs == nullptr)) return 1;
+    if (phase < 0 || phase > 3 || (phase >= 2 && configs == nullptr)) return 1;
     const core::CarParams m = ForcePhaseParams(params);
     CarState* c = static_cast<CarState*>(cars);
     const CarControls* ctl = static_cast<const CarControls*>(controls);
@@ -324,7 +325,11 @@ int rgt_car_force_phase(long n, void* cars, const void* controls, const void* co
             core::CarApplyBoost(c[k], ctl[k], m, qk);
         } else {
             const core::CarFrame frame = core::MakeCarFrame(c[k], cfg[k], m.car_mass);
-            core::CarApplyAirControl(c[k], ctl[k], cfg[k], m, frame, (flags & 1) != 0, (flags & 2) != 0, qk);
+            if (phase == 2) {
+                core::CarApplyAirControl(c[k], ctl[k], cfg[k], m, frame, (flags & 1) != 0, (flags & 2) != 0, qk);
+            } else {
+                core::CarApplyFlipTorque(c[k], ctl[k], cfg[k], m, frame, (flags & 1) != 0 ? ctl[k].pitch : 2.0f, qk);
+            }
         }
     }
     return 0;
@@ -349,7 +354,8 @@ int rgt_car_params(float* out) {
                        m.recharge_boost_per_second, m.recharge_boost_delay,
                        static_cast<float>(m.recharge_boost_enabled), m.car_air_throttle_accel,
                        m.flip_pitchlock_extra_time, m.car_air_control_pitch, m.car_air_control_yaw,
-                       m.car_air_control_roll, m.car_air_damp_pitch, m.car_air_damp_yaw, m.car_air_damp_roll};
+                       m.car_air_control_roll, m.car_air_damp_pitch, m.car_air_damp_yaw, m.car_air_damp_roll,
+                       m.flip_torque_x, m.flip_torque_y};
     std::memcpy(out, v, sizeof(v));
     return static_cast<int>(sizeof(v) / sizeof(float));
 }

@@ -49,7 +49,7 @@ __global__ void __launch_bounds__(64) CarPostTickKernel(CarState* cars, const Ca
     cars[k] = car;
 }
 
-// Car k's force phase (0: jump hold, 1: boost, 2: air control); queues may be null.
+// Car k's force phase (0: jump hold, 1: boost, 2: air control, 3: flip torque); queues may be null.
 __global__ void __launch_bounds__(64) CarForcePhaseKernel(CarState* cars, const CarControls* controls,
                                                           const CarConfig* configs, core::CarForceQueue* queues,
                                                           int32_t n, int32_t phase, int32_t flags, core::CarParams m) {
@@ -65,8 +65,13 @@ __global__ void __launch_bounds__(64) CarForcePhaseKernel(CarState* cars, const 
         core::CarApplyBoost(car, controls[k], m, qp);
     } else {
         const CarConfig cfg = configs[k];
+        const CarControls ctl = controls[k];
         const core::CarFrame frame = core::MakeCarFrame(car, cfg, m.car_mass);
-        core::CarApplyAirControl(car, controls[k], cfg, m, frame, (flags & 1) != 0, (flags & 2) != 0, qp);
+        if (phase == 2) {
+            core::CarApplyAirControl(car, ctl, cfg, m, frame, (flags & 1) != 0, (flags & 2) != 0, qp);
+        } else {
+            core::CarApplyFlipTorque(car, ctl, cfg, m, frame, (flags & 1) != 0 ? ctl.pitch : 2.0f, qp);
+        }
     }
     cars[k] = car;
     if (queues) queues[k] = q;
@@ -76,13 +81,14 @@ __global__ void __launch_bounds__(64) CarForcePhaseKernel(CarState* cars, const 
 
 extern "C" {
 
-// One force phase (0: jump hold, 1: boost, 2: air control) on n cars in place (device pointers; configs: one
-// CarConfig per car, read by air control only; queues: one force queue per car, or null); flags (air control): bit 0
-// updates air control, bit 1 applies the air throttle; params: a host list of the force-phase parameters.  Returns a
+// One force phase (0: jump hold, 1: boost, 2: air control, 3: flip torque) on n cars in place (device pointers;
+// configs: one CarConfig per car, read by phases 2 and 3; queues: one force queue per car, or null); flags: for air
+// control bit 0 updates air control and bit 1 applies the air throttle, for the flip torque bit 0 passes the
+// control's pitch as the raw pitch; params: a host list of the force-phase parameters.  Returns a
 // hipError_t, or hipErrorInvalidValue for an unknown phase, missing configs or a null parameter list.
 int rgp_car_force_phase(void* cars, const void* controls, const void* configs, void* queues, int32_t n,
                         int32_t phase, int32_t flags, const float* params, void* stream) {
-    if (phase < 0 || phase > 2 || (phase == 2 && configs == nullptr) || params == nullptr) {
+    if (phase < 0 || phase > 3 || (phase >= 2 && configs == nullptr) || params == nullptr) {
         return static_cast<int>(hipErrorInvalidValue);
     }
     if (n <= 0) return 0;

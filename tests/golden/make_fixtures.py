@@ -290,10 +290,10 @@ CAR_FORCE_FIXTURES = {
 }
 
 # (phase, queued, parameter set, flags) of each output: phase 0 the jump hold, 1 boost, 2 air control (flags bit 0
-# updates air control, bit 1 applies the air throttle); parameter set 0 the defaults, 1 the defaults with boost
-# recharge on.
+# updates air control, bit 1 applies the air throttle), 3 the flip torque (flags bit 0: the control's pitch is the
+# raw pitch); parameter set 0 the defaults, 1 the defaults with boost recharge on.
 FORCE_CASES = ((0, 1, 0, 3), (0, 0, 0, 3), (1, 1, 0, 3), (1, 0, 0, 3), (1, 1, 1, 3), (1, 0, 1, 3),
-               (2, 1, 0, 3), (2, 0, 0, 3), (2, 1, 0, 1), (2, 1, 0, 2))
+               (2, 1, 0, 3), (2, 0, 0, 3), (2, 1, 0, 1), (2, 1, 0, 2), (3, 1, 0, 1), (3, 0, 0, 1), (3, 1, 0, 0))
 
 
 def force_param_sets():
@@ -325,10 +325,13 @@ def perturb_force_inputs(cars, controls, seed):
         put(i, "time_since_boosted", rng.choice([0.0, 0.1, 0.2416, 0.2417, 0.25, 0.3, 1.0]))
         put(i, "internal_jump_time", rng.choice([0.0, 0.01, 0.0249, 0.03, 0.0499, 0.05, 0.06, 0.19, 0.2, 0.25, 0.5]))
         for name, p in (("is_boosting", 0.5), ("is_on_ground", 0.5), ("is_jumping", 0.4), ("has_jumped", 0.5),
-                        ("jump_hold_broken", 0.2), ("is_demoed", 0.08), ("is_flipping", 0.15), ("has_flipped", 0.4),
+                        ("jump_hold_broken", 0.2), ("is_demoed", 0.08), ("is_flipping", 0.3), ("has_flipped", 0.5),
                         ("basis_authoritative", 0.3)):
             c[i, lay[name]] = 1 if rng.random() < p else 0
-        put(i, "internal_flip_time", rng.choice([0.0, 0.5, 0.94, 0.95, 0.96, 1.5]))
+        put(i, "internal_flip_time", rng.choice([0.0, 0.02, 0.041, 0.3, 0.5, 0.64, 0.65, 0.94, 0.95, 0.96, 1.5]))
+        rel = np.zeros(3, np.float32)
+        rel[:2] = ((0, 0), (1, 0), (-1, 0), (0, 1), (0, -1), (0.6, -0.8), (-0.3, 0.95))[rng.integers(7)]
+        c[i, lay["flip_rel_torque"]:lay["flip_rel_torque"] + 12] = rel.view(np.uint8)
         put(i, "cal_pre_pitchlock_restart_time", rng.choice([-1.0, 0.0, 0.1, 0.29, 0.3, 0.5]))
         w = rng.uniform(-5, 5, 3).astype(np.float32)
         c[i, lay["cal_tick_start_ang_vel"]:lay["cal_tick_start_ang_vel"] + 12] = w.view(np.uint8)

@@ -100,6 +100,14 @@ def test_fixture_covers_the_branches():
     assert (c[:, lay["restored_inv_inertia_valid"]] == 1).sum() >= 5
     assert ((a1["n_torque"] - a0["n_torque"]) == 2).sum() >= 50
     assert ((a1["n_torque"] - a0["n_torque"]) == 1).sum() >= 1
+    # Flip torque (queued): flips that run on and queue a torque, flips that end, cancelling pitch.
+    fl = out[cases.index((3, 1, 0, 1))]
+    fq = queue_parts(f["out_queues"][cases.index((3, 1, 0, 1))])
+    on = fl[:, lay["is_flipping"]] == 1
+    assert on.sum() >= 10 and ((c[:, lay["is_flipping"]] == 1) & ~on).sum() >= 5
+    assert (fq["n_torque"][on] == a0["n_torque"][on] + 1).sum() >= 8
+    late = on & (f32(c, lay["internal_flip_time"])[:, 0] >= 0.041) & (pitch != 0)
+    assert late.sum() >= 3
 
 
 def test_oracle_reproduces_fixture():
@@ -119,7 +127,7 @@ def test_oracle_default_parameters_match_the_twin():
     assert np.array_equal(R.force_params(), f["params"][0])
 
 
-@pytest.mark.parametrize("case", range(10))
+@pytest.mark.parametrize("case", range(13))
 def test_twin_matches_fixture(case):
     T = twin()
     f, out = fixture()
@@ -234,7 +242,7 @@ def test_twin_matches_oracle_on_fresh_inputs():
                                run["envs"][t], run["teams"])
         c, k, q = perturb_force_inputs(mid, run["controls"][t], 500 + t)
         for ps in sets:
-            for phase, flags in ((0, 3), (1, 3), (2, 3), (2, 1), (2, 2), (2, 0)):
+            for phase, flags in ((0, 3), (1, 3), (2, 3), (2, 1), (2, 2), (2, 0), (3, 1), (3, 0)):
                 for queues in (q, None):
                     wc, wq = R.car_force_phase(phase, c, k, queues, ps, run["configs"], flags)
                     gc, gq = twin_phase(T, phase, c, k, queues, ps, run["configs"], flags)
@@ -279,13 +287,49 @@ def test_air_throttle_and_direct_air_control():
     assert (f32(d, lay["ang_vel"], 3) != f32(c, lay["ang_vel"], 3)).any(1).sum() >= 100
 
 
+def test_flip_torque_turns_the_car():
+    """A flip that runs on: without a queue the angular velocity gains R(q) (rint(x * flip_torque_x),
+    rint(y * flip_torque_y), 0) dt for the relative torque (x, y) (rows with no cancelling pitch: under 0.041 s of
+    flip, zero pitch or zero y), with R the car's basis columns when they are authoritative;
+    a flip past its torque time ends and leaves the car alone."""
+    f, out = fixture()
+    lay = layout()
+    cases = [tuple(x) for x in f["cases"]]
+    p = f["params"][0]
+    c = f["cars"]
+    d = out[cases.index((3, 0, 0, 1))]
+    ift = f32(c, lay["internal_flip_time"])[:, 0]
+    running = (c[:, lay["is_flipping"]] == 1) & (c[:, lay["has_flipped"]] == 1) & (ift < p[14])
+    assert (d[:, lay["is_flipping"]] == running).all()
+    ended = (c[:, lay["is_flipping"]] == 1) & ~running
+    assert ended.sum() >= 5 and np.array_equal(d[ended, lay["ang_vel"]:lay["ang_vel"] + 12],
+                                                c[ended, lay["ang_vel"]:lay["ang_vel"] + 12])
+    rel_all = f32(c, lay["flip_rel_torque"], 3)
+    pitch = f["controls"][:, 8:12].view(np.float32)[:, 0]
+    no_cancel = (ift < 0.041) | (pitch == 0) | (rel_all[:, 1] == 0)
+    rows = running & no_cancel
+    assert rows.sum() >= 5 and (rows & (c[:, lay["basis_authoritative"]] == 0)).sum() >= 3
+    rel = rel_all[rows].astype(np.float32)
+    local = np.stack([np.rint(rel[:, 0] * np.float32(p[22])), np.rint(rel[:, 1] * np.float32(p[23])),
+                      np.zeros(rows.sum())], 1)
+    # Quaternion cars turn by the rotation matrix, cars with an authoritative basis by their basis columns.
+    m = rotation_matrix(f32(c, lay["rot"], 4)[rows])
+    basis = np.stack([f32(c, lay[k], 3)[rows] for k in ("basis_forward", "basis_right", "basis_up")], 2)
+    auth = c[rows, lay["basis_authoritative"]] == 1
+    m[auth] = basis[auth]
+    want = np.einsum("nij,nj->ni", m, local) * p[0]
+    got = f32(d, lay["ang_vel"], 3)[rows].astype(np.float64) - f32(c, lay["ang_vel"], 3)[rows]
+    assert np.allclose(got, want, rtol=0, atol=1e-4)
+
+
 def test_twin_rejects_an_unknown_phase():
     T = twin()
     f = load_fixture(FIX)
     c = f["cars"][:1].copy()
     p = f["params"][0].copy()
     k = f["controls"][:1].copy()
-    assert T.rgt_car_force_phase(1, c.ctypes.data, k.ctypes.data, None, None, 3, 3, p.ctypes.data) == 1
+    assert T.rgt_car_force_phase(1, c.ctypes.data, k.ctypes.data, None, None, 4, 3, p.ctypes.data) == 1
+    assert T.rgt_car_force_phase(1, c.ctypes.data, k.ctypes.data, None, None, 3, 1, p.ctypes.data) == 1
     assert T.rgt_car_force_phase(1, c.ctypes.data, k.ctypes.data, None, None, 2, 3, p.ctypes.data) == 1
     assert np.array_equal(c, f["cars"][:1])
 
@@ -300,7 +344,7 @@ def gpu_lib():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("case", range(10))
+@pytest.mark.parametrize("case", range(13))
 def test_gpu_matches_fixture(case):
     import torch
     P = gpu_lib()
@@ -326,7 +370,8 @@ def test_gpu_matches_fixture(case):
 @pytest.mark.gpu
 def test_gpu_rejects_an_unknown_phase():
     P = gpu_lib()
-    p = np.zeros(22, np.float32)
-    assert P.rgp_car_force_phase(None, None, None, None, 1, 3, 3, p.ctypes.data, None) != 0
+    p = np.zeros(24, np.float32)
+    assert P.rgp_car_force_phase(None, None, None, None, 1, 4, 3, p.ctypes.data, None) != 0
     assert P.rgp_car_force_phase(None, None, None, None, 1, 2, 3, p.ctypes.data, None) != 0
+    assert P.rgp_car_force_phase(None, None, None, None, 1, 3, 1, p.ctypes.data, None) != 0
     assert P.rgp_car_force_phase(None, None, None, None, 1, 0, 3, None, None) != 0
