@@ -100,4 +100,34 @@ int rgp_car_force_phase(void* cars, const void* controls, const void* configs, v
     return static_cast<int>(hipGetLastError());
 }
 
+// n cars in place (device pointers; wheels may be null, and are only read).  Returns a hipError_t.
+int rgp_car_posttick(void* cars, const void* controls, const void* wheels, int32_t n, int32_t report, void* stream) {
+    if (n <= 0) return 0;
+    const core::CarParams m{};
+    hipLaunchKernelGGL(CarPostTickKernel, dim3((n + 63) / 64), dim3(64), 0, static_cast<hipStream_t>(stream),
+                       static_cast<CarState*>(cars), static_cast<const CarControls*>(controls),
+                       static_cast<const WheelRuntimeState*>(wheels), n, m, report);
+    return static_cast<int>(hipGetLastError());
+}
+
+// n_env x max_cars car slots in place on PA(scale, fillet) (device pointers; wheels may be null), the default car
+// parameters; unsupported: one device byte per slot.  Returns a hipError_t.
+int rgp_car_pretick(void* cars, const void* controls, const void* configs, void* wheels, const void* envs,
+                    const void* teams, int32_t n_env, int32_t max_cars, double scale, double fillet, uint64_t seed,
+                    void* unsupported, void* stream) {
+    const int32_t n = n_env * max_cars;
+    if (n <= 0) return 0;
+    const core::CarParams m{};
+    rgw::ArenaParams ap;
+    ap.scale = scale;
+    ap.fillet_radius = fillet;
+    const core::CarSpawnRules rules = SpawnRules(rgw::ProceduralArenaRules(ap));
+    hipLaunchKernelGGL(CarPreTickKernel, dim3((n + 63) / 64), dim3(64), 0, static_cast<hipStream_t>(stream),
+                       static_cast<CarState*>(cars), static_cast<const CarControls*>(controls),
+                       static_cast<const CarConfig*>(configs), static_cast<WheelRuntimeState*>(wheels),
+                       static_cast<const EnvState*>(envs), static_cast<const uint8_t*>(teams), n_env, max_cars, seed,
+                       m, rules, static_cast<uint8_t*>(unsupported));
+    return static_cast<int>(hipGetLastError());
+}
+
 }  // extern "C"

@@ -334,6 +334,15 @@ def test_twin_rejects_an_unknown_phase():
     assert np.array_equal(c, f["cars"][:1])
 
 
+def test_phase_library_exports_every_entry_point():
+    """The test-only phase library loads here and exports each phase's entry point (nothing is called)."""
+    if not os.path.exists(PHASES_PATH):
+        pytest.fail("librg_phases.so is not built")
+    P = ctypes.CDLL(PHASES_PATH)
+    for name in ("rgp_car_pretick", "rgp_car_posttick", "rgp_car_force_phase"):
+        assert hasattr(P, name), name
+
+
 def gpu_lib():
     if not os.path.exists(PHASES_PATH):
         pytest.fail("librg_phases.so is not built")
