@@ -115,6 +115,8 @@ struct ArenaInternal {
     // Windows the last tick's contact walk loaded in a group kernel (saturating; 0 after a one-lane kernel).  The
     // launch uses it to give the arenas with the longest walks more lanes; it does not affect the physics.
     uint16_t walk_rounds;
+    // Car arenas: the physics the tick met that this build does not step (kUnsupported* bits), sticky until reset.
+    uint32_t unsupported;
     BallMeshManifold ball_mesh[kMaxArenaMeshes];
 };
 

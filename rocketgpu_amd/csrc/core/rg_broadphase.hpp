@@ -47,6 +47,10 @@ struct SweepState {
     uint8_t min_edge[kSweepMaxHandles + 1][3];
     uint8_t max_edge[kSweepMaxHandles + 1][3];
     int16_t static_id[kSweepMaxHandles + 1];  // >= 0 static mesh index, < 0: -1 - body
+    // Each body's swept box at its last update (bt: min xyz, max xyz), valid while its bit is set; wheel rays are
+    // cast only against the bodies whose box they reach.
+    uint16_t body_aabb_valid;
+    float body_aabb[kMaxBodies][6];
 };
 
 struct BroadphaseState {
@@ -268,6 +272,7 @@ struct Sweep {
         h.free_list[h.free_count++] = static_cast<uint8_t>(handle);
         h.handle_count = static_cast<uint8_t>(h.handle_count - 1);
         h.body_handle[body] = 0;
+        h.body_aabb_valid = static_cast<uint16_t>(h.body_aabb_valid & ~(1u << body));
         for (int i = 0; i < h.obj_count; ++i) {
             if (h.obj[i] == body) {
                 h.obj[i] = h.obj[--h.obj_count];
@@ -285,6 +290,7 @@ RG_HD void SweepInitStatics(BroadphaseState& bp, const ArenaMeshView& mesh) {
     h.handle_count = 0;
     h.obj_count = 0;
     for (int b = 0; b < kMaxBodies; ++b) h.body_handle[b] = 0;
+    h.body_aabb_valid = 0;
     for (int axis = 0; axis < 3; ++axis) {
         h.edge_pos[axis][0] = 0;
         h.edge_handle[axis][0] = 0;

@@ -50,6 +50,11 @@ struct CarParams {
     float boost_spawn_amount = 33.3f;
     uint8_t unlimited_flips = 0;
     uint8_t unlimited_double_jumps = 0;
+    // Read by the vehicle update and the world step.
+    float gravity_z = -650.0f;
+    float ball_mass = 30.0f;
+    float boost_pad_cooldown_small = 4.0f;
+    float boost_pad_cooldown_big = 10.0f;
     // Read by the force phases (jump hold and boost).
     float jump_accel = 4375.0f / 3.0f;
     float boost_max = 100.0f;
@@ -1459,10 +1464,10 @@ RG_HD float DodgeCancelPitch(const CarState& car, const CarControls& c, float ra
 // Flip torque (ApplyFlipTorque, raycast_wheels_and_contact_prep.hpp:3432-3484): a flip ends when its torque time is
 // over; a running flip turns the car by its relative torque (rounded to whole units, pitch reduced by a cancelling
 // pitch input) as an angular acceleration, and, queued, as a torque about the game's quaternion axes through the
-// inverse of the frame's Bullet rows when a component reaches 1e-4.  The step defers that torque until after air
-// control; this phase queues it at once.  raw_pitch: the control's pitch before quantization, or 2 for none.
+// inverse of the frame's Bullet rows when a component reaches 1e-4.  The vehicle update defers that torque until
+// after air control (`deferred` receives it); without `deferred` it is queued at once.  raw_pitch: the control's pitch before quantization, or 2 for none.
 RG_HD void CarApplyFlipTorque(CarState& car, const CarControls& c, const CarConfig& cfg, const CarParams& m,
-                              const CarFrame& frame, float raw_pitch, CarForceQueue* q) {
+                              const CarFrame& frame, float raw_pitch, CarForceQueue* q, Vec3* deferred = nullptr) {
     if (car.is_flipping) car.is_flipping = (car.has_flipped && car.internal_flip_time < m.flip_torque_time) ? 1 : 0;
     if (!car.is_flipping) return;
     const Vec3 rel = car.flip_rel_torque;
@@ -1493,7 +1498,11 @@ RG_HD void CarApplyFlipTorque(CarState& car, const CarControls& c, const CarConf
         Vec3 w0, w1, w2;
         InverseRows(frame.inv_row0, frame.inv_row1, frame.inv_row2, w0, w1, w2);
         const Vec3 t = MulRows(w0, w1, w2, tw);
-        if (t.x != 0.0f || t.y != 0.0f || t.z != 0.0f) QueueCarTorque(*q, t);
+        if (deferred) {
+            *deferred = t;
+        } else if (t.x != 0.0f || t.y != 0.0f || t.z != 0.0f) {
+            QueueCarTorque(*q, t);
+        }
     }
 }
 

@@ -62,6 +62,11 @@ RG_HD int Ctz32(uint32_t x) {
 #endif
 }
 
+RG_HD int Ctz64(uint64_t x) {
+    const uint32_t lo = static_cast<uint32_t>(x);
+    return lo != 0u ? Ctz32(lo) : 32 + Ctz32(static_cast<uint32_t>(x >> 32));
+}
+
 // ------------------------------------------------------------------------------------------------------------
 // Vec3 / Quat.
 

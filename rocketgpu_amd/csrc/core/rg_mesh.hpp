@@ -148,6 +148,9 @@ struct RayHit {
     Vec3 normal = Vec3(0.0f, 0.0f, 1.0f);
     int32_t tri = -1;
     uint8_t hit = 0;
+    // Hits nearer than sqrt(min_fraction_sq) of the segment are ignored (a wheel ray starts inside the car's own
+    // hitbox; ArenaRayHit::cal_min_fraction_sq, arena_ball.hpp:3192-3204).  0: no limit.
+    float min_fraction_sq = 0.0f;
 };
 
 // Slab test of a leaf box against the ray (mesh_bvh.hpp:522-591: Bullet's btRayAabb2 with a normalized direction).
@@ -206,6 +209,7 @@ RG_HD void RayTriangle(const MeshTriangle& tri, int32_t tri_idx, const Vec3& fro
     n = Normalize(n, tri.normal);
     if (dist_a <= 0.0f) n = -n;
     if (distance < 0.0f || distance > 1.0f) return;
+    if (best.min_fraction_sq > 0.0f && distance * distance < best.min_fraction_sq) return;
     if (!best.hit || distance < best.fraction) {
         best.hit = 1;
         best.fraction = distance;
@@ -256,6 +260,7 @@ RG_HD RayCandidate RayTriangleCandidate(const MeshTriangle& tri, int32_t tri_idx
 
 RG_HD void CommitRayCandidate(const RayCandidate& c, RayHit& best) {
     if (!c.valid || c.distance >= best.fraction) return;
+    if (best.min_fraction_sq > 0.0f && c.distance * c.distance < best.min_fraction_sq) return;
     if (!best.hit || c.distance < best.fraction) {
         best.hit = 1;
         best.fraction = c.distance;
@@ -266,8 +271,9 @@ RG_HD void CommitRayCandidate(const RayCandidate& c, RayHit& best) {
 
 // Nearest hit of the segment from -> from + delta over every mesh (meshes in order, leaves in node order).
 template <typename Walk = SerialWalk>
-RG_HD RayHit CastArenaRay(const ArenaMeshView& m, const Vec3& from, const Vec3& delta) {
+RG_HD RayHit CastArenaRay(const ArenaMeshView& m, const Vec3& from, const Vec3& delta, float min_fraction_sq = 0.0f) {
     RayHit best;
+    best.min_fraction_sq = min_fraction_sq;
     const Vec3 to = from + delta;
     const Vec3 lo = Min3(from, to), hi = Max3(from, to);
     for (int k = 0; k < m.mesh_count; ++k) {

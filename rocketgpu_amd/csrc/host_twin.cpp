@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "batch_common.hpp"
+#include "core/rg_arena_step.hpp"
 #include "core/rg_car_phases.hpp"
 #include "core/rg_env_commands.hpp"
 
@@ -452,4 +453,142 @@ int rgt_math(int fn, const float* x, const float* y, float* out, long n) {
     return 0;
 }
 
+
+// ---- Car arenas (test-only twin of the car step) ----------------------------------------------------------------
+
+namespace {
+struct CarTwin {
+    ArenaMeshHost mesh;
+    core::ArenaMeshView view{};
+    core::StepParams params{};
+    core::CarParams m{};
+    core::CarSpawnRules rules{};
+    std::vector<Vec3> pad_pos;
+    int env_count = 0, max_cars = 0;
+    uint64_t seed = 0;
+    std::vector<BallState> balls;
+    std::vector<EnvState> envs;
+    std::vector<core::ArenaInternal> internal;
+    std::vector<BoostPadState> pads;
+    std::vector<CarState> cars;
+    std::vector<WheelRuntimeState> wheels;
+    std::vector<CarConfig> cfgs;
+    std::vector<CarControls> controls;
+    std::vector<uint8_t> teams;
+    std::vector<int32_t> counts;
+};
+}  // namespace
+
+// A batch of env_count car arenas with max_cars slots each on PA(scale, fillet); `seed` is the respawn seed.  Every
+// record starts as its default; the caller uploads the states (raw bytes, no canonicalization).
+void* rgt_cars_create(int env_count, int max_cars, double scale, double fillet_radius, uint64_t seed) {
+    try {
+        auto t = std::make_unique<CarTwin>();
+        rgw::ArenaParams ap;
+        ap.scale = scale;
+        ap.fillet_radius = fillet_radius;
+        t->mesh = BuildProceduralArenaMesh(ap);
+        t->view.tris = t->mesh.tris.data();
+        t->view.nodes = t->mesh.nodes.data();
+        t->view.meshes = t->mesh.meshes.data();
+        t->view.mesh_count = static_cast<int32_t>(t->mesh.meshes.size());
+        t->view.tri_count = static_cast<int32_t>(t->mesh.tris.size());
+        const rgw::ArenaRulesSpec rules = rgw::ProceduralArenaRules(ap);
+        t->params = DefaultStepParams(rules);
+        t->rules = SpawnRules(rules);
+        for (int k = 0; k < rules.pad_count; ++k) {
+            t->pad_pos.push_back(Vec3(rules.pad_pos[k].x, rules.pad_pos[k].y, rules.pad_pos[k].z));
+        }
+        t->m.frame_frac = t->params.frame_frac;
+        t->env_count = env_count;
+        t->max_cars = max_cars;
+        t->seed = seed;
+        const size_t n = static_cast<size_t>(env_count), cap = n * static_cast<size_t>(max_cars);
+        t->balls.assign(n, BallState{});
+        t->envs.assign(n, EnvState{});
+        t->internal.assign(n, core::ArenaInternal{});
+        std::memset(static_cast<void*>(t->internal.data()), 0, n * sizeof(core::ArenaInternal));
+        t->pads.assign(n * static_cast<size_t>(t->params.pad_count), BoostPadState{});
+        t->cars.assign(cap, CarState{});
+        t->wheels.assign(cap, WheelRuntimeState{});
+        t->cfgs.assign(cap, CarConfig{});
+        t->controls.assign(cap, CarControls{});
+        t->teams.assign(cap, 0);
+        t->counts.assign(n, 0);
+        return t.release();
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return nullptr;
+    }
+}
+
+void rgt_cars_destroy(void* p) { delete static_cast<CarTwin*>(p); }
+
+// which: 0 CarState, 1 BallState, 2 EnvState, 3 BoostPadState, 4 CarControls, 5 CarConfig, 6 env car counts (i32),
+// 7 teams (u8), 8 WheelRuntimeState; 9 the unsupported words (u32 per env; download only).
+static void* CarTwinArray(CarTwin& t, int which, size_t& item) {
+    switch (which) {
+    case 0: item = sizeof(CarState); return t.cars.data();
+    case 1: item = sizeof(BallState); return t.balls.data();
+    case 2: item = sizeof(EnvState); return t.envs.data();
+    case 3: item = sizeof(BoostPadState); return t.pads.data();
+    case 4: item = sizeof(CarControls); return t.controls.data();
+    case 5: item = sizeof(CarConfig); return t.cfgs.data();
+    case 6: item = sizeof(int32_t); return t.counts.data();
+    case 7: item = 1; return t.teams.data();
+    case 8: item = sizeof(WheelRuntimeState); return t.wheels.data();
+    default: item = 0; return nullptr;
+    }
+}
+
+int rgt_cars_upload(void* p, int which, const void* src, long n) {
+    CarTwin& t = *static_cast<CarTwin*>(p);
+    size_t item = 0;
+    void* dst = CarTwinArray(t, which, item);
+    if (dst == nullptr) return -1;
+    std::memcpy(dst, src, static_cast<size_t>(n) * item);
+    return 0;
+}
+
+int rgt_cars_download(void* p, int which, void* dst, long n) {
+    CarTwin& t = *static_cast<CarTwin*>(p);
+    if (which == 9) {
+        uint32_t* o = static_cast<uint32_t*>(dst);
+        for (long e = 0; e < n; ++e) o[e] = t.internal[static_cast<size_t>(e)].unsupported;
+        return 0;
+    }
+    size_t item = 0;
+    void* src = CarTwinArray(t, which, item);
+    if (src == nullptr) return -1;
+    std::memcpy(dst, src, static_cast<size_t>(n) * item);
+    return 0;
+}
+
+// `ticks` ticks of every arena with the uploaded controls (BatchSim::step(ticks)).
+int rgt_cars_step(void* p, int ticks) {
+    CarTwin& t = *static_cast<CarTwin*>(p);
+    auto scratch = std::make_unique<core::CarArenaTick>();
+    const core::PadLayout pads{t.pad_pos.data(), t.params.pad_count};
+    for (int k = 0; k < ticks; ++k) {
+        for (int e = 0; e < t.env_count; ++e) {
+            const size_t base = static_cast<size_t>(e) * t.max_cars;
+            core::CarArenaRefs a;
+            a.ball = &t.balls[e];
+            a.env = &t.envs[e];
+            a.st = &t.internal[e];
+            a.pads = t.pads.data() + static_cast<size_t>(e) * t.params.pad_count;
+            a.cars = t.cars.data() + base;
+            a.wheels = t.wheels.data() + base;
+            a.cfgs = t.cfgs.data() + base;
+            a.teams = t.teams.data() + base;
+            a.controls = t.controls.data() + base;
+            a.count = t.counts[e];
+            a.env_idx = e;
+            a.car_base = static_cast<int>(base);
+            core::CarRespawnContext rc{&t.rules, t.seed, e, 0u, 0u, 0};
+            core::StepCarArena(a, t.view, t.params, t.m, pads, rc, k + 1 == ticks, *scratch);
+        }
+    }
+    return 0;
+}
 }  // extern "C"
