@@ -31,6 +31,14 @@ extern "C" {
 #define RG_ARRAY_BOOST_PAD_STATES 3
 #define RG_ARRAY_CONTROLS 4
 #define RG_ARRAY_ENV_CAR_COUNT 5
+/* WheelRuntimeState records (292 B) per car slot: upload_wheel_runtime_states / download_wheel_runtime_states
+ * (rocketgpu.h:1037-1039). */
+#define RG_ARRAY_WHEEL_STATES 6
+/* Download only: one uint32 per env, the physics an arena with cars met that this build does not step (bit 0 a
+ * car's hitbox near a mesh triangle, 1 ball-car, 2 car-car, 3 a wheel ray near another car, 4 a car's predictive
+ * contact, 5 a respawn spot pick, 6 set-state wheel normals, 7 a three-wheel body); sticky until the arena is reset.
+ * An arena whose word is 0 has been stepped exactly as the reference steps it.  No counterpart in the reference. */
+#define RG_ARRAY_UNSUPPORTED 7
 
 typedef struct rg_batch rg_batch;
 
@@ -97,6 +105,13 @@ int rg_batch_set_env_car_count(rg_batch* batch, const int32_t* host_counts, size
 /* upload_car_states / upload_ball_states / upload_env_states / upload_boost_pad_states / upload_controls
  * (rocketgpu.h:1028-1034), public-state mode: uploaded ball states are canonicalized as the reference does. */
 int rg_batch_upload(rg_batch* batch, int array_id, const void* host_src, size_t count, void* stream);
+/* The same with the reference's StateUploadMode (rocketgpu.h:1027-1030): RG_UPLOAD_PUBLIC_STATE canonicalizes
+ * uploaded car and ball records (rg_batch_upload's behaviour), RG_UPLOAD_PRESERVE_INTERNAL takes them as given (a
+ * state downloaded from a batch resumes bit for bit). */
+#define RG_UPLOAD_PUBLIC_STATE 0
+#define RG_UPLOAD_PRESERVE_INTERNAL 1
+int rg_batch_upload_mode(rg_batch* batch, int array_id, const void* host_src, size_t count, int32_t mode,
+                         void* stream);
 /* download_*_states (rocketgpu.h:1036-1053); synchronous like the reference's non-async downloads. */
 int rg_batch_download(rg_batch* batch, int array_id, void* host_dst, size_t count, void* stream);
 
@@ -203,6 +218,13 @@ uint64_t rg_fold_digests(const uint64_t* host_env_digests, int64_t n);
  * launches of rocketgpu_amd/workload.h) for arenas first_arena.. on PA(scale, .).  No reference counterpart; the
  * reference's benchmark draws its own inputs (tools/bench_cpp.cpp:300-360). */
 int rg_workload_ball_states(uint64_t seed, int64_t first_arena, int64_t count, double scale, void* out_ball_states);
+
+/* Synthetic workload: seeded held controls (rocketgpu_amd/workload.h ControlStream: axes uniform in [-1, 1], jump /
+ * boost / handbrake with probability 0.1 / p_boost / 0.05, each draw held 1..30 ticks) for cars_per_arena slots of
+ * arenas first_arena..first_arena + arena_count - 1, written as CarControls [ticks][arena_count * cars_per_arena].
+ * No reference counterpart. */
+int rg_workload_controls(uint64_t seed, int64_t first_arena, int64_t arena_count, int32_t cars_per_arena,
+                         int32_t ticks, float p_boost, void* out_controls);
 
 /* Serving loop (no reference counterpart; the reference has no server).  `clients` simulated actors own disjoint
  * ranges of min_arenas..max_arenas resident arenas and send "step my arenas k ticks" requests (k drawn from

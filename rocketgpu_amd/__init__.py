@@ -25,9 +25,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("RG_LIBRARY_PATH") or os.path.join(HERE, "_build", "librocketgpu_amd.so")
 
 RG_ARRAY = {"car_states": 0, "ball_states": 1, "env_states": 2, "boost_pad_states": 3, "controls": 4,
-            "env_car_count": 5}
+            "env_car_count": 5, "wheel_states": 6, "unsupported": 7}
 ITEMSIZE = {"car_states": 880, "ball_states": 288, "env_states": 44, "boost_pad_states": 16, "controls": 24,
-            "env_car_count": 4, "car_configs": 120, "teams": 1, "env_commands": 16}
+            "env_car_count": 4, "car_configs": 120, "teams": 1, "env_commands": 16, "wheel_states": 292,
+            "unsupported": 4}
 
 # EnvCommand (include/rocketgpu/rocketgpu.h:683-707 of the reference), 16 bytes.
 ENV_COMMAND_DTYPE = np.dtype([("reset_to_kickoff", "u1"), ("reset_scoreboard", "u1"), ("reset_tick_count", "u1"),
@@ -128,6 +129,7 @@ def lib():
         L.rg_batch_car_capacity.restype = ctypes.c_int64
         L.rg_batch_set_env_car_count.argtypes = [vp, vp, ctypes.c_size_t, vp]
         L.rg_batch_upload.argtypes = [vp, ctypes.c_int, vp, ctypes.c_size_t, vp]
+        L.rg_batch_upload_mode.argtypes = [vp, ctypes.c_int, vp, ctypes.c_size_t, ctypes.c_int32, vp]
         L.rg_batch_download.argtypes = [vp, ctypes.c_int, vp, ctypes.c_size_t, vp]
         L.rg_batch_step.argtypes = [vp, ctypes.c_int32, vp]
         L.rg_batch_step_with_device_controls.argtypes = [vp, vp, ctypes.c_int32, ctypes.c_int32, vp]
@@ -135,6 +137,8 @@ def lib():
         L.rg_batch_device_view.argtypes = [vp, ctypes.POINTER(_DeviceView)]
         L.rg_batch_env_digests.argtypes = [vp, vp, vp]
         L.rg_workload_ball_states.argtypes = [ctypes.c_uint64, ctypes.c_int64, ctypes.c_int64, ctypes.c_double, vp]
+        L.rg_workload_controls.argtypes = [ctypes.c_uint64, ctypes.c_int64, ctypes.c_int64, ctypes.c_int32,
+                                           ctypes.c_int32, ctypes.c_float, vp]
         L.rg_batch_step_arenas.argtypes = [vp, vp, vp, ctypes.c_int32, ctypes.c_int32, vp]
         L.rg_batch_step_graph.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, vp]
         L.rg_batch_ball_path.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_float, vp, vp, vp]
@@ -185,6 +189,15 @@ def workload_ball_states(seed: int, first_arena: int, count: int, scale: float =
     return out
 
 
+def workload_controls(seed: int, first_arena: int, arena_count: int, cars_per_arena: int, ticks: int,
+                      p_boost: float = 0.3) -> np.ndarray:
+    """Seeded held controls ([ticks, arena_count * cars_per_arena, 24] uint8 CarControls records)."""
+    out = np.zeros((ticks, arena_count * cars_per_arena, ITEMSIZE["controls"]), dtype=np.uint8)
+    _check(lib().rg_workload_controls(seed, first_arena, arena_count, cars_per_arena, ticks, p_boost,
+                                      out.ctypes.data))
+    return out
+
+
 def _stream_ptr(stream):
     if stream is None:
         return None
@@ -221,18 +234,21 @@ class BatchSim:
 
     def _count(self, name):
         return {"car_states": self.car_capacity, "controls": self.car_capacity, "car_configs": self.car_capacity,
-                "teams": self.car_capacity, "boost_pad_states": self.boost_pad_capacity}.get(name, self.env_count)
+                "teams": self.car_capacity, "wheel_states": self.car_capacity,
+                "boost_pad_states": self.boost_pad_capacity}.get(name, self.env_count)
 
     def set_env_car_count(self, counts, stream=None):
         a = np.ascontiguousarray(counts, dtype=np.int32)
         _check(lib().rg_batch_set_env_car_count(self._h, a.ctypes.data, a.size, _stream_ptr(stream)))
 
-    def upload(self, name: str, rows, stream=None):
+    def upload(self, name: str, rows, stream=None, preserve_internal: bool = False):
+        """upload_*_states; preserve_internal selects StateUploadMode::PreserveInternal (records as given)."""
         a = np.ascontiguousarray(rows)
         n = a.nbytes // ITEMSIZE[name]
         if n * ITEMSIZE[name] != a.nbytes:
             raise ValueError(f"{name}: byte size is not a multiple of {ITEMSIZE[name]}")
-        _check(lib().rg_batch_upload(self._h, RG_ARRAY[name], a.ctypes.data, n, _stream_ptr(stream)))
+        _check(lib().rg_batch_upload_mode(self._h, RG_ARRAY[name], a.ctypes.data, n, 1 if preserve_internal else 0,
+                                          _stream_ptr(stream)))
 
     def download_state_array(self, name: str, host_ptr: int = 0, count: int | None = None, stream=None):
         """download_state_array (python/rocketgpu_bindings.cpp:984-1000): `count` records (all by default) of

@@ -128,4 +128,95 @@ inline void CanonicalizeBall(BallState& b) {
     b.deactivation_time = 0.0f;
 }
 
+// Bullet's btMatrix3x3::getRotation (math.hpp:988-1028): Shepperd's method on the largest diagonal term.
+inline Quat BulletBasisToQuat(const Vec3& f, const Vec3& r, const Vec3& u) {
+    const float m00 = f.x, m01 = r.x, m02 = u.x;
+    const float m10 = f.y, m11 = r.y, m12 = u.y;
+    const float m20 = f.z, m21 = r.z, m22 = u.z;
+    const float trace = m00 + m11 + m22;
+    float q[4], x;  // x y z w before scaling
+    if (trace > 0.0f) {
+        x = trace + 1.0f;
+        q[0] = m21 - m12, q[1] = m02 - m20, q[2] = m10 - m01, q[3] = x;
+    } else {
+        const int i = m00 < m11 ? (m11 < m22 ? 2 : 1) : (m00 < m22 ? 2 : 0);
+        if (i == 0) {
+            x = m00 - m11 - m22 + 1.0f;
+            q[0] = x, q[1] = m10 + m01, q[2] = m20 + m02, q[3] = m21 - m12;
+        } else if (i == 1) {
+            x = m11 - m22 - m00 + 1.0f;
+            q[0] = m01 + m10, q[1] = x, q[2] = m21 + m12, q[3] = m02 - m20;
+        } else {
+            x = m22 - m00 - m11 + 1.0f;
+            q[0] = m02 + m20, q[1] = m12 + m21, q[2] = x, q[3] = m10 - m01;
+        }
+    }
+    const float scale = 0.5f / std::sqrt(x);
+    return Quat(q[3] * scale, q[0] * scale, q[1] * scale, q[2] * scale);
+}
+
+// Public car state upload (CanonicalizeImportedCarStatesInPlace, src/rocketgpu.cpp:355-371): a car that has been
+// stepped gets its working fields back from the last report (and reports them again afterwards), its contact history
+// restarts, position and velocity go through bt, and the orientation becomes an authoritative basis with the Bullet
+// quaternion of that basis.
+inline void CanonicalizeCar(CarState& c) {
+    c.pad_frame_pos_valid = 0;
+    constexpr uint8_t kVel = 1, kAngVel = 2, kJumped = 4, kDoubleJumped = 8, kFlipped = 16, kSupersonic = 32,
+                      kValid = 128;
+    uint8_t kept = 0;
+    if (c.tick_count_since_update != 0 && (c.reported_flags & kValid)) {
+        kept = kValid;
+        if (c.vel.x == c.reported_vel.x && c.vel.y == c.reported_vel.y && c.vel.z == c.reported_vel.z) {
+            c.vel = c.internal_vel;
+            kept |= kVel;
+        }
+        if (c.ang_vel.x == c.reported_ang_vel.x && c.ang_vel.y == c.reported_ang_vel.y &&
+            c.ang_vel.z == c.reported_ang_vel.z) {
+            c.ang_vel = c.internal_ang_vel;
+            kept |= kAngVel;
+        }
+        const uint8_t rf = c.reported_flags;
+        core::RestoreWorkingFlag(c.has_jumped, c.internal_has_jumped, kJumped, rf, kept);
+        core::RestoreWorkingFlag(c.has_double_jumped, c.internal_has_double_jumped, kDoubleJumped, rf, kept);
+        core::RestoreWorkingFlag(c.has_flipped, c.internal_has_flipped, kFlipped, rf, kept);
+        core::RestoreWorkingFlag(c.is_supersonic, c.internal_is_supersonic, kSupersonic, rf, kept);
+    }
+    c.restored_inv_inertia_valid = 0;
+    c.world_contact.wheel_ball_mask = 0;
+    c.previous_ball_contact_start = c.last_ball_contact_start = c.last_world_surface_contact = ~0ULL;
+    c.first_world_after_previous_ball = c.first_world_after_last_ball = ~0ULL;
+    c.last_flip_or_jump_start = c.last_directional_flip_start = ~0ULL;
+    c.ball_contact_last_tick = c.ball_contact_this_tick = c.last_ball_started_with_world = 0;
+    c.flip_or_jump_last_tick = core::HasFlipOrJump(c) ? 1 : 0;
+    c.pos_bt = Vec3(ImportedScalarBt(c.pos.x), ImportedScalarBt(c.pos.y), ImportedScalarBt(c.pos.z));
+    c.vel_bt = Vec3(ImportedScalarBt(c.vel.x), ImportedScalarBt(c.vel.y), ImportedScalarBt(c.vel.z));
+    c.pos = Vec3(c.pos_bt.x * 50.0f, c.pos_bt.y * 50.0f, c.pos_bt.z * 50.0f);
+    c.vel = Vec3(c.vel_bt.x * 50.0f, c.vel_bt.y * 50.0f, c.vel_bt.z * 50.0f);
+    if (!c.basis_authoritative) {
+        core::QuatColumns(c.rot, c.basis_forward, c.basis_right, c.basis_up);
+        c.basis_authoritative = 1;
+    }
+    c.rot = BulletBasisToQuat(c.basis_forward, c.basis_right, c.basis_up);
+    if (kept & kValid) {
+        c.internal_vel = c.vel;
+        c.internal_ang_vel = c.ang_vel;
+        c.internal_has_jumped = c.has_jumped;
+        c.internal_has_double_jumped = c.has_double_jumped;
+        c.internal_has_flipped = c.has_flipped;
+        c.internal_is_supersonic = c.is_supersonic;
+        if (kept & kVel) c.vel = c.reported_vel; else c.reported_vel = c.vel;
+        if (kept & kAngVel) c.ang_vel = c.reported_ang_vel; else c.reported_ang_vel = c.ang_vel;
+        uint8_t flags = kValid;
+        const auto report = [&](uint8_t& field, uint8_t bit) {
+            if (kept & bit) field = (c.reported_flags & bit) ? 1 : 0;
+            if (field) flags |= bit;
+        };
+        report(c.has_jumped, kJumped);
+        report(c.has_double_jumped, kDoubleJumped);
+        report(c.has_flipped, kFlipped);
+        report(c.is_supersonic, kSupersonic);
+        c.reported_flags = flags;
+    }
+}
+
 }  // namespace rg

@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "batch_common.hpp"
+#include "core/rg_arena_step.hpp"
 #include "core/rg_env_commands.hpp"
 #include "group_walk.hpp"
 #include "rocketgpu_amd/rg_capi.h"
@@ -269,6 +270,64 @@ __global__ void __launch_bounds__(64) ResetToKickoffKernel(StepArgs a, uint64_t 
     ClearEnvControls(a, e);
 }
 
+// What a car arena's tick reads beyond the ball arena's: the car records, the car constants, the arena's spawn
+// spots and pads, the batch seed of the respawn draws, and where the tick's controls come from.
+struct CarStepArgs {
+    CarState* cars;
+    WheelRuntimeState* wheels;
+    const CarConfig* cfgs;
+    const uint8_t* teams;
+    // Controls of tick t for car slot k: controls[t * control_stride + k] (stride 0: the same row every tick).
+    const CarControls* controls;
+    int64_t control_stride;
+    core::CarParams m;
+    core::CarSpawnRules rules;
+    Vec3 pad_pos[rgw::kMaxPads];
+    uint64_t seed;
+};
+
+// One lane per arena, its ticks back to back: arenas without cars take the ball tick, arenas with cars the full
+// car tick (one lane walks all of its queries).  Fields the game reports are refreshed on the call's last tick.
+__global__ void __launch_bounds__(64) StepCarArenasKernel(StepArgs a, CarStepArgs c, int32_t ticks) {
+    const int i = static_cast<int>(blockIdx.x) * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x);
+    if (i >= a.count) return;
+    const int e = a.arena_ids ? a.arena_ids[i] : i;
+    if (e < 0 || e >= a.env_count) return;
+    const int n = a.arena_ticks ? a.arena_ticks[i] : ticks;
+    const int count = a.env_car_count[e];
+    if (count == 0) {
+        core::BallTickScratch scratch;
+        for (int t = 0; t < n; ++t) Tick<false>(a.balls[e], a.envs[e], a.internal[e], a, scratch, core::EventSink{});
+        return;
+    }
+    core::CarArenaTick scratch;
+    const int base = e * a.max_cars;
+    const core::PadLayout pads{c.pad_pos, a.params.pad_count};
+    for (int t = 0; t < n; ++t) {
+        core::CarArenaRefs r;
+        r.ball = &a.balls[e];
+        r.env = &a.envs[e];
+        r.st = &a.internal[e];
+        r.pads = a.pads + static_cast<size_t>(e) * a.params.pad_count;
+        r.cars = c.cars + base;
+        r.wheels = c.wheels + base;
+        r.cfgs = c.cfgs + base;
+        r.teams = c.teams + base;
+        r.controls = c.controls + static_cast<int64_t>(t) * c.control_stride + base;
+        r.count = count;
+        r.env_idx = e;
+        r.car_base = base;
+        const core::CarRespawnContext rc{&c.rules, c.seed, e, 0u, 0u, 0};
+        core::StepCarArena(r, a.mesh, a.params, c.m, pads, rc, t + 1 == n, scratch);
+    }
+}
+
+// The unsupported words of the arenas (0 for every arena stepped in full).
+__global__ void __launch_bounds__(256) UnsupportedKernel(const core::ArenaInternal* st, int32_t n, uint32_t* out) {
+    const int e = static_cast<int>(blockIdx.x) * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x);
+    if (e < n) out[e] = st[e].unsupported;
+}
+
 template <int G, bool kEvents>
 __device__ inline void StepGroupArenaWith(const StepArgs& a, int e, int n, core::BallTickScratch& scratch) {
     const core::EventSink events = EnvEvents(a, e, (static_cast<int>(threadIdx.x) % G) == 0);
@@ -445,6 +504,13 @@ public:
             params_.ball_drag <= 0.0f ? 1.0f
                                       : core::UcrtPowf(core::Max(0.0f, 1.0f - params_.ball_drag), params_.tick_dt);
         params_.enable_scoring = cfg.enable_scoring;
+        car_args_ = CarStepArgs{};
+        car_args_.m.tick_dt = cfg.tick_dt;
+        car_args_.m.frame_frac = params_.frame_frac;
+        car_args_.rules = SpawnRules(rules);
+        for (int k = 0; k < rules.pad_count && k < rgw::kMaxPads; ++k) {
+            car_args_.pad_pos[k] = Vec3(rules.pad_pos[k].x, rules.pad_pos[k].y, rules.pad_pos[k].z);
+        }
         const size_t n = static_cast<size_t>(cfg.env_count);
         const size_t cap = n * static_cast<size_t>(cfg.max_cars_per_env);
         balls_.Resize(n);
@@ -453,6 +519,7 @@ public:
         car_count_.Resize(n);
         cars_.Resize(cap);
         controls_.Resize(cap);
+        wheels_.Resize(cap);
         teams_.Resize(cap);
         configs_.Resize(cap);
         pads_.Resize(n * static_cast<size_t>(params_.pad_count));
@@ -468,6 +535,8 @@ public:
         std::vector<CarState> cars(cap);
         std::vector<CarControls> controls(cap);
         std::vector<CarConfig> configs(cap);
+        std::vector<WheelRuntimeState> wheels(cap);
+        ZeroedDefaults(wheels);
         ZeroedDefaults(cars);
         ZeroedDefaults(controls);
         ZeroedDefaults(configs);
@@ -478,7 +547,10 @@ public:
         cars_.Upload(cars.data(), cap, nullptr);
         controls_.Upload(controls.data(), cap, nullptr);
         configs_.Upload(configs.data(), cap, nullptr);
+        wheels_.Upload(wheels.data(), cap, nullptr);
         teams_.Upload(teams.data(), cap, nullptr);
+        host_counts_.assign(n, 0);
+        unsupported_.Resize(n);
         Check(hipMemset(internal_.data(), 0, n * sizeof(core::ArenaInternal)), "hipMemset");
         if (pads_.size()) Check(hipMemset(pads_.data(), 0, pads_.size() * sizeof(BoostPadState)), "hipMemset");
         Check(hipDeviceSynchronize(), "hipDeviceSynchronize");
@@ -526,6 +598,10 @@ public:
 
     // reset_to_kickoff (src/rocketgpu.cpp:2085-2133): every env to a new match's kickoff.
     void ResetToKickoff(uint64_t seed, hipStream_t s) {
+        if (has_cars_) {
+            throw InvalidArgument("kickoff resets of arenas with cars are not built yet (upload the states instead)");
+        }
+        seed_ = seed;
         EnsureSettledBall();
         StepArgs a = BaseArgs();
         const int grid = (cfg_.env_count + 63) / 64;
@@ -536,6 +612,7 @@ public:
     // apply_env_commands (src/rocketgpu.cpp:2136-2218): command i to env i for i < count.  Same checks and messages.
     void ApplyEnvCommands(const EnvCommand* host_commands, size_t count, hipStream_t s) {
         if (host_commands == nullptr) throw InvalidArgument("host_commands must not be null");
+        RejectCarCommands();
         if (count > static_cast<size_t>(cfg_.env_count)) {
             throw std::out_of_range("apply_env_commands_host count exceeds env_count");
         }
@@ -556,6 +633,7 @@ public:
     // apply_env_commands_device (src/rocketgpu.cpp:2221-2248): the commands are read on the device.
     void ApplyEnvCommandsDevice(const EnvCommand* device_commands, size_t count, hipStream_t s) {
         if (device_commands == nullptr) throw InvalidArgument("device_commands must not be null");
+        RejectCarCommands();
         if (count > static_cast<size_t>(cfg_.env_count)) {
             throw std::out_of_range("apply_env_commands_device count exceeds env_count");
         }
@@ -571,6 +649,7 @@ public:
         if (ticks <= 0) return;
         if (controls == nullptr) throw InvalidArgument("device_controls_by_tick must not be null");
         if (commands == nullptr) throw InvalidArgument("device_env_commands_by_tick must not be null");
+        RejectCarCommands();
         EnsureSettledBall();
         if (control_stride != 0 && control_stride < car_capacity()) {
             throw InvalidArgument("control_stride must be zero or at least the total car capacity");
@@ -602,21 +681,25 @@ public:
             if (counts[i] < 0 || counts[i] > cfg_.max_cars_per_env) {
                 throw std::out_of_range("env car count outside 0..max_cars_per_env");
             }
-            if (counts[i] != 0) {
-                throw InvalidArgument("arenas with cars are not supported by this build yet (ball-only arenas only)");
-            }
         }
         car_count_.Upload(counts, n, s);
+        host_counts_.assign(counts, counts + n);
+        has_cars_ = std::any_of(host_counts_.begin(), host_counts_.end(), [](int32_t c) { return c > 0; });
+        DestroyGraph();
         Check(hipStreamSynchronize(s), "hipStreamSynchronize");
     }
 
-    void Upload(int id, const void* src, size_t n, hipStream_t s) {
+    // preserve: StateUploadMode::PreserveInternal (rocketgpu.h:1027-1030), the records are taken as they are;
+    // otherwise PublicState: car and ball records are canonicalized as the reference's upload does.
+    void Upload(int id, const void* src, size_t n, hipStream_t s, bool preserve = false) {
         switch (id) {
         case RG_ARRAY_BALL_STATES: {
             Expect(n, balls_.size(), "ball states");
             std::vector<BallState> tmp(n);
-            std::memcpy(tmp.data(), src, n * sizeof(BallState));
-            for (BallState& b : tmp) CanonicalizeBall(b);
+            std::memcpy(static_cast<void*>(tmp.data()), src, n * sizeof(BallState));
+            if (!preserve) {
+                for (BallState& b : tmp) CanonicalizeBall(b);
+            }
             balls_.Upload(tmp.data(), n, s);
             Check(hipStreamSynchronize(s), "hipStreamSynchronize");
             break;
@@ -631,14 +714,25 @@ public:
             controls_.Upload(static_cast<const CarControls*>(src), n, s);
             Check(hipStreamSynchronize(s), "hipStreamSynchronize");
             break;
-        case RG_ARRAY_CAR_STATES:
+        case RG_ARRAY_CAR_STATES: {
             Expect(n, cars_.size(), "car states");
-            cars_.Upload(static_cast<const CarState*>(src), n, s);
+            std::vector<CarState> tmp(n);
+            std::memcpy(static_cast<void*>(tmp.data()), src, n * sizeof(CarState));
+            if (!preserve) {
+                for (CarState& c : tmp) CanonicalizeCar(c);
+            }
+            cars_.Upload(tmp.data(), n, s);
             Check(hipStreamSynchronize(s), "hipStreamSynchronize");
             break;
+        }
         case RG_ARRAY_BOOST_PAD_STATES:
             Expect(n, pads_.size(), "boost pad states");
             pads_.Upload(static_cast<const BoostPadState*>(src), n, s);
+            Check(hipStreamSynchronize(s), "hipStreamSynchronize");
+            break;
+        case RG_ARRAY_WHEEL_STATES:
+            Expect(n, wheels_.size(), "wheel runtime states");
+            wheels_.Upload(static_cast<const WheelRuntimeState*>(src), n, s);
             Check(hipStreamSynchronize(s), "hipStreamSynchronize");
             break;
         default:
@@ -654,6 +748,15 @@ public:
         case RG_ARRAY_CONTROLS: controls_.Download(static_cast<CarControls*>(dst), n, s); break;
         case RG_ARRAY_BOOST_PAD_STATES: pads_.Download(static_cast<BoostPadState*>(dst), n, s); break;
         case RG_ARRAY_ENV_CAR_COUNT: car_count_.Download(static_cast<int32_t*>(dst), n, s); break;
+        case RG_ARRAY_WHEEL_STATES: wheels_.Download(static_cast<WheelRuntimeState*>(dst), n, s); break;
+        case RG_ARRAY_UNSUPPORTED: {
+            const int grid = (cfg_.env_count + 255) / 256;
+            hipLaunchKernelGGL(UnsupportedKernel, dim3(grid), dim3(256), 0, s, internal_.data(), cfg_.env_count,
+                               unsupported_.data());
+            Check(hipGetLastError(), "UnsupportedKernel launch");
+            unsupported_.Download(static_cast<uint32_t*>(dst), n, s);
+            break;
+        }
         default: throw InvalidArgument("unknown state array id");
         }
         Check(hipStreamSynchronize(s), "hipStreamSynchronize");
@@ -936,6 +1039,10 @@ public:
         if (stride != 0 && stride < car_capacity()) {
             throw InvalidArgument("control_stride must be zero or at least the total car capacity");
         }
+        if (has_cars_) {
+            LaunchCars(nullptr, nullptr, cfg_.env_count, ticks, static_cast<const CarControls*>(controls), stride, s);
+            return;
+        }
         Step(ticks, s);
     }
 
@@ -956,6 +1063,7 @@ public:
     void BallPath(int32_t samples, int32_t ticks, float goal_y, float* out, int32_t* result, hipStream_t s) {
         if (samples <= 0 || ticks <= 0) return;
         if (!out || !result) throw InvalidArgument("ball_path output pointers must not be null");
+        if (has_cars_) throw InvalidArgument("ball_path requires arenas without cars");
         const StepArgs a = BaseArgs();
         const int grid = (cfg_.env_count + 63) / 64;
         if (a.event_windows) {
@@ -975,6 +1083,10 @@ public:
     }
 
 private:
+    void RejectCarCommands() const {
+        if (has_cars_) throw InvalidArgument("env commands on arenas with cars are not built yet");
+    }
+
     void FillHeader(SnapshotHeader& h) const {
         std::memcpy(h.magic, kSnapshotMagic, sizeof(h.magic));
         h.version = kSnapshotVersion;
@@ -1129,7 +1241,33 @@ private:
         settled_ready_ = true;
     }
 
+    // Car arenas in the batch: one lane per arena with the full car tick (ball arenas in the same launch take the
+    // ball tick).  Physics events of car arenas are not written.
+    void LaunchCars(const int32_t* ids, const int32_t* ticks_list, int32_t count, int32_t ticks,
+                    const CarControls* controls, int64_t stride, hipStream_t s) {
+        StepArgs a = BaseArgs();
+        a.arena_ids = ids;
+        a.arena_ticks = ticks_list;
+        a.count = count;
+        a.arenas_per_wave = 64;
+        CarStepArgs c = car_args_;
+        c.cars = cars_.data();
+        c.wheels = wheels_.data();
+        c.cfgs = configs_.data();
+        c.teams = teams_.data();
+        c.controls = controls;
+        c.control_stride = stride;
+        c.seed = seed_;
+        const int grid = (count + 63) / 64;
+        hipLaunchKernelGGL(StepCarArenasKernel, dim3(grid), dim3(64), 0, s, a, c, ticks);
+        Check(hipGetLastError(), "StepCarArenasKernel launch");
+    }
+
     void Launch(const int32_t* ids, const int32_t* ticks_list, int32_t count, int32_t ticks, hipStream_t s) {
+        if (has_cars_) {
+            LaunchCars(ids, ticks_list, count, ticks, controls_.data(), 0, s);
+            return;
+        }
         StepArgs a = BaseArgs();
         a.arena_ids = ids;
         a.arena_ticks = ticks_list;
@@ -1254,6 +1392,12 @@ private:
     DeviceArray<int32_t> car_count_;
     DeviceArray<CarState> cars_;
     DeviceArray<CarControls> controls_;
+    DeviceArray<WheelRuntimeState> wheels_;
+    DeviceArray<uint32_t> unsupported_;
+    std::vector<int32_t> host_counts_;
+    bool has_cars_ = false;
+    uint64_t seed_ = 0;
+    CarStepArgs car_args_{};
     DeviceArray<uint8_t> teams_;
     DeviceArray<CarConfig> configs_;
     DeviceArray<BoostPadState> pads_;
@@ -1400,6 +1544,17 @@ int rg_batch_upload(rg_batch* b, int id, const void* src, size_t count, void* st
     return Guard([&] { b->sim->Upload(id, src, count, S(stream)); });
 }
 
+int rg_batch_upload_mode(rg_batch* b, int id, const void* src, size_t count, int32_t mode, void* stream) {
+    return Guard([&] {
+        if (!b || !b->sim) throw rg::InvalidArgument("null batch");
+        if (mode != RG_UPLOAD_PUBLIC_STATE && mode != RG_UPLOAD_PRESERVE_INTERNAL) {
+            throw rg::InvalidArgument("upload mode must be RG_UPLOAD_PUBLIC_STATE or RG_UPLOAD_PRESERVE_INTERNAL");
+        }
+        if (count && !src) throw rg::InvalidArgument("host_src must not be null");
+        b->sim->Upload(id, src, count, S(stream), mode == RG_UPLOAD_PRESERVE_INTERNAL);
+    });
+}
+
 int rg_batch_download(rg_batch* b, int id, void* dst, size_t count, void* stream) {
     return Guard([&] { b->sim->Download(id, dst, count, S(stream)); });
 }
@@ -1533,6 +1688,24 @@ int rg_batch_env_digests(rg_batch* b, uint64_t* device_out, void* stream) {
 
 uint64_t rg_fold_digests(const uint64_t* d, int64_t n) {
     return n > 0 && d ? rgw::Fnv1a(rgw::kFnvOffset, d, static_cast<size_t>(n) * sizeof(uint64_t)) : rgw::kFnvOffset;
+}
+
+int rg_workload_controls(uint64_t seed, int64_t first_arena, int64_t arena_count, int32_t cars_per_arena,
+                         int32_t ticks, float p_boost, void* out) {
+    return Guard([&] {
+        if (arena_count < 0 || cars_per_arena < 0 || ticks < 0 ||
+            (arena_count * cars_per_arena * ticks > 0 && !out)) {
+            throw rg::InvalidArgument("bad workload request");
+        }
+        const int64_t cars = arena_count * cars_per_arena;
+        rgw::Controls* o = static_cast<rgw::Controls*>(out);
+        for (int64_t c = 0; c < cars; ++c) {
+            rgw::ControlStream cs;
+            cs.seed(seed, static_cast<uint64_t>(first_arena + c / cars_per_arena),
+                    static_cast<uint64_t>(c % cars_per_arena), p_boost);
+            for (int32_t t = 0; t < ticks; ++t) o[static_cast<int64_t>(t) * cars + c] = cs.next();
+        }
+    });
 }
 
 int rg_workload_ball_states(uint64_t seed, int64_t first_arena, int64_t count, double scale, void* out) {

@@ -73,3 +73,16 @@ def oracle_or_skip():
     if not refsim.available():
         pytest.skip("oracle/_ref not built (reference sources absent)")
     return refsim
+
+
+def twin_cars():
+    """The host twin's car-arena API (rgt_cars_*)."""
+    T = twin()
+    vp = ctypes.c_void_p
+    T.rgt_cars_create.restype = vp
+    T.rgt_cars_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_uint64]
+    T.rgt_cars_destroy.argtypes = [vp]
+    T.rgt_cars_upload.argtypes = [vp, ctypes.c_int, vp, ctypes.c_long]
+    T.rgt_cars_download.argtypes = [vp, ctypes.c_int, vp, ctypes.c_long]
+    T.rgt_cars_step.argtypes = [vp, ctypes.c_int]
+    return T

@@ -1,0 +1,67 @@
+"""The car-arena tick of the host twin against the reference CPU build, tick by tick.
+
+Arenas of two Octanes (the 1v1 of the headline workload) and of mixed bodies start from the reference's kickoff and
+step seeded held controls (workload.h ControlStream).  Every record of every arena must match the reference byte for
+byte on every tick while the arena's unsupported word is 0, and the word must rise no later than the first
+difference.  The test also requires most arena-ticks to be compared, so it cannot pass by flagging everything.
+"""
+import numpy as np
+import pytest
+
+from car_cases import arena_mismatches, reference_at_kickoff, reference_records
+from conftest import oracle_or_skip, twin_cars
+
+
+def _twin_records(T, h, n_env, cars):
+    cap = n_env * cars
+    def get(which, shape, dtype=np.uint8):
+        a = np.zeros(shape, dtype)
+        assert T.rgt_cars_download(h, which, a.ctypes.data, shape[0]) == 0
+        return a
+    return {"cars": get(0, (cap, 880)), "wheels": get(8, (cap, 292)), "balls": get(1, (n_env, 288)),
+            "envs": get(2, (n_env, 44)), "pads": get(3, (n_env * 34, 16))}, get(9, (n_env,), np.uint32)
+
+
+def _run(n_env, cars, ticks, seed, presets=None, p_boost=0.3):
+    R = oracle_or_skip()
+    T = twin_cars()
+    import rocketgpu_amd as rga
+
+    ref, counts, teams, cfgs = reference_at_kickoff(R, n_env, cars, seed, presets)
+    h = T.rgt_cars_create(n_env, cars, 1.0, 256.0, seed)
+    assert h
+    rec = reference_records(R, ref, n_env, cars)
+    for which, arr in ((6, counts), (7, teams), (5, cfgs), (0, rec["cars"]), (8, rec["wheels"]), (1, rec["balls"]),
+                       (2, rec["envs"]), (3, rec["pads"])):
+        a = np.ascontiguousarray(arr)
+        assert T.rgt_cars_upload(h, which, a.ctypes.data, a.shape[0]) == 0
+    controls = rga.workload_controls(seed, 0, n_env, cars, ticks, p_boost)
+    compared = 0
+    flagged_at = np.full(n_env, -1)
+    for t in range(ticks):
+        ctl = np.ascontiguousarray(controls[t])
+        assert T.rgt_cars_upload(h, 4, ctl.ctypes.data, ctl.shape[0]) == 0
+        ref.step_with_controls(ctl, 1, ctl.shape[0])
+        assert T.rgt_cars_step(h, 1) == 0
+        mine, uns = _twin_records(T, h, n_env, cars)
+        flagged_at[(uns != 0) & (flagged_at < 0)] = t
+        bad = arena_mismatches(mine, reference_records(R, ref, n_env, cars), n_env, cars)
+        live = uns == 0
+        assert not live[bad].any(), f"tick {t}: arenas {bad[live[bad]]} differ from the reference while unflagged"
+        compared += int(live.sum())
+    T.rgt_cars_destroy(h)
+    ref.close()
+    return compared, flagged_at
+
+
+def test_twin_cars_1v1_octane_match_reference():
+    n_env, ticks = 12, 400
+    compared, flagged_at = _run(n_env, 2, ticks, seed=7)
+    assert compared >= 0.4 * n_env * ticks, f"only {compared} of {n_env * ticks} arena-ticks compared"
+
+
+@pytest.mark.parametrize("cars", [4, 6])
+def test_twin_cars_mixed_bodies_match_reference(cars):
+    n_env, ticks = 6, 200
+    compared, _ = _run(n_env, cars, ticks, seed=11 + cars, presets=np.arange(n_env * cars) % 6)
+    assert compared >= 0.2 * n_env * ticks, f"only {compared} of {n_env * ticks} arena-ticks compared"
