@@ -166,6 +166,13 @@ int rg_batch_step_with_device_commands(rg_batch* batch, const void* device_contr
 
 /* step(ticks, stream)  (rocketgpu.h:1068). */
 int rg_batch_step(rg_batch* batch, int32_t ticks, void* stream);
+/* step(ticks, stream) of a ball-only batch through the split schedule (launch stages 6 and 7 below) with the
+ * schedule's arena lists in a fixed device array of the library instead of batch-owned memory: it allocates and
+ * frees nothing.  No counterpart in the reference; it is for callers that cannot allocate device memory (code under
+ * graph capture, tight serving loops).  At most 262,144 arenas (RG_EINVAL beyond, and for batches with cars).  The
+ * array (2 MiB per GPU, held until the process ends) is shared by every call on one GPU: calls that overlap on one
+ * GPU race. */
+int rg_batch_step_split_no_alloc(rg_batch* batch, int32_t ticks, void* stream);
 /* step_with_device_controls(device_controls_by_tick, ticks, control_stride, stream)  (rocketgpu.h:1075). */
 int rg_batch_step_with_device_controls(rg_batch* batch, const void* device_controls_by_tick, int32_t ticks,
                                        int32_t control_stride, void* stream);

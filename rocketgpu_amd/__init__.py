@@ -132,6 +132,7 @@ def lib():
         L.rg_batch_upload_mode.argtypes = [vp, ctypes.c_int, vp, ctypes.c_size_t, ctypes.c_int32, vp]
         L.rg_batch_download.argtypes = [vp, ctypes.c_int, vp, ctypes.c_size_t, vp]
         L.rg_batch_step.argtypes = [vp, ctypes.c_int32, vp]
+        L.rg_batch_step_split_no_alloc.argtypes = [vp, ctypes.c_int32, vp]
         L.rg_batch_step_with_device_controls.argtypes = [vp, vp, ctypes.c_int32, ctypes.c_int32, vp]
         L.rg_batch_synchronize.argtypes = [vp, vp]
         L.rg_batch_device_view.argtypes = [vp, ctypes.POINTER(_DeviceView)]
@@ -355,6 +356,11 @@ class BatchSim:
         _check(lib().rg_batch_step_arenas(self._h, ctypes.c_void_p(device_ids_ptr),
                                           ctypes.c_void_p(device_ticks_ptr or None), count, ticks,
                                           _stream_ptr(stream)))
+
+    def step_split_no_alloc(self, ticks: int = 1, stream=None):
+        """step(ticks) through the split schedule with its arena lists in the library's fixed device array: allocates
+        no device memory (ball-only batches of at most 262,144 arenas; overlapping calls on one GPU race)."""
+        _check(lib().rg_batch_step_split_no_alloc(self._h, ticks, _stream_ptr(stream)))
 
     def step_graph(self, ticks_per_launch: int, launches: int = 1, stream=None):
         """`launches` x step(ticks_per_launch), captured once into a hipGraph and replayed."""

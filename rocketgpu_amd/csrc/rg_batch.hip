@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <stdexcept>
 #include <string>
@@ -411,6 +412,31 @@ __global__ void __launch_bounds__(256) PartitionArenasKernel(const core::ArenaIn
     lists[cap + atomicAdd(&counts[1], 1)] = e;
 }
 
+// The split schedule's lists for rg_batch_step_split_no_alloc, in a fixed array of the library instead of
+// batch-owned memory: words [0, 2) are the two list counts, [64, 64 + 2 kNoAllocMaxArenas) the heavy then light
+// arena ids.  Each GPU holds its own copy (2 MiB) until the process ends; calls on one GPU share it.
+constexpr int32_t kNoAllocMaxArenas = 262144;
+constexpr int32_t kNoAllocListOffset = 64;
+static __device__ __align__(256) int32_t scratch[kNoAllocListOffset + 2 * kNoAllocMaxArenas];
+
+// Device address of `scratch` on `device`, looked up once per device with that device current (the host-side name
+// of a __device__ array is a stand-in, not a device address).
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+int32_t* g_scratch_ptr[kMaxDevices] = {};
+
+int32_t* NoAllocScratch(int device) {
+    if (device < 0 || device >= kMaxDevices) throw std::runtime_error("device ordinal outside 0..63");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (g_scratch_ptr[device] == nullptr) {
+        void* p = nullptr;
+        Check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+        if (p == nullptr) throw std::runtime_error("hipGetSymbolAddress(scratch) returned null");
+        g_scratch_ptr[device] = static_cast<int32_t*>(p);
+    }
+    return g_scratch_ptr[device];
+}
+
 __device__ inline uint64_t Fnv(uint64_t h, const void* data, int n) {
     const unsigned char* p = static_cast<const unsigned char*>(data);
     for (int i = 0; i < n; ++i) {
@@ -491,6 +517,7 @@ public:
         if (!(cfg.tick_dt > 0.0f)) throw InvalidArgument("tick_dt must be positive");
         int device = 0;
         Check(hipGetDevice(&device), "hipGetDevice");
+        no_alloc_scratch_ = NoAllocScratch(device);
         mesh_host_ = BuildArenaMesh(meshes);
         tris_.Resize(mesh_host_.tris.size());
         nodes_.Resize(mesh_host_.nodes.size());
@@ -933,6 +960,32 @@ public:
         Launch(nullptr, nullptr, cfg_.env_count, ticks, s);
     }
 
+    // Step(ticks) through the split schedule (heavy arenas by their last contact walk on `gh` lanes, light ones on 8,
+    // workgroups of 64 lanes) with the lists in the library's fixed `scratch` array: no device memory is allocated
+    // or freed.  Ball-only batches (the car kernel keeps its per-tick state lane-private and has no lists: Step
+    // allocates nothing for car batches).  The counts are cleared by a kernel at the start of every call; calls
+    // that overlap on one GPU race on the lists.
+    void StepSplitNoAlloc(int32_t ticks, hipStream_t s) {
+        if (ticks < 0) throw InvalidArgument("ticks must be non-negative");
+        if (cfg_.env_count > kNoAllocMaxArenas) {
+            throw InvalidArgument("step_split_no_alloc supports at most " + std::to_string(kNoAllocMaxArenas) +
+                                  " arenas; this batch has " + std::to_string(cfg_.env_count));
+        }
+        if (has_cars_) throw InvalidArgument("step_split_no_alloc steps ball-only batches; use step for car arenas");
+        if (ticks == 0) return;
+        StepArgs a = BaseArgs();
+        a.count = cfg_.env_count;
+        a.arenas_per_wave = 8;
+        const int gh = stage_ == kStageSplit16 ? 16 : 32;
+        int32_t* const counts = no_alloc_scratch_;
+        int32_t* const lists = no_alloc_scratch_ + kNoAllocListOffset;
+        if (a.event_windows) {
+            LaunchSplit<true>(a, ticks, lists, counts, gh, 64, s);
+        } else {
+            LaunchSplit<false>(a, ticks, lists, counts, gh, 64, s);
+        }
+    }
+
     // Steps a list of distinct arenas (device ids), each for its own tick count (device array, or `ticks` for all
     // when null).  The serving loop's launch.
     void StepArenas(const int32_t* device_ids, const int32_t* device_ticks, int32_t count, int32_t ticks,
@@ -1297,25 +1350,27 @@ private:
 
     // Stages 6 and 7 over the whole batch: clear the list counts, sort the arenas by their last contact walk, and
     // step both lists in one launch (the heavy arenas' workgroups first, so they start first).
+    // `lists` holds 2 env_count ids and `counts` 2 words; `gh` lanes per heavy arena (32 or 16), `block` lanes per
+    // workgroup (64, 128 or 256).
     template <bool kEvents>
-    void LaunchSplit(const StepArgs& a, int32_t ticks, hipStream_t s) {
+    void LaunchSplit(const StepArgs& a, int32_t ticks, int32_t* lists, int32_t* counts, int gh, int block,
+                     hipStream_t s) {
         const int n = cfg_.env_count;
-        // The heavy list holds at most env_count arenas (cap_divisor >= 1): the lists fit the 2 env_count allocation.
+        // The heavy list holds at most env_count arenas (cap_divisor >= 1): the lists fit 2 env_count words.
         const int cap = n / split_cap_divisor_ > 1 ? n / split_cap_divisor_ : 1;
-        hipLaunchKernelGGL(ClearSplitCountsKernel, dim3(1), dim3(64), 0, s, split_counts_.data());
+        hipLaunchKernelGGL(ClearSplitCountsKernel, dim3(1), dim3(64), 0, s, counts);
         hipLaunchKernelGGL(PartitionArenasKernel, dim3((n + 255) / 256), dim3(256), 0, s, internal_.data(), n,
-                           split_rounds_, cap, split_lists_.data(), split_counts_.data());
-        const int gh = stage_ == kStageSplit32 ? 32 : 16;
-        const int heavy_blocks = (cap * gh + block_ - 1) / block_;
-        const int light_blocks = static_cast<int>((static_cast<int64_t>(n) * 8 + block_ - 1) / block_);
-        const size_t lds = sizeof(core::BallTickScratch) * static_cast<size_t>(block_ / 8);
-        const dim3 grid(static_cast<unsigned>(heavy_blocks + light_blocks)), block(block_);
+                           split_rounds_, cap, lists, counts);
+        const int heavy_blocks = (cap * gh + block - 1) / block;
+        const int light_blocks = static_cast<int>((static_cast<int64_t>(n) * 8 + block - 1) / block);
+        const size_t lds = sizeof(core::BallTickScratch) * static_cast<size_t>(block / 8);
+        const dim3 grid(static_cast<unsigned>(heavy_blocks + light_blocks)), dims(block);
         if (gh == 32) {
-            hipLaunchKernelGGL((StepBallArenasSplitKernel<32, kEvents>), grid, block, lds, s, a, ticks,
-                               split_lists_.data(), split_counts_.data(), cap, heavy_blocks);
+            hipLaunchKernelGGL((StepBallArenasSplitKernel<32, kEvents>), grid, dims, lds, s, a, ticks, lists, counts,
+                               cap, heavy_blocks);
         } else {
-            hipLaunchKernelGGL((StepBallArenasSplitKernel<16, kEvents>), grid, block, lds, s, a, ticks,
-                               split_lists_.data(), split_counts_.data(), cap, heavy_blocks);
+            hipLaunchKernelGGL((StepBallArenasSplitKernel<16, kEvents>), grid, dims, lds, s, a, ticks, lists, counts,
+                               cap, heavy_blocks);
         }
         Check(hipGetLastError(), "StepBallArenasSplitKernel launch");
     }
@@ -1323,7 +1378,8 @@ private:
     template <bool kEvents>
     void LaunchStep(const StepArgs& a, int32_t count, int32_t ticks, hipStream_t s) {
         if (IsSplitStage(stage_) && a.arena_ids == nullptr && a.arena_ticks == nullptr) {
-            LaunchSplit<kEvents>(a, ticks, s);
+            LaunchSplit<kEvents>(a, ticks, split_lists_.data(), split_counts_.data(),
+                                 stage_ == kStageSplit32 ? 32 : 16, block_, s);
             return;
         }
         if (IsGroupStage(stage_)) {
@@ -1409,6 +1465,7 @@ private:
     DeviceArray<uint64_t> digests_;
     DeviceArray<int32_t> split_lists_;   // stages 6 and 7: heavy then light arena ids
     DeviceArray<int32_t> split_counts_;
+    int32_t* no_alloc_scratch_ = nullptr;  // device address of `scratch` on this batch's device
     int split_rounds_ = kSplitRoundsDefault;
     int split_cap_divisor_ = kSplitCapDivisorDefault;
     int block_ = 64;
@@ -1561,6 +1618,10 @@ int rg_batch_download(rg_batch* b, int id, void* dst, size_t count, void* stream
 
 int rg_batch_step(rg_batch* b, int32_t ticks, void* stream) {
     return Guard([&] { b->sim->Step(ticks, S(stream)); });
+}
+
+int rg_batch_step_split_no_alloc(rg_batch* b, int32_t ticks, void* stream) {
+    return Guard([&] { b->sim->StepSplitNoAlloc(ticks, S(stream)); });
 }
 
 int rg_batch_step_with_device_controls(rg_batch* b, const void* controls, int32_t ticks, int32_t stride,
