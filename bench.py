@@ -4,22 +4,24 @@ A step is one call of step_with_device_controls(controls[T, car_capacity], T tic
 with the state resident in HBM.  Timing: HIP events recorded on the stream the step kernel is launched on, around
 each step; the timed region (K steps) is bracketed by device synchronization.
 
-The workload is 16,384 ball arenas on the procedural arena PA(1, 256) (20,332 triangles) with seeded ball launches
-and T = 8 ticks per step.  Cars are not simulated by this build yet, so the arenas carry no cars; config.workload
-says so.  The roofline object prices the step kernel's algorithmic bytes per arena-tick against the 8 TB/s HBM
-peak: 2 x (BallState 288 + EnvState 44 + 34 pads x 16 + world solver state 11,712) + 4 = 25,180 B at 0 cars.
-cpu_baseline times the reference CPU build (oracle/_ref) on this machine's host cores over the same tick window of
-the same workload.
+Headline workload (--workload mixed, the default): 16,384 arenas on the procedural arena PA(1, 256) (20,332
+triangles), 55% 1v1 / 27% 2v2 / 18% 3v3 (drawn per arena from the seed), every car's body drawn uniformly from the
+six stock hitboxes, teams split evenly, a new-match kickoff (reset_to_kickoff with settled rows) and seeded held
+controls ([64 ticks, car_capacity] on the device, step k reading rows 8k mod 64 onwards), T = 8 ticks per step.
+Arenas that meet a body contact this build does not compute yet (car-mesh, ball-car, car-car) carry a nonzero
+unsupported word from then on; the line reports their share at the end of the run.
 
-Beside the headline:
-- the launch shape, which the tuner picks over lanes per workgroup, where the arena state and the per-tick contact
-  scratch live (HBM in place, lane-private memory, LDS), and arenas per wavefront (64 down to 4: more waves per CU,
-  less divergence);
-- the other step forms, each on a fresh batch over the same window: T launches of one tick, and hipGraph replays;
-- the serving loop (simulated clients, Poisson arrivals, continuous batching over 262,144 resident arenas), with
-  p50/p99 request latency.
+The roofline object prices the step kernel at the reference's algorithmic bytes per arena-tick
+B(n) = 2 (12588 + 2456 n + 130 n (n - 1)) + 144 n + 4 for n cars (44,195 B at the 55/27/18 mix) against the 8 TB/s
+HBM peak; `traffic` is the measured HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by a plain
+copy's counters) when a traffic file for this workload exists, and `limiter` says what the counters show bounds
+the kernel.  cpu_baseline times the reference CPU build (oracle/_ref) on this host's cores over the same arenas and
+tick window: a warm-up window, then the median of three consecutive windows.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--arenas N] [--ticks T] [--block B --stage S --per-wave A]
+--workload ball keeps the ball-only workload (16,384 arenas, no cars) with its launch-shape tuner, step forms and
+serving loop.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--arenas N] [--ticks T] [--workload mixed|ball]
                     [--no-forms] [--no-serve] [--cpu-ticks C] [--calibrate-copy-mib M]
 """
 from __future__ import annotations
@@ -54,7 +56,7 @@ def bytes_per_arena_tick(n_cars: float) -> float:
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=100)
+    p.add_argument("--steps", type=int, default=20)
     p.add_argument("--warmup", type=int, default=5)
     p.add_argument("--arenas", type=int, default=16384)
     p.add_argument("--ticks", type=int, default=8, help="ticks per step (T of step_with_device_controls)")
@@ -74,7 +76,8 @@ def parse():
     p.add_argument("--serve-arenas", type=int, default=262144)
     p.add_argument("--cpu-ticks", type=int, default=-1,
                    help="ticks of the reference CPU sample (-1: the GPU's timed window; 0: skip)")
-    p.add_argument("--traffic", default=os.path.join(REPO, "profiles", "step_ball_arenas_16384_traffic.json"))
+    p.add_argument("--traffic", default="")
+    p.add_argument("--workload", choices=("mixed", "ball"), default="mixed")
     p.add_argument("--calibrate-copy-mib", type=int, default=0,
                    help="first run one plain device-to-device copy of this many MiB (a known-size reference for "
                         "the FETCH_SIZE / WRITE_SIZE counters)")
@@ -101,11 +104,21 @@ def make_batch(rga, np, n, max_cars, seed):
     return sim, init_ball, init_env
 
 
+def host_cores():
+    """Threads for the reference CPU build: the cores this process may run on, within the host's CPU share when the
+    environment states one (OMP_NUM_THREADS)."""
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    share = os.environ.get("OMP_NUM_THREADS", "")
+    if share.isdigit() and int(share) > 0:
+        cores = min(cores, int(share))
+    return cores
+
+
 def cpu_baseline(args, init_ball, init_env, start_tick, ticks):
     """The reference CPU build on this host: the same arenas over the same tick window as the GPU's timed steps."""
     if ticks <= 0:
         return None
-    cores = min(16, os.cpu_count() or 1)
+    cores = host_cores()
     os.environ["ROCKETGPU_CPU_THREADS"] = str(cores)
     import numpy as np
     from oracle import refsim as R
@@ -157,6 +170,192 @@ def split_sweep(args, rga, np, torch, stream, n, T):
                       "warmup": args.warmup}))
 
 
+MIX = (0.55, 0.27, 0.18)  # 1v1 / 2v2 / 3v3
+CTRL_TICKS = 64           # rows of the seeded control tensor on the device
+
+
+def mixed_workload(np, n, seed):
+    """Per arena car count (2, 4 or 6 at the 55/27/18 mix), per car slot body (0-5: Octane, Dominus, Plank,
+    Breakout, Hybrid, Merc, uniform) and team (first half of an arena's cars Blue, the rest Orange)."""
+    rng = np.random.default_rng(seed)
+    u = rng.random(n)
+    counts = np.where(u < MIX[0], 2, np.where(u < MIX[0] + MIX[1], 4, 6)).astype(np.int32)
+    presets = rng.integers(0, 6, n * 6).astype(np.uint8)
+    slot = np.arange(n * 6) % 6
+    teams = (slot >= np.repeat(counts, 6) // 2).astype(np.uint8)
+    return counts, presets, teams
+
+
+def make_car_batch(rga, np, n, seed, workload):
+    counts, presets, teams = workload
+    table = np.stack([rga.car_preset_config(i) for i in range(6)])
+    sim = rga.BatchSim(n, 6)
+    sim.set_env_car_count(counts)
+    sim.set_teams(teams)
+    sim.set_car_configs(table[presets])
+    sim.reset_to_kickoff(seed)
+    return sim
+
+
+def cpu_baseline_cars(args, workload, controls, first_step, steps, T):
+    """The reference CPU build on this host over the same arenas, kickoff and controls as the GPU: steps up to
+    `first_step` untimed, then three consecutive windows of ceil(steps / 3) steps of T ticks; the median rate."""
+    if steps <= 0:
+        return None
+    cores = host_cores()
+    os.environ["ROCKETGPU_CPU_THREADS"] = str(cores)
+    import numpy as np
+    from oracle import refsim as R
+
+    if not R.available():
+        return {"value": None, "unit": "arena-ticks/s", "cores": cores, "kind": "reference",
+                "sample": "reference CPU build (oracle/_ref) not present on this machine"}
+    counts, presets, teams = workload
+    n = counts.size
+    ref = R.RefSim(n, 6, respawn_seed=args.seed)
+    ref.set_env_car_count(counts)
+    ref.set_teams(teams)
+    ref.set_car_presets(presets)
+    ref.reset_to_kickoff(args.seed)
+    rows = controls.reshape(CTRL_TICKS, -1)
+
+    def run(k0, k1):
+        for k in range(k0, k1):
+            r = (k * T) % CTRL_TICKS
+            ref.step_with_controls(rows[r:r + T], T, n * 6)
+
+    run(0, first_step)
+    per = max(1, -(-steps // 3))
+    rates = []
+    for w in range(3):
+        k0 = first_step + w * per
+        t0 = time.perf_counter()
+        run(k0, k0 + per)
+        rates.append(n * per * T / (time.perf_counter() - t0))
+    ref.close()
+    return {"value": float(np.median(rates)), "unit": "arena-ticks/s", "cores": cores, "kind": "reference",
+            "host_cpus": os.cpu_count(),
+            "sample": f"{n} mixed arenas (same kickoff, bodies and controls as the GPU run), warm-up to tick "
+                      f"{first_step * T}, then 3 windows of {per * T} ticks; median of "
+                      f"{', '.join(f'{r / 1e6:.3f}' for r in rates)} M arena-ticks/s; ROCKETGPU_CPU_THREADS={cores}; "
+                      "the reference computes every body contact, this build flags the arenas that meet one"}
+
+
+def bench_cars(args, rga, np, torch, stream):
+    n, T = args.arenas, args.ticks
+    if CTRL_TICKS % T:
+        raise SystemExit(f"--ticks must divide {CTRL_TICKS}")
+    workload = mixed_workload(np, n, args.seed)
+    counts = workload[0]
+    controls = rga.workload_controls(args.seed, 0, n, 6, CTRL_TICKS)
+    ctrl = torch.from_numpy(controls.reshape(CTRL_TICKS, -1)).cuda()
+    row_bytes = ctrl.shape[1]
+
+    def ctrl_ptr(k):
+        return ctrl.data_ptr() + ((k * T) % CTRL_TICKS) * row_bytes
+
+    # Arenas per wavefront of the car kernel (one lane per arena; fewer arenas per wave, more waves per CU): the
+    # tuner times each on a fresh batch over the warm-up window plus five steps and keeps the fastest.
+    sweep = {}
+    per_wave = args.per_wave if args.block else None
+    if per_wave is None:
+        for cand in (64, 32, 16, 8, 4):
+            tsim = make_car_batch(rga, np, n, args.seed, workload)
+            tsim.set_launch_shape(64, 0, cand)
+            for k in range(args.warmup):
+                tsim.step_with_device_controls(ctrl_ptr(k), T, tsim.car_capacity, stream=stream)
+            sweep[f"perwave{cand}"] = timed(torch, stream, lambda: tsim.step_with_device_controls(
+                ctrl_ptr(args.warmup), T, tsim.car_capacity, stream=stream), 5)
+            tsim.close()
+            if per_wave is None or sweep[f"perwave{cand}"] < sweep[f"perwave{per_wave}"]:
+                per_wave = cand
+
+    sim = make_car_batch(rga, np, n, args.seed, workload)
+    sim.set_launch_shape(64, 0, per_wave)
+    cap = sim.car_capacity
+    torch.cuda.synchronize()
+    for k in range(args.warmup):
+        sim.step_with_device_controls(ctrl_ptr(k), T, cap, stream=stream)
+    stream.synchronize()
+    torch.cuda.synchronize()
+    events = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+              for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    for k, (a, b) in enumerate(events):
+        a.record(stream)
+        sim.step_with_device_controls(ctrl_ptr(args.warmup + k), T, cap, stream=stream)
+        b.record(stream)
+    stream.synchronize()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kernel_ms = [a.elapsed_time(b) for a, b in events]
+    avg_kernel_s = sum(kernel_ms) / len(kernel_ms) / 1e3
+    envs = sim.download_state_array("env_states")
+    ticks_done = envs[:, 0:4].copy().view(np.uint32)[:, 0]
+    assert np.all(ticks_done == (args.warmup + args.steps) * T), "tick counters do not match the steps run"
+    unsupported = sim.download_state_array("unsupported").view(np.uint32).ravel()
+    sim.close()
+
+    arena_ticks = n * T * args.steps
+    value = arena_ticks / wall
+    b_arena = float(np.mean([bytes_per_arena_tick(c) for c in counts]))
+    achieved = n * T * b_arena / avg_kernel_s / 1e9
+    traffic = None
+    traffic_file = args.traffic or os.path.join(REPO, "profiles", "step_car_arenas_16384_traffic.json")
+    if os.path.exists(traffic_file):
+        with open(traffic_file) as fh:
+            tr = json.load(fh)
+        if tr.get("arenas") == n and tr.get("ticks") == T and tr.get("seed") == args.seed:
+            traffic = tr.get("hbm_bytes_per_launch")
+
+    forms = None
+    if not args.no_forms:
+        # Each form on a fresh batch from the same kickoff over the same tick window, with the batch's own control
+        # row (zero after the kickoff) instead of the control tensor: T single-tick launches, one T-tick launch,
+        # and hipGraph replays of both.
+        def form(run):
+            fsim = make_car_batch(rga, np, n, args.seed, workload)
+            fsim.set_launch_shape(64, 0, per_wave)
+            for _ in range(args.warmup):
+                run(fsim)
+            ms = timed(torch, stream, lambda: run(fsim), args.steps)
+            fsim.close()
+            return ms
+
+        forms = {"T": T, "window_ticks": [args.warmup * T, (args.warmup + args.steps) * T], "ms_per_T_ticks": {
+            "plain_T_launches_of_1_tick": form(lambda s: [s.step(1, stream=stream) for _ in range(T)]),
+            "one_launch_of_T_ticks": form(lambda s: s.step(T, stream=stream)),
+            "hipgraph_T_launches_of_1_tick": form(lambda s: s.step_graph(1, T, stream=stream)),
+            "hipgraph_one_launch_of_T_ticks": form(lambda s: s.step_graph(T, 1, stream=stream))}}
+
+    cpu_steps = args.steps if args.cpu_ticks < 0 else -(-args.cpu_ticks // T)
+    cpu = cpu_baseline_cars(args, workload, controls, args.warmup, cpu_steps, T)
+    mix = {str(c): round(float((counts == c).mean()), 4) for c in (2, 4, 6)}
+    line = {
+        "metric": "arena-ticks/s", "value": value, "unit": "arena-ticks/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded car counts, bodies and held controls; kickoff start; procedural arena)",
+        "config": {"workload": f"{n} mixed arenas (1v1/2v2/3v3 at {MIX[0]:.0%}/{MIX[1]:.0%}/{MIX[2]:.0%}, six "
+                               f"stock bodies uniform per car), procedural arena PA(1, 256) 20,332 triangles, "
+                               f"step_with_device_controls T={T}",
+                   "arenas": n, "ticks_per_step": T, "cars_per_arena_mix": mix,
+                   "cars": int(counts.sum()), "parallelism": "single GPU",
+                   "kernel_launch": f"one lane per arena, {per_wave} arenas per 64-lane wavefront",
+                   "arenas_per_wave_sweep_ms": sweep or None},
+        "arenas_unsupported_frac_at_end": float((unsupported != 0).mean()),
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "limiter": "latency (one lane per arena walks its wheel rays, broadphase and solver serially; "
+                                "dependent loads at low occupancy)",
+                     "kernel": "StepCarArenasKernel", "kernel_avg_ms": avg_kernel_s * 1e3,
+                     "bytes_per_arena_tick": b_arena},
+        "forms": forms,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -178,9 +377,19 @@ def main():
         torch.cuda.synchronize()
         del src, dst
     stream = torch.cuda.Stream()
+    if args.workload == "mixed":
+        bench_cars(args, rga, np, torch, stream)
+        return
     if args.split_sweep:
         split_sweep(args, rga, np, torch, stream, n, T)
         return
+    bench_balls(args, rga, np, torch, stream)
+
+
+def bench_balls(args, rga, np, torch, stream):
+    """The ball-only workload: 16,384 arenas without cars, seeded ball launches."""
+    n, T = args.arenas, args.ticks
+    traffic_file = args.traffic or os.path.join(REPO, "profiles", "step_ball_arenas_16384_traffic.json")
     g = torch.Generator(device="cuda")
     g.manual_seed(args.seed)
 
@@ -236,8 +445,8 @@ def main():
     per_launch_bytes = n * T * bytes_per_arena_tick(0)
     achieved = per_launch_bytes / avg_kernel_s / 1e9
     traffic = None
-    if os.path.exists(args.traffic):
-        with open(args.traffic) as fh:
+    if os.path.exists(traffic_file):
+        with open(traffic_file) as fh:
             tr = json.load(fh)
         # Measured per kernel form: the staging and lanes per arena decide the traffic, the block size does not.
         for entry in tr.get("entries", []):
@@ -294,6 +503,8 @@ def main():
                    "launch_shape": {"block": shape[0], "stage": shape[1], "arenas_per_wave": shape[2]}, "launch_shape_sweep_ms": sweep or None},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "limiter": "latency (dependent BVH and state loads at low occupancy; measured traffic is a "
+                                "small fraction of the algorithmic bytes)",
                      "kernel": kernel_name(shape), "kernel_avg_ms": avg_kernel_s * 1e3,
                      "bytes_per_arena_tick": bytes_per_arena_tick(0)},
         "forms": forms,

@@ -217,6 +217,10 @@ int rg_batch_device_view(rg_batch* batch, rg_device_view* out);
  * counterpart: it replaces downloading every state to hash it (tools/bench_cpp.cpp:505-525) for parity checks. */
 int rg_batch_env_digests(rg_batch* batch, uint64_t* device_out, void* stream);
 
+/* The CarConfig of a stock body: 0 Octane, 1 Dominus, 2 Plank, 3 Breakout, 4 Hybrid, 5 Merc, 6 Psyclops,
+ * 7 Batmobile (OctaneConfig() ... BatmobileConfig(), rocketgpu.h:1098-1108); out receives one CarConfig. */
+int rg_car_preset_config(int32_t preset, void* out_car_config);
+
 /* FNV-1a 64 over the bytes of n per-env digests (host memory): the batch digest of one tick.  Parity utility
  * with no reference counterpart. */
 uint64_t rg_fold_digests(const uint64_t* host_env_digests, int64_t n);

@@ -147,6 +147,7 @@ def lib():
         L.rg_batch_set_split.argtypes = [vp, ctypes.c_int32, ctypes.c_int32]
         L.rg_serve_run.argtypes = [vp, ctypes.POINTER(ServeConfig), ctypes.POINTER(ServeReport), vp]
         L.rg_fold_digests.argtypes = [vp, ctypes.c_int64]
+        L.rg_car_preset_config.argtypes = [ctypes.c_int32, vp]
         L.rg_fold_digests.restype = ctypes.c_uint64
         L.rg_batch_boost_pad_capacity.argtypes = [vp]
         L.rg_batch_boost_pad_capacity.restype = ctypes.c_int64
@@ -196,6 +197,17 @@ def workload_controls(seed: int, first_arena: int, arena_count: int, cars_per_ar
     out = np.zeros((ticks, arena_count * cars_per_arena, ITEMSIZE["controls"]), dtype=np.uint8)
     _check(lib().rg_workload_controls(seed, first_arena, arena_count, cars_per_arena, ticks, p_boost,
                                       out.ctypes.data))
+    return out
+
+
+CAR_PRESETS = ("octane", "dominus", "plank", "breakout", "hybrid", "merc", "psyclops", "batmobile")
+
+
+def car_preset_config(preset) -> np.ndarray:
+    """The CarConfig record (120 bytes) of a stock body, by index or name (OctaneConfig() ... BatmobileConfig())."""
+    i = CAR_PRESETS.index(preset.lower()) if isinstance(preset, str) else int(preset)
+    out = np.zeros(ITEMSIZE["car_configs"], dtype=np.uint8)
+    _check(lib().rg_car_preset_config(i, out.ctypes.data))
     return out
 
 

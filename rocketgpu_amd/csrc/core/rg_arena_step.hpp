@@ -53,14 +53,6 @@ struct CarArenaRefs {
 // ------------------------------------------------------------------------------------------------------------
 // Visit order (rocketgpu.h:575-626).
 
-RG_HD int VisitSlot(const EnvState& env, int k, int count) {
-    if (env.car_visit_order_mode == 1 || env.car_visit_order_mode == 3) {
-        return static_cast<int>((env.car_visit_order_packed >> (4 * k)) & 0xFu);
-    }
-    if (env.car_visit_order_mode == 2) return count - 1 - k;
-    return k;
-}
-
 // Mode 3 keeps the game's actor list: a car that comes back goes after the last present car, a car demolished this
 // tick swaps places with the last present car.
 RG_HD void UpdateVisitOrder(EnvState& env, const CarState* cars, int count) {
@@ -735,8 +727,12 @@ RG_HD void StepCarArena(CarArenaRefs& a, const ArenaMeshView& mesh, const StepPa
         }
         for (int i = 0; i < count; ++i) {
             if (((demoed >> i) & 1u) == 0u) continue;
+            // A due car with a chosen spot (a kickoff re-creation or an override) keeps it; one without draws a
+            // free respawn spot here (PickRespawnSpotsForEnv, env_step.hpp:6615-6650), not built.
             const float left = Max(0.0f, a.cars[i].demo_respawn_timer - dt);
-            if (!(Max(0.0f, left - dt) > 0.0f)) unsupported |= kUnsupportedRespawnSpot;
+            if (!(Max(0.0f, left - dt) > 0.0f) && a.cars[i].respawn_slot_override_plus1 == 0) {
+                unsupported |= kUnsupportedRespawnSpot;
+            }
         }
     }
     env.goal_scored_this_tick = 0;

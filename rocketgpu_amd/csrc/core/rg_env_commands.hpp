@@ -57,8 +57,37 @@ RG_HD uint8_t DrawRespawnOrder(uint64_t seed, int env_idx) {
     return static_cast<uint8_t>(packed);
 }
 
-// EnvState of a kickoff reset (ResetEnvToKickoffState with no cars, no respawn generator and no order override).
-RG_HD void ResetKickoffEnvState(EnvState& env, int env_idx, uint64_t seed, bool preserve_score, uint8_t seed_mode) {
+// The car slot visited k-th of `count` (rocketgpu.h:575-626): slot order (mode 0), reversed (2), or the packed
+// 4-bit order (1, and 3, the game's actor list).
+RG_HD int VisitSlot(const EnvState& env, int k, int count) {
+    if (env.car_visit_order_mode == 1 || env.car_visit_order_mode == 3) {
+        return static_cast<int>((env.car_visit_order_packed >> (4 * k)) & 0xFu);
+    }
+    if (env.car_visit_order_mode == 2) return count - 1 - k;
+    return k;
+}
+
+// The spot order a kickoff reset hands out per team: spot of the team's n-th car = (bits[team] >> 3n) & 7.
+struct KickoffPlan {
+    uint32_t order_bits[2];
+};
+
+// Of two kickoff spots (x, y), the one a short team's last car takes (KickoffSpotPreferred, arena_ball.hpp:2632):
+// nearer the centre; then the one with x y < 0; then the larger |y|.
+RG_HD bool KickoffSpotNearer(float ax, float ay, float bx, float by) {
+    const float da = ax * ax + ay * ay, db = bx * bx + by * by;
+    if (da != db) return da < db;
+    const bool la = ax * ay < 0.0f, lb = bx * by < 0.0f;
+    if (la != lb) return la;
+    return Abs(ay) > Abs(by);
+}
+
+// EnvState of a kickoff reset (ResetEnvToKickoffState with no respawn generator and no order override,
+// arena_ball.hpp:2845-2925) and the spots the env's cars take.  `teams` (0 Blue, 1 Orange) holds the env's `count`
+// cars by slot and `spots` the Blue kickoff spots' ground positions; both may be null when count is 0.
+RG_HD KickoffPlan ResetKickoffEnvState(EnvState& env, int env_idx, uint64_t seed, bool preserve_score,
+                                       uint8_t seed_mode, const uint8_t* teams = nullptr, int count = 0,
+                                       const Vec3* spots = nullptr) {
     const EnvState saved = env;
     EnvState next{};
     next.kickoff_resets = static_cast<uint8_t>(saved.kickoff_resets + 1);
@@ -82,11 +111,15 @@ RG_HD void ResetKickoffEnvState(EnvState& env, int env_idx, uint64_t seed, bool 
         next.tick = saved.tick;
         next.last_scored_team = saved.last_scored_team;
     }
+    // Spot order: identity for mode 3, the stored match order's next rotation for a follow-up kickoff, else drawn.
+    int order[5] = {0, 1, 2, 3, 4};
     if (seed_mode != 3) {
         if (next_kickoff) {
+            for (int j = 0; j < 5; ++j) {
+                order[j] = (saved.kickoff_match_order >> (3 * ((saved.kickoff_match_index + j) % 5))) & 7;
+            }
             next.kickoff_match_index = static_cast<uint8_t>((saved.kickoff_match_index + 1) % 5);
         } else {
-            int order[5];
             DrawKickoffOrder(seed, env_idx, order);
             uint32_t packed = 0x8000u;
             for (int j = 0; j < 5; ++j) packed |= static_cast<uint32_t>(order[j]) << (3 * j);
@@ -95,7 +128,36 @@ RG_HD void ResetKickoffEnvState(EnvState& env, int env_idx, uint64_t seed, bool 
             next.respawn_match_order = DrawRespawnOrder(seed, env_idx);
         }
     }
+    int team_order[2][5];
+    for (int j = 0; j < 5; ++j) team_order[0][j] = team_order[1][j] = order[j];
+    int blues = 0;
+    for (int k = 0; k < count; ++k) blues += teams[VisitSlot(next, k, count)] == 0 ? 1 : 0;
+    const int oranges = count - blues;
+    // A follow-up kickoff with uneven teams moves the short team's last car to the best of its first four spots
+    // (PreferKickoffSpotForShortTeam, arena_ball.hpp:2642-2653).
+    if (next_kickoff && blues != oranges) {
+        const int t = blues < oranges ? 0 : 1;
+        const int n = t == 0 ? blues : oranges;
+        if (n > 0 && n < 4) {
+            int* o = team_order[t];
+            int best = 0;
+            for (int i = 1; i < 4; ++i) {
+                if (KickoffSpotNearer(spots[o[i]].x, spots[o[i]].y, spots[o[best]].x, spots[o[best]].y)) best = i;
+            }
+            if (best >= n) {
+                const int keep = o[best];
+                o[best] = o[n - 1];
+                o[n - 1] = keep;
+            }
+        }
+    }
     env = next;
+    KickoffPlan plan;
+    for (int t = 0; t < 2; ++t) {
+        plan.order_bits[t] = 0;
+        for (int j = 0; j < 5; ++j) plan.order_bits[t] |= static_cast<uint32_t>(team_order[t][j]) << (3 * j);
+    }
+    return plan;
 }
 
 // The ball of a kickoff on an arena with rules (ResetBall with ArenaRules, arena_ball.hpp:2775-2810): at the spawn,

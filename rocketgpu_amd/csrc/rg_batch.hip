@@ -18,7 +18,9 @@
 #include <vector>
 
 #include "batch_common.hpp"
+#include "car_presets.hpp"
 #include "core/rg_arena_step.hpp"
+#include "core/rg_car_kickoff.hpp"
 #include "core/rg_env_commands.hpp"
 #include "group_walk.hpp"
 #include "rocketgpu_amd/rg_capi.h"
@@ -288,9 +290,12 @@ struct CarStepArgs {
 };
 
 // One lane per arena, its ticks back to back: arenas without cars take the ball tick, arenas with cars the full
-// car tick (one lane walks all of its queries).  Fields the game reports are refreshed on the call's last tick.
+// car tick (one lane walks all of its queries).  Workgroups are one wavefront; its first a.arenas_per_wave lanes
+// take one arena each (fewer arenas per wave: more waves per CU to hide the dependent loads).  Fields the game
+// reports are refreshed on the call's last tick.
 __global__ void __launch_bounds__(64) StepCarArenasKernel(StepArgs a, CarStepArgs c, int32_t ticks) {
-    const int i = static_cast<int>(blockIdx.x) * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x);
+    if (static_cast<int>(threadIdx.x) >= a.arenas_per_wave) return;
+    const int i = static_cast<int>(blockIdx.x) * a.arenas_per_wave + static_cast<int>(threadIdx.x);
     if (i >= a.count) return;
     const int e = a.arena_ids ? a.arena_ids[i] : i;
     if (e < 0 || e >= a.env_count) return;
@@ -321,6 +326,47 @@ __global__ void __launch_bounds__(64) StepCarArenasKernel(StepArgs a, CarStepArg
         const core::CarRespawnContext rc{&c.rules, c.seed, e, 0u, 0u, 0};
         core::StepCarArena(r, a.mesh, a.params, c.m, pads, rc, t + 1 == n, scratch);
     }
+}
+
+// Env e's records as the car cores see them (the controls are the batch's own row).
+__device__ inline core::CarArenaRefs CarRefsOf(const StepArgs& a, const CarStepArgs& c, int e) {
+    const int base = e * a.max_cars;
+    core::CarArenaRefs r;
+    r.ball = &a.balls[e];
+    r.env = &a.envs[e];
+    r.st = &a.internal[e];
+    r.pads = EnvPads(a, e);
+    r.cars = c.cars + base;
+    r.wheels = c.wheels + base;
+    r.cfgs = c.cfgs + base;
+    r.teams = c.teams + base;
+    r.controls = a.controls + base;
+    r.count = a.env_car_count[e];
+    r.env_idx = e;
+    r.car_base = base;
+    return r;
+}
+
+// reset_to_kickoff of a batch with cars: every env to a new match's kickoff; `rows.cars` null: no settled rows.
+__global__ void __launch_bounds__(64) ResetToKickoffCarsKernel(StepArgs a, CarStepArgs c, core::KickoffRows rows,
+                                                               uint64_t seed) {
+    const int e = static_cast<int>(blockIdx.x) * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x);
+    if (e >= a.env_count) return;
+    core::CarArenaRefs r = CarRefsOf(a, c, e);
+    core::ResetToKickoffCars(r, a.controls + r.car_base, a.max_cars, seed, rows.cars ? &rows : nullptr, c.rules,
+                             a.params, c.m);
+}
+
+// apply_env_commands of a batch with cars: command e to env e, for e < count.
+__global__ void __launch_bounds__(64) ApplyEnvCommandsCarsKernel(StepArgs a, CarStepArgs c, core::KickoffRows rows,
+                                                                 const EnvCommand* commands, int32_t count) {
+    const int e = static_cast<int>(blockIdx.x) * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x);
+    if (e >= count) return;
+    const EnvCommand cmd = commands[e];
+    if (!core::CommandActive(cmd)) return;
+    core::CarArenaRefs r = CarRefsOf(a, c, e);
+    core::ApplyEnvCommandCars(r, a.controls + r.car_base, a.max_cars, cmd, rows.cars ? &rows : nullptr, c.rules,
+                              a.params, c.m);
 }
 
 // The unsupported words of the arenas (0 for every arena stepped in full).
@@ -515,16 +561,7 @@ public:
             throw InvalidArgument("max_cars_per_env must be 0..8");
         }
         if (!(cfg.tick_dt > 0.0f)) throw InvalidArgument("tick_dt must be positive");
-        int device = 0;
-        Check(hipGetDevice(&device), "hipGetDevice");
-        no_alloc_scratch_ = NoAllocScratch(device);
         mesh_host_ = BuildArenaMesh(meshes);
-        tris_.Resize(mesh_host_.tris.size());
-        nodes_.Resize(mesh_host_.nodes.size());
-        infos_.Resize(mesh_host_.meshes.size());
-        tris_.Upload(mesh_host_.tris.data(), mesh_host_.tris.size(), nullptr);
-        nodes_.Upload(mesh_host_.nodes.data(), mesh_host_.nodes.size(), nullptr);
-        infos_.Upload(mesh_host_.meshes.data(), mesh_host_.meshes.size(), nullptr);
         params_ = DefaultStepParams(rules);
         params_.tick_dt = cfg.tick_dt;
         params_.ball_damping_scale =
@@ -538,6 +575,33 @@ public:
         for (int k = 0; k < rules.pad_count && k < rgw::kMaxPads; ++k) {
             car_args_.pad_pos[k] = Vec3(rules.pad_pos[k].x, rules.pad_pos[k].y, rules.pad_pos[k].z);
         }
+        Allocate();
+    }
+
+    // A batch on another batch's arena (mesh, spawn rules, pads and physics) with its own arena and car counts and
+    // settled-kickoff switch: the run that records settled kickoff rows.
+    BatchSim(const BatchSim& arena, int env_count, int max_cars, bool kickoff_settled) : cfg_(arena.cfg_) {
+        cfg_.env_count = env_count;
+        cfg_.max_cars_per_env = max_cars;
+        mesh_host_ = arena.mesh_host_;
+        params_ = arena.params_;
+        car_args_ = arena.car_args_;
+        kickoff_settled_ = kickoff_settled;
+        Allocate();
+    }
+
+    // Device buffers, default records and the constructor's kickoff reset with seed 0.
+    void Allocate() {
+        const rg_sim_config& cfg = cfg_;
+        int device = 0;
+        Check(hipGetDevice(&device), "hipGetDevice");
+        no_alloc_scratch_ = NoAllocScratch(device);
+        tris_.Resize(mesh_host_.tris.size());
+        nodes_.Resize(mesh_host_.nodes.size());
+        infos_.Resize(mesh_host_.meshes.size());
+        tris_.Upload(mesh_host_.tris.data(), mesh_host_.tris.size(), nullptr);
+        nodes_.Upload(mesh_host_.nodes.data(), mesh_host_.nodes.size(), nullptr);
+        infos_.Upload(mesh_host_.meshes.data(), mesh_host_.meshes.size(), nullptr);
         const size_t n = static_cast<size_t>(cfg.env_count);
         const size_t cap = n * static_cast<size_t>(cfg.max_cars_per_env);
         balls_.Resize(n);
@@ -594,6 +658,7 @@ public:
         if (n > static_cast<size_t>(car_capacity())) throw std::out_of_range("set_teams count exceeds car_capacity");
         if (n && !teams) throw InvalidArgument("host_teams must not be null");
         teams_.Upload(teams, n, s);
+        kick_dirty_ = true;
         Check(hipStreamSynchronize(s), "hipStreamSynchronize");
     }
     void SetCarConfigs(const CarConfig* configs, size_t n, hipStream_t s) {
@@ -602,6 +667,7 @@ public:
         }
         if (n && !configs) throw InvalidArgument("host_configs must not be null");
         configs_.Upload(configs, n, s);
+        kick_dirty_ = true;
         Check(hipStreamSynchronize(s), "hipStreamSynchronize");
     }
 
@@ -626,7 +692,14 @@ public:
     // reset_to_kickoff (src/rocketgpu.cpp:2085-2133): every env to a new match's kickoff.
     void ResetToKickoff(uint64_t seed, hipStream_t s) {
         if (has_cars_) {
-            throw InvalidArgument("kickoff resets of arenas with cars are not built yet (upload the states instead)");
+            seed_ = seed;
+            const bool rows = EnsureKickoffRows();
+            const StepArgs a = BaseArgs();
+            const int grid = (cfg_.env_count + 63) / 64;
+            hipLaunchKernelGGL(ResetToKickoffCarsKernel, dim3(grid), dim3(64), 0, s, a, CarArgs(controls_.data(), 0),
+                               KickoffRowsArg(rows), seed);
+            Check(hipGetLastError(), "ResetToKickoffCarsKernel launch");
+            return;
         }
         seed_ = seed;
         EnsureSettledBall();
@@ -639,7 +712,6 @@ public:
     // apply_env_commands (src/rocketgpu.cpp:2136-2218): command i to env i for i < count.  Same checks and messages.
     void ApplyEnvCommands(const EnvCommand* host_commands, size_t count, hipStream_t s) {
         if (host_commands == nullptr) throw InvalidArgument("host_commands must not be null");
-        RejectCarCommands();
         if (count > static_cast<size_t>(cfg_.env_count)) {
             throw std::out_of_range("apply_env_commands_host count exceeds env_count");
         }
@@ -650,8 +722,15 @@ public:
             settled_reset = settled_reset || (host_commands[i].reset_to_kickoff && host_commands[i].kickoff_countdown_ticks == 0);
         }
         if (active == 0) return;
-        if (settled_reset) EnsureSettledBall();
         if (commands_.size() < count) commands_.Resize(static_cast<size_t>(cfg_.env_count));
+        if (has_cars_) {
+            const bool rows = settled_reset && EnsureKickoffRows();
+            commands_.Upload(host_commands, count, s);
+            LaunchApplyCommandsCars(commands_.data(), static_cast<int32_t>(count), rows, s);
+            Check(hipStreamSynchronize(s), "hipStreamSynchronize");
+            return;
+        }
+        if (settled_reset) EnsureSettledBall();
         commands_.Upload(host_commands, count, s);
         LaunchApplyCommands(commands_.data(), static_cast<int32_t>(count), s);
         Check(hipStreamSynchronize(s), "hipStreamSynchronize");
@@ -660,9 +739,13 @@ public:
     // apply_env_commands_device (src/rocketgpu.cpp:2221-2248): the commands are read on the device.
     void ApplyEnvCommandsDevice(const EnvCommand* device_commands, size_t count, hipStream_t s) {
         if (device_commands == nullptr) throw InvalidArgument("device_commands must not be null");
-        RejectCarCommands();
         if (count > static_cast<size_t>(cfg_.env_count)) {
             throw std::out_of_range("apply_env_commands_device count exceeds env_count");
+        }
+        if (has_cars_) {
+            const bool rows = EnsureKickoffRows();
+            if (count > 0) LaunchApplyCommandsCars(device_commands, static_cast<int32_t>(count), rows, s);
+            return;
         }
         EnsureSettledBall();
         if (count == 0) return;
@@ -1296,13 +1379,7 @@ private:
 
     // Car arenas in the batch: one lane per arena with the full car tick (ball arenas in the same launch take the
     // ball tick).  Physics events of car arenas are not written.
-    void LaunchCars(const int32_t* ids, const int32_t* ticks_list, int32_t count, int32_t ticks,
-                    const CarControls* controls, int64_t stride, hipStream_t s) {
-        StepArgs a = BaseArgs();
-        a.arena_ids = ids;
-        a.arena_ticks = ticks_list;
-        a.count = count;
-        a.arenas_per_wave = 64;
+    CarStepArgs CarArgs(const CarControls* controls, int64_t stride) const {
         CarStepArgs c = car_args_;
         c.cars = cars_.data();
         c.wheels = wheels_.data();
@@ -1311,7 +1388,137 @@ private:
         c.controls = controls;
         c.control_stride = stride;
         c.seed = seed_;
+        return c;
+    }
+
+    // The settled kickoff rows (the reference's ensure_kickoff_settled / record_kickoff_settled,
+    // src/rocketgpu.cpp:1439-1612), recomputed after teams or car configs change.  The batch's car slots are grouped
+    // by (CarConfig bytes, team) in slot order; each new group is run in a batch of its own on this arena, one arena
+    // per group with max(max_cars, 5) cars of that group: a new-match reset with the game's countdown (kickoff seed
+    // mode 3: car k at kickoff spot k, k >= 5 at the respawn spots), kKickoffCountdownTicks ticks, then one at a
+    // time until every countdown has ended.  Row (group, k) is car k's car and wheel state then; the ball row is the
+    // first run's ball.  False when settled kickoffs are off.
+    bool EnsureKickoffRows() {
+        if (!kickoff_settled_) return false;
+        if (!kick_dirty_) return true;
+        const size_t cap = static_cast<size_t>(car_capacity());
+        const int spawns = std::max(cfg_.max_cars_per_env, 5);
+        std::vector<CarConfig> configs(cap);
+        std::vector<uint8_t> teams(cap);
+        configs_.Download(configs.data(), cap, nullptr);
+        teams_.Download(teams.data(), cap, nullptr);
+        Check(hipStreamSynchronize(nullptr), "hipStreamSynchronize");
+        std::vector<int32_t> group(cap);
+        std::vector<size_t> first_new;
+        const size_t known = kick_groups_.size();
+        for (size_t i = 0; i < cap; ++i) {
+            size_t g = 0;
+            while (g < kick_groups_.size() &&
+                   !(kick_groups_[g].team == teams[i] &&
+                     std::memcmp(&kick_groups_[g].config, &configs[i], sizeof(CarConfig)) == 0)) {
+                ++g;
+            }
+            if (g == kick_groups_.size()) {
+                kick_groups_.push_back({configs[i], teams[i]});
+                first_new.push_back(i);
+            }
+            group[i] = static_cast<int32_t>(g);
+        }
+        if (!first_new.empty()) {
+            try {
+                RecordKickoffRows(first_new.size(), spawns, known);
+            } catch (...) {
+                kick_groups_.resize(known);
+                kick_rows_car_.resize(known * static_cast<size_t>(spawns));
+                kick_rows_wheels_.resize(known * static_cast<size_t>(spawns));
+                throw;
+            }
+        }
+        const size_t rows = kick_rows_car_.size();
+        kick_cars_.Resize(rows);
+        kick_wheels_.Resize(rows);
+        kick_group_.Resize(cap);
+        kick_ball_.Resize(1);
+        kick_cars_.Upload(kick_rows_car_.data(), rows, nullptr);
+        kick_wheels_.Upload(kick_rows_wheels_.data(), rows, nullptr);
+        kick_group_.Upload(group.data(), cap, nullptr);
+        kick_ball_.Upload(&kick_rows_ball_, 1, nullptr);
+        Check(hipStreamSynchronize(nullptr), "hipStreamSynchronize");
+        kick_spawns_ = spawns;
+        kick_dirty_ = false;
+        return true;
+    }
+
+    void RecordKickoffRows(size_t groups, int spawns, size_t known) {
+        BatchSim run(*this, static_cast<int>(groups), spawns, false);
+        const size_t m = groups * static_cast<size_t>(spawns);
+        std::vector<int32_t> counts(groups, spawns);
+        std::vector<CarConfig> configs(m);
+        std::vector<uint8_t> teams(m);
+        for (size_t i = 0; i < m; ++i) {
+            configs[i] = kick_groups_[known + i / spawns].config;
+            teams[i] = kick_groups_[known + i / spawns].team;
+        }
+        run.SetEnvCarCount(counts.data(), groups, nullptr);
+        run.SetTeams(teams.data(), m, nullptr);
+        run.SetCarConfigs(configs.data(), m, nullptr);
+        EnvCommand c{};
+        c.reset_to_kickoff = 1;
+        c.reset_scoreboard = 1;
+        c.kickoff_seed_mode = 3;
+        c.kickoff_countdown_ticks = core::kKickoffCountdownGameTimers;
+        const std::vector<EnvCommand> cmds(groups, c);
+        run.ApplyEnvCommands(cmds.data(), groups, nullptr);
+        run.Step(core::kKickoffCountdownTicks, nullptr);
+        std::vector<EnvState> envs(groups);
+        for (int k = 0;; ++k) {
+            run.envs_.Download(envs.data(), groups, nullptr);
+            Check(hipStreamSynchronize(nullptr), "hipStreamSynchronize");
+            if (std::all_of(envs.begin(), envs.end(), [](const EnvState& e) { return e.countdown_phase == 0; })) break;
+            if (k == 16) throw std::runtime_error("kickoff_settled: the countdown did not end");
+            run.Step(1, nullptr);
+        }
+        std::vector<CarState> cars(m);
+        std::vector<WheelRuntimeState> wheels(m);
+        run.cars_.Download(cars.data(), m, nullptr);
+        run.wheels_.Download(wheels.data(), m, nullptr);
+        BallState ball;
+        run.balls_.Download(&ball, 1, nullptr);
+        Check(hipStreamSynchronize(nullptr), "hipStreamSynchronize");
+        kick_rows_car_.insert(kick_rows_car_.end(), cars.begin(), cars.end());
+        kick_rows_wheels_.insert(kick_rows_wheels_.end(), wheels.begin(), wheels.end());
+        if (known == 0) kick_rows_ball_ = ball;
+    }
+
+    core::KickoffRows KickoffRowsArg(bool on) const {
+        core::KickoffRows r{};
+        if (on) {
+            r.cars = kick_cars_.data();
+            r.wheels = kick_wheels_.data();
+            r.group = kick_group_.data();
+            r.ball = kick_ball_.data();
+            r.spawns = kick_spawns_;
+        }
+        return r;
+    }
+
+    void LaunchApplyCommandsCars(const EnvCommand* commands, int32_t count, bool rows, hipStream_t s) {
+        const StepArgs a = BaseArgs();
         const int grid = (count + 63) / 64;
+        hipLaunchKernelGGL(ApplyEnvCommandsCarsKernel, dim3(grid), dim3(64), 0, s, a, CarArgs(controls_.data(), 0),
+                           KickoffRowsArg(rows), commands, count);
+        Check(hipGetLastError(), "ApplyEnvCommandsCarsKernel launch");
+    }
+
+    void LaunchCars(const int32_t* ids, const int32_t* ticks_list, int32_t count, int32_t ticks,
+                    const CarControls* controls, int64_t stride, hipStream_t s) {
+        StepArgs a = BaseArgs();
+        a.arena_ids = ids;
+        a.arena_ticks = ticks_list;
+        a.count = count;
+        a.arenas_per_wave = per_wave_;
+        const CarStepArgs c = CarArgs(controls, stride);
+        const int grid = (count + per_wave_ - 1) / per_wave_;
         hipLaunchKernelGGL(StepCarArenasKernel, dim3(grid), dim3(64), 0, s, a, c, ticks);
         Check(hipGetLastError(), "StepCarArenasKernel launch");
     }
@@ -1454,6 +1661,22 @@ private:
     bool has_cars_ = false;
     uint64_t seed_ = 0;
     CarStepArgs car_args_{};
+    // Settled kickoff rows: the (config, team) groups seen so far, their rows on the host and the device, and each
+    // car slot's group.
+    struct KickoffGroup {
+        CarConfig config;
+        uint8_t team;
+    };
+    std::vector<KickoffGroup> kick_groups_;
+    std::vector<CarState> kick_rows_car_;
+    std::vector<WheelRuntimeState> kick_rows_wheels_;
+    BallState kick_rows_ball_{};
+    DeviceArray<CarState> kick_cars_;
+    DeviceArray<WheelRuntimeState> kick_wheels_;
+    DeviceArray<int32_t> kick_group_;
+    DeviceArray<BallState> kick_ball_;
+    int kick_spawns_ = 0;
+    bool kick_dirty_ = true;
     DeviceArray<uint8_t> teams_;
     DeviceArray<CarConfig> configs_;
     DeviceArray<BoostPadState> pads_;
@@ -1546,7 +1769,7 @@ extern "C" {
 
 const char* rg_last_error(void) { return rg::g_last_error.c_str(); }
 void rg_set_last_error(const char* message) { rg::g_last_error = message ? message : ""; }
-const char* rg_version(void) { return "rocketgpu_amd 0.1 (gfx950; ball-only arenas)"; }
+const char* rg_version(void) { return "rocketgpu_amd 0.2 (gfx950)"; }
 
 int rg_batch_create(const rg_sim_config* config, const rg_arena_layout* layout, rg_batch** out) {
     return Guard([&] {
@@ -1745,6 +1968,15 @@ int rg_batch_ball_path(rg_batch* b, int32_t samples, int32_t ticks, float goal_y
 
 int rg_batch_env_digests(rg_batch* b, uint64_t* device_out, void* stream) {
     return Guard([&] { b->sim->EnvDigests(device_out, S(stream)); });
+}
+
+int rg_car_preset_config(int32_t preset, void* out) {
+    return Guard([&] {
+        if (preset < 0 || preset >= rg::kPresetCount) throw rg::InvalidArgument("car preset must be 0..7");
+        if (!out) throw rg::InvalidArgument("out must not be null");
+        const rg::CarConfig c = rg::CarPresetConfig(preset);
+        std::memcpy(out, &c, sizeof(rg::CarConfig));
+    });
 }
 
 uint64_t rg_fold_digests(const uint64_t* d, int64_t n) {

@@ -41,3 +41,11 @@ def arena_mismatches(a, b, n_env, cars):
     bad |= (a["envs"] != b["envs"]).any(axis=1)
     bad |= (a["pads"].reshape(n_env, -1) != b["pads"].reshape(n_env, -1)).any(axis=1)
     return np.nonzero(bad)[0]
+
+
+def batch_records(sim):
+    """The records of a rocketgpu_amd.BatchSim in the layout of reference_records, and its unsupported words."""
+    get = sim.download_state_array
+    return ({"cars": get("car_states"), "wheels": get("wheel_states"), "balls": get("ball_states"),
+             "envs": get("env_states"), "pads": get("boost_pad_states")},
+            get("unsupported").view(np.uint32).ravel())
