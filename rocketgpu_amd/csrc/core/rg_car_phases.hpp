@@ -143,46 +143,43 @@ RG_HD Vec3 UeQuatRotate(const Quat& q, const Vec3& v) {
     return Vec3((cx * w2 + d * v.x) + t * q.x, (cy * w2 + d * v.y) + t * q.y, (cz * w2 + d * v.z) + t * q.z);
 }
 
-// UE3 rotator units: 65536 per turn; the sine table has 16384 entries (math.hpp:1442-1453).
+// The game's integer rotators (UE3 FRotator, math.hpp:1442-1533 of the reference): 65,536 units per turn, angles
+// read through a 16,384-entry sine table (entry j = sin(2 pi j / 16384), indexed by units / 4), radians rounded to
+// units half up.
 struct Ue3Rotator {
     int pitch, yaw, roll;
 };
-RG_HD float Ue3SinTab(int j) {
-    const float a = ((static_cast<float>(j) + static_cast<float>(j)) * 3.14159274f) * 6.103515625e-05f;
-    return UcrtSinf(a);
-}
-RG_HD float Ue3RotatorSin(int a) { return Ue3SinTab((a >> 2) & 16383); }
-RG_HD int Ue3Round(float v) { return static_cast<int>(rintf(v + v + 0.5f)) >> 1; }
 
-// FQuat -> FRotator (math.hpp:1492-1511).
+RG_HD float Ue3SinTab(int j) {
+    const float turns = (static_cast<float>(j) + static_cast<float>(j)) * 3.14159274f;
+    return UcrtSinf(turns * 6.103515625e-05f);  // 6.1e-5 = 1 / 16384
+}
+RG_HD float Ue3RotatorSin(int units) { return Ue3SinTab((units >> 2) & 16383); }
+RG_HD float Ue3RotatorCos(int units) { return Ue3RotatorSin(units + 16384); }
+
+// Round to the nearest unit, halves up: round(2 v + 0.5) / 2 in integers.
+RG_HD int Ue3Round(float v) { return static_cast<int>(rintf(v + v + 0.5f)) >> 1; }
+RG_HD int RadiansToRotator(float radians) { return Ue3Round(radians * 32768.0f / 3.14159274f); }
+
+// The rotator of a quaternion: yaw and pitch from the forward axis, roll from the up and right axes seen in the
+// frame turned by the yaw alone (its right axis is (-sin yaw, cos yaw, -0)).
 RG_HD Ue3Rotator Ue3QuatToRotator(const Quat& q) {
-    const float x2 = q.x + q.x, y2 = q.y + q.y, z2 = q.z + q.z;
-    const float xx = q.x * x2, xy = q.x * y2, xz = q.x * z2;
-    const float yy = q.y * y2, yz = q.y * z2, zz = q.z * z2;
-    const float wx = q.w * x2, wy = q.w * y2, wz = q.w * z2;
-    const Vec3 fwd(1.0f - (yy + zz), xy + wz, xz - wy);
-    const Vec3 right(xy - wz, 1.0f - (xx + zz), yz + wx);
-    const Vec3 up(xz + wy, yz - wx, 1.0f - (xx + yy));
+    Vec3 fwd, right, up;
+    UeQuatAxes(q, fwd, right, up);
     Ue3Rotator r;
-    r.yaw = Ue3Round(UcrtAtan2f(fwd.y, fwd.x) * 32768.0f / 3.14159274f);
-    const float fwd_xy = static_cast<float>(sqrt(static_cast<double>(fwd.y * fwd.y + fwd.x * fwd.x)));
-    r.pitch = Ue3Round(UcrtAtan2f(fwd.z, fwd_xy) * 32768.0f / 3.14159274f);
-    const float sy = Ue3RotatorSin(r.yaw);
-    const float cy = Ue3RotatorSin(r.yaw + 16384);
-    const float z_dot = (up.y * cy + up.x * -sy) + up.z * -0.0f;
-    const float y_dot = (right.y * cy + right.x * -sy) + right.z * -0.0f;
-    r.roll = Ue3Round(UcrtAtan2f(z_dot, y_dot) * 32768.0f / 3.14159274f);
+    r.yaw = RadiansToRotator(UcrtAtan2f(fwd.y, fwd.x));
+    const float ground = static_cast<float>(sqrt(static_cast<double>(fwd.y * fwd.y + fwd.x * fwd.x)));
+    r.pitch = RadiansToRotator(UcrtAtan2f(fwd.z, ground));
+    const Vec3 yawed_right(-Ue3RotatorSin(r.yaw), Ue3RotatorCos(r.yaw), -0.0f);
+    r.roll = RadiansToRotator(UcrtAtan2f(Dot(up, yawed_right), Dot(right, yawed_right)));
     return r;
 }
 
-// Up axis of a rotator (math.hpp:1524-1533).
+// The up axis of a rotator: the third column of its rotation matrix.
 RG_HD Vec3 Ue3RotatorUpAxis(const Ue3Rotator& r) {
-    const float sy = Ue3RotatorSin(r.yaw);
-    const float cy = Ue3RotatorSin(r.yaw + 16384);
-    const float sp = Ue3RotatorSin(r.pitch);
-    const float cp = Ue3RotatorSin(r.pitch + 16384);
-    const float sr = Ue3RotatorSin(r.roll);
-    const float cr = Ue3RotatorSin(r.roll + 16384);
+    const float sp = Ue3RotatorSin(r.pitch), cp = Ue3RotatorCos(r.pitch);
+    const float sy = Ue3RotatorSin(r.yaw), cy = Ue3RotatorCos(r.yaw);
+    const float sr = Ue3RotatorSin(r.roll), cr = Ue3RotatorCos(r.roll);
     const float cr_sp = cr * sp;
     return Vec3(-(cr_sp * cy + sy * sr), cy * sr - cr_sp * sy, cp * cr);
 }
@@ -627,9 +624,11 @@ RG_HD int BodyDampingClampCar(CarState& car, const CarParams& m) {
 // ------------------------------------------------------------------------------------------------------------
 // The pre-physics jump state machine (PreTickCarPrePhysics, env_step.hpp:2981-3120).
 
-constexpr uint8_t kCalPreAirJumpStarted = 1, kCalPreFlipStarted = 2, kCalPreFlipVelChanged = 4,
-                  kCalPreAirJumpUsedBefore = 8, kCalPreSetV3Tick = 16, kCalPreFlippingAtStart = 32,
-                  kCalPreWasJumping = 64;
+// Bits of CarState::cal_pre_flags, the pre-tick's record for the vehicle update: a ground jump started, a flip
+// started, the flip's impulse changed the velocity, the car had used its air jump before, supersonic came from a
+// set state, the car was flipping / jumping when the tick began.
+constexpr uint8_t kPreGroundJumpStarted = 1, kPreFlipStarted = 2, kPreFlipMovedCar = 4, kPreHadAirJump = 8,
+                  kPreSupersonicFromSet = 16, kPreFlippingAtStart = 32, kPreWasJumping = 64;
 
 // Forward speed of the tick-start rotation applied to a velocity (SnapshotForwardSpeed,
 // raycast_wheels_and_contact_prep.hpp:989-1015): the quaternion's x axis rotated the game's way.
@@ -644,113 +643,136 @@ RG_HD float ForwardSpeed(const Quat& q, const Vec3& v) {
     return (fwd.x * v.x + fwd.y * v.y) + fwd.z * v.z;
 }
 
+// The car's jump-related state as the tick found it: every decision below compares against these values, not
+// against fields the tick has already changed.
+struct JumpTickStart {
+    bool flipping, flipped, jumped, double_jumped, jumping, on_ground;
+    bool jump_held;        // last tick's jump control
+    bool fresh;            // first tick since the state was set
+    float flip_time, jump_time;
+    int wheels_on_ground;
+};
+
+RG_HD JumpTickStart CaptureJumpTickStart(const CarState& car) {
+    JumpTickStart s;
+    s.flipping = car.is_flipping != 0;
+    s.flipped = car.has_flipped != 0;
+    s.jumped = car.has_jumped != 0;
+    s.double_jumped = car.has_double_jumped != 0;
+    s.jumping = car.is_jumping != 0;
+    s.on_ground = car.is_on_ground != 0;
+    s.jump_held = car.last_controls.jump != 0;
+    s.fresh = car.tick_count_since_update == 0;
+    s.flip_time = car.internal_flip_time;
+    s.jump_time = car.internal_jump_time;
+    s.wheels_on_ground = 0;
+    for (int w = 0; w < 4; ++w) s.wheels_on_ground += car.wheels_with_contact[w] != 0 ? 1 : 0;
+    return s;
+}
+
+// The game's own jump bookkeeping, kept in the wheel runtime state (its cal_* fields): whether the game counts the
+// car as having jumped and double-jumped, the ground flag it read at the last frame boundary, and whether a render
+// frame is open.  `grounded`: three or more wheels on the ground at the tick's start.  Returns true when the game
+// refuses a ground jump this tick (it still counts the last jump).
+RG_HD bool UpdateGameJumpWindow(WheelRuntimeState& w, const JumpTickStart& s, bool grounded, float frame_frac) {
+    if (s.fresh && !w.cal_restored) {
+        // A car whose state was just set: the game's view starts from the set flags.
+        w.cal_game_jumped = s.jumped ? 1 : 0;
+        w.cal_game_double_jumped = (s.double_jumped || s.flipped) && !grounded ? 1 : 0;
+        w.cal_cached_ground = grounded ? 1 : 0;
+        w.cal_jump_inactive_time = s.jumping ? 0.0f : s.jump_time;
+        w.cal_frame_open = 0;
+    }
+    w.cal_restored = 0;
+    if (w.cal_frame_open == 0) {
+        // Frame boundary: the game re-reads the ground, and a landing clears both jumps.
+        if (grounded && w.cal_cached_ground == 0) w.cal_game_jumped = w.cal_game_double_jumped = 0;
+        w.cal_cached_ground = grounded ? 1 : 0;
+    }
+    const bool refuse = w.cal_game_jumped != 0;
+    // On the ground long enough after a jump ended: the game resets its jump.
+    const bool reset = w.cal_game_jumped != 0 && grounded && !s.jumping && w.cal_jump_inactive_time >= 0.125f;
+    if (reset) {
+        w.cal_game_jumped = w.cal_game_double_jumped = 0;
+        w.cal_cached_ground = 1;
+    }
+    w.cal_frame_open = frame_frac < 0.0f ? 1 : 0;
+    return refuse;
+}
+
+// The game has closed the car's air-jump window: it double-jumped or flipped already, or its jump ended longer ago
+// than the double-jump delay.
+RG_HD bool GameAirWindowClosed(const WheelRuntimeState& w, const CarParams& m) {
+    return w.cal_game_double_jumped != 0 || (w.cal_game_jumped != 0 && w.cal_jump_inactive_time > m.doublejump_max_delay);
+}
+
+// The jump state machine before physics (the reference's PreTickCarPrePhysics, env_step.hpp:2981-3120): supersonic
+// tracking, what a new jump press becomes (an auto-flip when the car lies on its roof, a ground jump, or a double
+// jump / flip in the air, each gated by the game's own jump bookkeeping), and the per-tick values the later phases
+// read (cal_pre_flags, the forward speed at the tick's start, the pitch-lock restart time).
 RG_HD void PreTickCarPrePhysics(CarState& car, CarControls controls, const CarConfig& cfg, const CarParams& m,
                                 float frame_frac, float dt, WheelRuntimeState* ws, const Quat& start_rot,
                                 const Vec3& start_vel_uu) {
     car.cal_pre_flags = 0;
-    const uint8_t start_is_flipping = car.is_flipping;
-    const uint8_t start_has_flipped = car.has_flipped;
-    const uint8_t start_has_jumped = car.has_jumped;
-    const uint8_t start_has_double_jumped = car.has_double_jumped;
-    const uint8_t start_is_jumping = car.is_jumping;
-    const uint8_t start_is_on_ground = car.is_on_ground;
-    const float start_internal_flip_time = car.internal_flip_time;
-    const float start_internal_jump_time = car.internal_jump_time;
-    const bool start_last_jump = car.last_controls.jump != 0;
-    const uint64_t start_ticks = car.tick_count_since_update;
-    const int start_wheel_contacts = static_cast<int>(car.wheels_with_contact[0] != 0) +
-                                     static_cast<int>(car.wheels_with_contact[1] != 0) +
-                                     static_cast<int>(car.wheels_with_contact[2] != 0) +
-                                     static_cast<int>(car.wheels_with_contact[3] != 0);
+    const JumpTickStart s = CaptureJumpTickStart(car);
     car.demo_no_contact_response = car.is_demoed ? 1 : 0;
     if (car.is_demoed) return;
     if (car.driving_disabled) controls.jump = 0;
-    const bool set_v3_tick = car.supersonic_set_v3 && start_ticks == 0;
+    // A set state's supersonic flag stands in for this tick's supersonic update.
+    const bool supersonic_from_set = car.supersonic_set_v3 && s.fresh;
     if (car.supersonic_set_v3) {
         car.supersonic_set_v3 = 0;
     } else {
         UpdateSupersonic(car, start_vel_uu, m, dt);
     }
     car.vel_touched_solve_stage = 0;
+
+    // A press outside a flip goes to the auto-flip first: one that starts now or one already running.
+    const bool press = controls.jump && !s.jump_held;
+    const bool press_outside_flip = press && !s.flipping;
+    const bool starts_auto_flip = press_outside_flip && !car.is_auto_flipping && AutoFlipGate(car);
+    const bool press_to_auto_flip = press_outside_flip && (car.is_auto_flipping || starts_auto_flip);
+    CarFrame frame{};
+    const bool has_frame = car.is_auto_flipping != 0 ||
+                           (press && car.world_contact.has_contact && car.world_contact.contact_normal.z > 0.707f);
+    if (has_frame) frame = MakeCarFrame(car, cfg, m.car_mass);
+
+    if (s.fresh && ws != nullptr && ws->cal_restored) car.flip_timer_live = 1;
+    const bool flip_clock_runs = !s.flipped && !s.flipping && (s.flip_time > 0.0f || car.flip_timer_live);
+    const float pitchlock_restart_time = flip_clock_runs ? s.flip_time : -1.0f;
+
+    const bool grounded = ws != nullptr && s.wheels_on_ground >= 3;
+    const bool ground_jump_refused = ws != nullptr && UpdateGameJumpWindow(*ws, s, grounded, frame_frac);
+    const float start_forward_speed = ForwardSpeed(start_rot, start_vel_uu);
+    const bool ground_jump = press && s.on_ground && !s.jumping && !ground_jump_refused && !press_to_auto_flip;
+    if (ground_jump) StartJump(car, m);
+
+    // The press in the air: a double jump or a flip, unless the game's window is closed.
+    const bool window_closed = ws != nullptr && GameAirWindowClosed(*ws, m);
+    const bool air_press = press && !grounded && !window_closed && !press_to_auto_flip;
+    const bool had_air_jump = car.has_double_jumped || car.has_flipped;
+    const bool try_air = air_press && !ground_jump;
+    float raw_pitch = 2.0f;
+    if (try_air) controls = PreTickControls(car, controls, raw_pitch);
+    const float dodge_forward_speed = try_air ? ForwardSpeed(car.rot, car.vel) : 0.0f;
+    const Vec3 vel_before = car.vel_bt;
     bool flip_started = false;
+    UpdateDoubleJumpOrFlipState(car, controls, cfg, m, try_air, dodge_forward_speed, dt, ws != nullptr && try_air,
+                                raw_pitch, &flip_started);
+    const bool flip_moved_car = car.vel_bt.x != vel_before.x || car.vel_bt.y != vel_before.y ||
+                                car.vel_bt.z != vel_before.z;
+    ApplyAutoFlip(car, m, dt, starts_auto_flip, has_frame ? &frame : nullptr);
 
-    const bool jump_pressed = controls.jump && !start_last_jump;
-    const bool flip_over_starts = jump_pressed && !start_is_flipping && !car.is_auto_flipping && AutoFlipGate(car);
-    const bool flip_over_takes_press = jump_pressed && !start_is_flipping && (car.is_auto_flipping || flip_over_starts);
-    const bool need_frame = car.is_auto_flipping != 0 ||
-                            (jump_pressed && car.world_contact.has_contact && car.world_contact.contact_normal.z > 0.707f);
-    CarFrame frame_storage;
-    const CarFrame* frame = nullptr;
-    if (need_frame) {
-        frame_storage = MakeCarFrame(car, cfg, m.car_mass);
-        frame = &frame_storage;
-    }
-    if (start_ticks == 0 && ws != nullptr && ws->cal_restored) car.flip_timer_live = 1;
-    const float pitchlock_restart_time =
-        (!start_has_flipped && !start_is_flipping && (start_internal_flip_time > 0.0f || car.flip_timer_live))
-            ? start_internal_flip_time
-            : -1.0f;
-
-    bool air_jump_started = false;
-    const bool cal_jump = ws != nullptr;
-    const bool cal_jump_g = cal_jump && start_wheel_contacts >= 3;
-    bool cal_jump_blocked = false;
-    if (cal_jump) {
-        WheelRuntimeState& jw = *ws;
-        if (start_ticks == 0 && !jw.cal_restored) {
-            jw.cal_game_jumped = start_has_jumped ? 1 : 0;
-            jw.cal_game_double_jumped = (start_has_double_jumped || start_has_flipped) && !cal_jump_g ? 1 : 0;
-            jw.cal_cached_ground = cal_jump_g ? 1 : 0;
-            jw.cal_jump_inactive_time = start_is_jumping ? 0.0f : start_internal_jump_time;
-            jw.cal_frame_open = 0;
-        }
-        jw.cal_restored = 0;
-        if (!jw.cal_frame_open) {
-            if (cal_jump_g && !jw.cal_cached_ground) {
-                jw.cal_game_jumped = 0;
-                jw.cal_game_double_jumped = 0;
-            }
-            jw.cal_cached_ground = cal_jump_g ? 1 : 0;
-        }
-        cal_jump_blocked = jw.cal_game_jumped != 0;
-        if (jw.cal_game_jumped && cal_jump_g && !start_is_jumping && jw.cal_jump_inactive_time >= 0.125f) {
-            jw.cal_game_jumped = 0;
-            jw.cal_game_double_jumped = 0;
-            jw.cal_cached_ground = 1;
-        }
-        jw.cal_frame_open = frame_frac < 0.0f ? 1 : 0;
-    }
-    const uint8_t was_jumping = start_is_jumping;
-    const float forward_speed_uu = ForwardSpeed(start_rot, start_vel_uu);
-    if (jump_pressed && start_is_on_ground && !start_is_jumping && !cal_jump_blocked && !flip_over_takes_press) {
-        StartJump(car, m);
-        air_jump_started = true;
-    }
-    const bool air_window_closed =
-        cal_jump && (ws->cal_game_double_jumped ||
-                     (ws->cal_game_jumped && ws->cal_jump_inactive_time > m.doublejump_max_delay));
-    const bool air_jump_pressed = jump_pressed && !(cal_jump && cal_jump_g) && !air_window_closed && !flip_over_takes_press;
-    const bool press_as_air = cal_jump && !cal_jump_g && jump_pressed && !air_window_closed && !flip_over_takes_press;
-    const uint8_t air_jump_used_before = (car.has_double_jumped || car.has_flipped) ? 1 : 0;
-
-    bool flip_vel_changed = false;
-    {
-        const bool try_air_jump = air_jump_pressed && !air_jump_started;
-        float dodge_raw_pitch = 2.0f;
-        if (try_air_jump) controls = PreTickControls(car, controls, dodge_raw_pitch);
-        const float fbw_forward_speed = try_air_jump ? ForwardSpeed(car.rot, car.vel) : 0.0f;
-        const Vec3 before = car.vel_bt;
-        UpdateDoubleJumpOrFlipState(car, controls, cfg, m, try_air_jump, fbw_forward_speed, dt,
-                                    press_as_air && !air_jump_started, dodge_raw_pitch, &flip_started);
-        flip_vel_changed = car.vel_bt.x != before.x || car.vel_bt.y != before.y || car.vel_bt.z != before.z;
-    }
-    ApplyAutoFlip(car, m, dt, flip_over_starts, frame);
-    car.cal_pre_flags = static_cast<uint8_t>(
-        (air_jump_started ? kCalPreAirJumpStarted : 0) | (flip_started ? kCalPreFlipStarted : 0) |
-        (flip_vel_changed ? kCalPreFlipVelChanged : 0) | (air_jump_used_before ? kCalPreAirJumpUsedBefore : 0) |
-        (set_v3_tick ? kCalPreSetV3Tick : 0) | (start_is_flipping ? kCalPreFlippingAtStart : 0) |
-        (was_jumping ? kCalPreWasJumping : 0));
-    car.cal_pre_forward_speed_uu = forward_speed_uu;
+    uint8_t flags = 0;
+    flags |= ground_jump ? kPreGroundJumpStarted : 0;
+    flags |= flip_started ? kPreFlipStarted : 0;
+    flags |= flip_moved_car ? kPreFlipMovedCar : 0;
+    flags |= had_air_jump ? kPreHadAirJump : 0;
+    flags |= supersonic_from_set ? kPreSupersonicFromSet : 0;
+    flags |= s.flipping ? kPreFlippingAtStart : 0;
+    flags |= s.jumping ? kPreWasJumping : 0;
+    car.cal_pre_flags = flags;
+    car.cal_pre_forward_speed_uu = start_forward_speed;
     car.cal_pre_pitchlock_restart_time = pitchlock_restart_time;
 }
 
@@ -784,42 +806,61 @@ RG_HD float WrapAngle(float a) {
     return a;
 }
 
-// Yaw quaternion with the exact eighth-turn special cases (math.hpp:838-863).
+// The quaternion of a yaw about z (math.hpp:838-863 of the reference): within 1e-6 of an odd multiple of an eighth
+// turn or of a quarter turn, the half angle's exact sine and cosine; otherwise UCRT cos and sin of the half angle.
 RG_HD Quat FromYaw(float yaw) {
-    constexpr float kEps = 1e-6f;
-    constexpr float kSin8 = 0.38268343236508977173f, kCos8 = 0.92387953251128675613f;
-    constexpr float kSqrtHalf = 0.7071067811865475244f;
-    if (Abs(yaw - (Pi() * 0.25f)) <= kEps) return Quat(kCos8, 0.0f, 0.0f, kSin8);
-    if (Abs(yaw + (Pi() * 0.25f)) <= kEps) return Quat(kCos8, 0.0f, 0.0f, -kSin8);
-    if (Abs(yaw - (Pi() * 0.75f)) <= kEps) return Quat(kSin8, 0.0f, 0.0f, kCos8);
-    if (Abs(yaw + (Pi() * 0.75f)) <= kEps) return Quat(kSin8, 0.0f, 0.0f, -kCos8);
-    if (Abs(yaw - HalfPi()) <= kEps) return Quat(kSqrtHalf, 0.0f, 0.0f, kSqrtHalf);
-    if (Abs(yaw + HalfPi()) <= kEps) return Quat(kSqrtHalf, 0.0f, 0.0f, -kSqrtHalf);
-    const float h = 0.5f * yaw;
-    return Quat(UcrtCosf(h), 0.0f, 0.0f, UcrtSinf(h));
+    constexpr float kSin8 = 0.38268343236508977173f, kCos8 = 0.92387953251128675613f;  // sin, cos of pi / 8
+    constexpr float kHalf = 0.7071067811865475244f;                                       // sin, cos of pi / 4
+    struct Exact {
+        float angle, w, z;
+    };
+    const Exact exact[6] = {{Pi() * 0.25f, kCos8, kSin8},   {-(Pi() * 0.25f), kCos8, -kSin8},
+                            {Pi() * 0.75f, kSin8, kCos8},   {-(Pi() * 0.75f), kSin8, -kCos8},
+                            {HalfPi(), kHalf, kHalf},        {-HalfPi(), kHalf, -kHalf}};
+    for (const Exact& e : exact) {
+        if (Abs(yaw - e.angle) <= 1e-6f) return Quat(e.w, 0.0f, 0.0f, e.z);
+    }
+    const float half = 0.5f * yaw;
+    return Quat(UcrtCosf(half), 0.0f, 0.0f, UcrtSinf(half));
 }
 
-// A yaw in rotator units as a quaternion built from the game's rotation matrix (math.hpp:1456-1476).
-RG_HD Quat Ue3QuatFromYawRotator(int yaw) {
-    const float sy = Ue3RotatorSin(yaw), cy = Ue3RotatorSin(yaw + 16384);
-    const float cp = Ue3RotatorSin(16384);
-    const float m[3][3] = {{cy * cp, cp * sy, 0.0f}, {-(cp * sy), cy * cp, -0.0f}, {-0.0f, 0.0f, cp * cp}};
-    const float tr = (m[1][1] + m[0][0]) + m[2][2];
-    if (tr > 0.0f) {
-        const float inv_s = 1.0f / sqrtf(tr + 1.0f);
-        const float s = inv_s * 0.5f;
-        return Quat(0.5f / inv_s, (m[1][2] - m[2][1]) * s, (m[2][0] - m[0][2]) * s, (m[0][1] - m[1][0]) * s);
+// The game's quaternion of a rotation matrix m (rows: the rotated x, y, z axes), UE3's FQuat(FMatrix): the trace
+// branch, else the branch of the largest diagonal entry; 1 / sqrt is taken first and the scalar part is 0.5 / that.
+RG_HD Quat UeQuatFromMatrix(const float m[3][3]) {
+    const float trace = (m[1][1] + m[0][0]) + m[2][2];
+    if (trace > 0.0f) {
+        const float root = 1.0f / sqrtf(trace + 1.0f);
+        const float h = root * 0.5f;
+        return Quat(0.5f / root, (m[1][2] - m[2][1]) * h, (m[2][0] - m[0][2]) * h, (m[0][1] - m[1][0]) * h);
     }
-    int i = m[0][0] < m[1][1] ? 1 : 0;
-    if (m[i][i] <= m[2][2] && m[2][2] != m[i][i]) i = 2;
-    const int j = i == 2 ? 0 : i + 1, k = j == 2 ? 0 : j + 1;
-    const float inv_s = 1.0f / sqrtf(((m[i][i] - m[j][j]) - m[k][k]) + 1.0f);
-    const float s = inv_s * 0.5f;
+    int a = m[0][0] < m[1][1] ? 1 : 0;
+    if (m[a][a] <= m[2][2] && m[2][2] != m[a][a]) a = 2;
+    const int b = (a + 1) % 3, c = (a + 2) % 3;
+    const float root = 1.0f / sqrtf(((m[a][a] - m[b][b]) - m[c][c]) + 1.0f);
+    const float h = root * 0.5f;
     float v[3];
-    v[i] = 0.5f / inv_s;
-    v[j] = (m[i][j] + m[j][i]) * s;
-    v[k] = (m[i][k] + m[k][i]) * s;
-    return Quat((m[j][k] - m[k][j]) * s, v[0], v[1], v[2]);
+    v[a] = 0.5f / root;
+    v[b] = (m[a][b] + m[b][a]) * h;
+    v[c] = (m[a][c] + m[c][a]) * h;
+    return Quat((m[b][c] - m[c][b]) * h, v[0], v[1], v[2]);
+}
+
+// The rotation matrix of a pure yaw (pitch and roll 0, their table cosine kept as the game computes it), and its
+// quaternion.
+RG_HD Quat Ue3QuatFromYawRotator(int yaw) {
+    const float sy = Ue3RotatorSin(yaw), cy = Ue3RotatorCos(yaw);
+    const float cp = Ue3RotatorCos(0);
+    const float m[3][3] = {{cy * cp, cp * sy, 0.0f}, {-(cp * sy), cy * cp, -0.0f}, {-0.0f, 0.0f, cp * cp}};
+    return UeQuatFromMatrix(m);
+}
+
+// UE3's FQuat product a * b, its terms in the game's order.
+RG_HD Quat UeQuatMul(const Quat& a, const Quat& b) {
+    const float w = ((a.w * b.w - a.x * b.x) - a.y * b.y) - a.z * b.z;
+    const float x = ((a.w * b.x + a.x * b.w) + a.y * b.z) - a.z * b.y;
+    const float y = ((a.w * b.y + a.y * b.w) + a.z * b.x) - a.x * b.z;
+    const float z = ((a.w * b.z + a.z * b.w) + a.x * b.y) - a.y * b.x;
+    return Quat(w, x, y, z);
 }
 
 RG_HD int SpawnYawRotatorUnits(float yaw) { return static_cast<int>(yaw * (32768.0f / 3.14159274f) + 0.5f); }
@@ -827,11 +868,7 @@ RG_HD int SpawnYawRotatorUnits(float yaw) { return static_cast<int>(yaw * (32768
 // The spawned car's authoritative basis from its yaw in rotator units (half a turn more for Orange).
 RG_HD void SeedCarBasisFromSpawnRotator(CarState& car, int blue_yaw, bool orange) {
     Quat q = Ue3QuatFromYawRotator(blue_yaw);
-    if (orange) {
-        const Quat h = Ue3QuatFromYawRotator(32768);
-        q = Quat(((h.w * q.w - h.x * q.x) - h.y * q.y) - h.z * q.z, ((h.w * q.x + h.x * q.w) + h.y * q.z) - h.z * q.y,
-                 ((h.w * q.y + h.y * q.w) + h.z * q.x) - h.x * q.z, ((h.w * q.z + h.z * q.w) + h.x * q.y) - h.y * q.x);
-    }
+    if (orange) q = UeQuatMul(Ue3QuatFromYawRotator(32768), q);
     GameQuatToColumns(q, car.basis_forward, car.basis_right, car.basis_up);
     car.basis_authoritative = 1;
 }
@@ -909,17 +946,17 @@ RG_HD void PlaceCarAtWorldSpawn(CarState& car, const Vec3& pos, float yaw, float
 RG_HD void RespawnPlace(const CarSpawnRules& rules, int team, int spot, int dir, Vec3& ground, float& yaw) {
     ground = rules.respawn_pos[team][spot & 3];
     yaw = rules.respawn_yaw[team][spot & 3];
+    if (dir < 0) return;
+    // The spot's facing (cos yaw, sin yaw) from its yaw quaternion, turned by dir eighths of a turn.
+    constexpr float h = 0.70710677f;
+    constexpr float kEighthCos[8] = {1.0f, h, 0.0f, -h, -1.0f, -h, 0.0f, h};
+    constexpr float kEighthSin[8] = {0.0f, h, 1.0f, h, 0.0f, -h, -1.0f, -h};
     const Quat q = FromYaw(WrapAngle(yaw));
-    const float fx = q.w * q.w - q.z * q.z;
-    const float fy = 2.0f * q.w * q.z;
-    if (dir >= 0) {
-        constexpr float k = 0.70710677f;
-        const int d = dir & 7;
-        const float ux = d == 0 ? 1.0f : (d == 1 || d == 7) ? k : (d == 2 || d == 6) ? 0.0f : (d == 3 || d == 5) ? -k : -1.0f;
-        const float uy = (d == 0 || d == 4) ? 0.0f : (d == 1 || d == 3) ? k : d == 2 ? 1.0f : d == 6 ? -1.0f : -k;
-        ground.x += 192.0f * (fx * ux - fy * uy);
-        ground.y += 192.0f * (fx * uy + fy * ux);
-    }
+    const float face_x = q.w * q.w - q.z * q.z;
+    const float face_y = 2.0f * q.w * q.z;
+    const float c = kEighthCos[dir & 7], sn = kEighthSin[dir & 7];
+    ground.x += 192.0f * (face_x * c - face_y * sn);
+    ground.y += 192.0f * (face_x * sn + face_y * c);
 }
 
 RG_HD void RespawnCar(CarState& car, const CarSpawnRules& rules, int team, int slot, float spawn_z, float boost,
@@ -1117,52 +1154,91 @@ RG_HD void AdvanceJumpReportState(CarState& car, const CarParams& m, float dt, b
     }
 }
 
-// The state the game would report: working velocities and flags move to the internal_* fields, the public
-// has_jumped / has_double_jumped / has_flipped become the game's, and the RLBot dodge fields are derived.
+// RLBot's air state of a car: 0 on the ground, 1 jumping, 2 just double-jumped (within 13 ticks), 3 flipping, 4 in
+// the air otherwise.
+RG_HD uint8_t RlbotAirState(const CarState& car) {
+    if (car.is_jumping) return 1;
+    if (car.is_flipping) return 3;
+    if (car.is_on_ground) return 0;
+    return car.double_jump_ticks < 13 ? 2 : 4;
+}
+
+// The game's jump flags at the end of a tick: from its own bookkeeping when the wheel runtime state is there (a
+// frame boundary reached on three wheels clears them), else from the working flags.
+struct GameJumpFlags {
+    bool jumped, air_jumped;  // air_jumped: double jump or flip
+};
+
+RG_HD GameJumpFlags GameJumpFlagsOf(const CarState& car, const WheelRuntimeState* w) {
+    GameJumpFlags g;
+    if (w == nullptr) {
+        g.jumped = car.has_jumped != 0;
+        g.air_jumped = car.has_double_jumped || car.has_flipped;
+        return g;
+    }
+    g.jumped = w->cal_game_jumped != 0;
+    g.air_jumped = w->cal_game_double_jumped != 0;
+    int wheels = 0;
+    for (int k = 0; k < 4; ++k) wheels += car.wheels_with_contact[k] != 0 ? 1 : 0;
+    if (w->cal_frame_open == 0 && w->cal_cached_ground == 0 && wheels >= 3) g.jumped = g.air_jumped = false;
+    return g;
+}
+
+// The RLBot v5 dodge fields: whether the car has dodged (kept until it lands), the time left to dodge after a jump
+// (-1 when it cannot), the time since the dodge began, and the dodge direction from the flip's relative torque.
+RG_HD void DeriveDodgeFields(CarState& car, const WheelRuntimeState* w, const CarParams& m, float dt) {
+    const bool flipped = car.internal_has_flipped != 0;
+    const bool dodged = car.air_state != 0 && (car.rlbotv5_dodged || flipped);
+    car.rlbotv5_dodged = dodged ? 1 : 0;
+    const float since_jump = w != nullptr ? w->cal_jump_inactive_time : car.air_time_since_jump;
+    const bool can_dodge = !car.is_on_ground && !car.is_jumping && car.game_has_jumped &&
+                           !car.internal_has_double_jumped && !dodged &&
+                           since_jump < m.doublejump_max_delay - 0.5f * dt;
+    car.dodge_timeout = can_dodge ? m.doublejump_max_delay - since_jump : -1.0f;
+    const bool settled = car.is_on_ground && !car.is_jumping && !car.is_flipping;
+    if (flipped && !settled) {
+        car.dodge_elapsed = car.internal_flip_time;
+    } else {
+        car.dodge_elapsed = dodged ? car.dodge_elapsed + dt : 0.0f;
+    }
+    car.dodge_dir_x = car.flip_rel_torque.y;
+    car.dodge_dir_y = -car.flip_rel_torque.x;
+}
+
+// The state the game reports at the end of a step call (the reference's ReportRLBotCarFields, env_step.hpp:
+// 5857-5910): the working velocities and jump flags are kept in the internal_* fields for the next tick, the
+// public jump flags become the game's, the RLBot fields are derived, and reported_* records what was reported so
+// the next tick can tell a caller's edit from the report.
 RG_HD void ReportCarFields(CarState& car, const WheelRuntimeState* jw, const CarParams& m, float dt) {
     if (car.is_demoed && car.tick_count_since_update == 0) {
         car.reported_flags = 0;
         return;
     }
-    const bool dj_started = car.has_double_jumped && !car.internal_has_double_jumped;
+    const bool double_jump_began = car.has_double_jumped && !car.internal_has_double_jumped;
     car.internal_vel = car.vel;
     car.internal_ang_vel = car.ang_vel;
     car.internal_has_jumped = car.has_jumped;
     car.internal_has_double_jumped = car.has_double_jumped;
     car.internal_has_flipped = car.has_flipped;
     car.internal_is_supersonic = car.is_supersonic;
-    AdvanceJumpReportState(car, m, dt, dj_started);
-    const bool on_ground_state = car.is_on_ground && !car.is_jumping && !car.is_flipping;
+    AdvanceJumpReportState(car, m, dt, double_jump_began);
 
-    bool game_jumped = jw ? jw->cal_game_jumped != 0 : car.has_jumped != 0;
-    bool game_double_jumped = jw ? jw->cal_game_double_jumped != 0 : (car.has_double_jumped || car.has_flipped);
-    const int wheels = static_cast<int>(car.wheels_with_contact[0] != 0) + static_cast<int>(car.wheels_with_contact[1] != 0) +
-                       static_cast<int>(car.wheels_with_contact[2] != 0) + static_cast<int>(car.wheels_with_contact[3] != 0);
-    if (jw && !jw->cal_frame_open && !jw->cal_cached_ground && wheels >= 3) game_jumped = game_double_jumped = false;
-    car.has_jumped = game_jumped;
-    car.has_double_jumped = game_double_jumped && car.internal_has_double_jumped;
-    car.has_flipped = game_double_jumped && !car.has_double_jumped;
+    const GameJumpFlags g = GameJumpFlagsOf(car, jw);
+    car.has_jumped = g.jumped ? 1 : 0;
+    car.has_double_jumped = g.air_jumped && car.internal_has_double_jumped ? 1 : 0;
+    car.has_flipped = g.air_jumped && !car.has_double_jumped ? 1 : 0;
+    car.air_state = RlbotAirState(car);
+    DeriveDodgeFields(car, jw, m, dt);
 
-    const uint8_t air_state = car.is_jumping ? 1 : (car.is_flipping ? 3 : (car.is_on_ground ? 0 : (car.double_jump_ticks < 13 ? 2 : 4)));
-    car.air_state = air_state;
-    const bool flipped = car.internal_has_flipped != 0;
-    const bool dodged = (car.rlbotv5_dodged || flipped) && air_state != 0;
-    car.rlbotv5_dodged = dodged ? 1 : 0;
-    const float jump_inactive = jw ? jw->cal_jump_inactive_time : car.air_time_since_jump;
-    const bool dodge_window = !car.is_on_ground && !car.is_jumping && car.game_has_jumped &&
-                              !car.internal_has_double_jumped && !dodged &&
-                              jump_inactive < m.doublejump_max_delay - 0.5f * dt;
-    car.dodge_timeout = dodge_window ? m.doublejump_max_delay - jump_inactive : -1.0f;
-    car.dodge_elapsed = (flipped && !on_ground_state) ? car.internal_flip_time : (dodged ? car.dodge_elapsed + dt : 0.0f);
-    car.dodge_dir_x = car.flip_rel_torque.y;
-    car.dodge_dir_y = -car.flip_rel_torque.x;
-
-    constexpr uint8_t kJumped = 4, kDoubleJumped = 8, kFlipped = 16, kSupersonic = 32, kValid = 128;
+    constexpr uint8_t kValid = 128;
+    uint8_t flags = kValid;
+    if (car.has_jumped) flags |= 4;
+    if (car.has_double_jumped) flags |= 8;
+    if (car.has_flipped) flags |= 16;
+    if (car.is_supersonic) flags |= 32;
     car.reported_vel = car.vel;
     car.reported_ang_vel = car.ang_vel;
-    car.reported_flags = static_cast<uint8_t>(kValid | (car.has_jumped ? kJumped : 0) |
-                                              (car.has_double_jumped ? kDoubleJumped : 0) |
-                                              (car.has_flipped ? kFlipped : 0) | (car.is_supersonic ? kSupersonic : 0));
+    car.reported_flags = flags;
 }
 
 // The in-between ticks of a multi-tick step report nothing; the counters still advance.
@@ -1335,93 +1411,120 @@ RG_HD void CarApplyBoost(CarState& car, const CarControls& controls, const CarPa
     }
 }
 
-// Air control (ApplyAirControl, raycast_wheels_and_contact_prep.hpp:3048-3241) with the tick's inputs: the pitch-lock
-// restart time and the damping angular velocity are the car's own tick-start fields.  Pitch input is locked out while
-// a flip runs and for flip_pitchlock_extra_time after it.  Part one: the input torque about the frame's axes minus
-// the damping, as an angular acceleration (through the game-unit inertia when queued; skipped when queued with an
-// authoritative basis).  Part two, queued only: the same input and damping about the game's quaternion axes in rotator
-// units, each through the inverse of the frame's Bullet rows when one of its components reaches 1e-4.  Then the air
-// throttle (full while boosting) along the game's forward axis.
+// Rotator units per radian-free unit of the game's torque tables: 2 pi / 65536 (one rotator unit), and the same
+// scaled by 1000 (the game's torque-to-acceleration factor).
+constexpr float kRotatorUnit = 2.0f * Pi() / 65536.0f;
+constexpr float kRotatorTorque = (2.0f * Pi() / 65536.0f) * 1000.0f;
+
+// Whether pitch input is locked out: while a flip runs, for flip_pitchlock_extra_time after a flip's torque time,
+// and while a restarted flip clock (cal_pre_pitchlock_restart_time) is inside that extra time.
+RG_HD bool PitchLocked(const CarState& car, const CarParams& m) {
+    if (car.is_flipping) return true;
+    if (car.has_flipped && car.internal_flip_time < m.flip_torque_time + m.flip_pitchlock_extra_time) return true;
+    return car.cal_pre_pitchlock_restart_time >= 0.0f && car.cal_pre_pitchlock_restart_time < m.flip_pitchlock_extra_time;
+}
+
+// A torque written as rates about the game's right, up and forward axes, summed per world component in the game's
+// matrix order (right term, a zero term, up term, forward term): the zero term turns a negative zero into +0 as the
+// game's product does.
+RG_HD Vec3 GameAxesTorque(const Vec3& r, const Vec3& u, const Vec3& f, float about_r, float about_u, float about_f) {
+    const auto one = [&](float rc, float uc, float fc) { return ((about_r * rc + 0.0f) + about_u * uc) + about_f * fc; };
+    return Vec3(one(r.x, u.x, f.x), one(r.y, u.y, f.y), one(r.z, u.z, f.z));
+}
+
+// The damping counterpart: the three rates subtracted from zero in the same axis order.
+RG_HD Vec3 GameAxesDamping(const Vec3& r, const Vec3& u, const Vec3& f, float about_r, float about_u, float about_f) {
+    const auto one = [&](float rc, float uc, float fc) { return ((0.0f - about_r * rc) - about_u * uc) - about_f * fc; };
+    return Vec3(one(r.x, u.x, f.x), one(r.y, u.y, f.y), one(r.z, u.z, f.z));
+}
+
+// A game-space torque through the inverse of the frame's Bullet inertia rows, queued when any component reaches
+// the game's 1e-4 threshold.
+RG_HD bool TorqueAboveGate(const Vec3& t) {
+    constexpr float kGate = 1.0e-4f;
+    return Abs(t.x) >= kGate || Abs(t.y) >= kGate || Abs(t.z) >= kGate;
+}
+
+RG_HD Vec3 ThroughInverseFrame(const CarFrame& frame, const Vec3& t) {
+    Vec3 w0, w1, w2;
+    InverseRows(frame.inv_row0, frame.inv_row1, frame.inv_row2, w0, w1, w2);
+    return MulRows(w0, w1, w2, t);
+}
+
+// An angular acceleration in game units as the queued Bullet torque (through the game-unit inertia).
+RG_HD Vec3 GameAccelAsTorque(const CarState& car, const CarConfig& cfg, const CarParams& m, const Vec3& accel) {
+    Vec3 i0, i1, i2, w0, w1, w2;
+    InvInertiaWorldRowsUu(car, cfg, m.car_mass, i0, i1, i2);
+    InverseRows(i0, i1, i2, w0, w1, w2);
+    return MulRows(w0, w1, w2, accel);
+}
+
+// Air control, part one: the control torque about the frame's pitch (-right), yaw (up) and roll (-forward) axes
+// minus the damping of the tick-start angular velocity, as a game-unit angular acceleration.
+RG_HD Vec3 FrameAirAccel(const CarControls& c, const CarFrame& frame, const Vec3& w, float pitch_gain,
+                         const CarParams& m) {
+    const Vec3 pitch_axis = -frame.right, yaw_axis = frame.up, roll_axis = -frame.fwd;
+    Vec3 input(0.0f, 0.0f, 0.0f);
+    if (c.pitch != 0.0f || c.yaw != 0.0f || c.roll != 0.0f) {
+        const Vec3 p = c.pitch * pitch_axis * pitch_gain * m.car_air_control_pitch;
+        const Vec3 y = c.yaw * yaw_axis * m.car_air_control_yaw;
+        const Vec3 r = c.roll * roll_axis * m.car_air_control_roll;
+        input = (p + y) + r;
+    }
+    const float held_pitch = 1.0f - Min(Abs(c.pitch * pitch_gain), 1.0f);
+    const float held_yaw = 1.0f - Min(Abs(c.yaw), 1.0f);
+    const float damp_pitch = Dot(pitch_axis, w) * m.car_air_damp_pitch * held_pitch;
+    const float damp_yaw = Dot(yaw_axis, w) * m.car_air_damp_yaw * held_yaw;
+    const float damp_roll = Dot(roll_axis, w) * m.car_air_damp_roll;
+    const Vec3 damping = (yaw_axis * damp_yaw + pitch_axis * damp_pitch) + roll_axis * damp_roll;
+    return (input - damping) * kRotatorTorque;
+}
+
+// Air control, part two (queued steps only): the same input and damping about the game's quaternion axes in
+// rotator units, each queued as its own torque through the inverse frame when it passes the gate.
+RG_HD void QueueGameAirTorques(const CarState& car, const CarControls& c, const CarFrame& frame, const Vec3& w,
+                               float pitch_gain, const CarParams& m, CarForceQueue& q) {
+    Vec3 f, r, u;
+    UeQuatAxes(car.rot, f, r, u);
+    const float pitch = c.pitch * pitch_gain;
+    const float rate_r = (m.car_air_control_pitch * -pitch) * kRotatorUnit;
+    const float rate_u = (m.car_air_control_yaw * c.yaw) * kRotatorUnit;
+    const float rate_f = (-m.car_air_control_roll * c.roll) * kRotatorUnit;
+    const Vec3 input = GameAxesTorque(r, u, f, rate_r, rate_u, rate_f) * 1000.0f;
+    const float damp_r = (1.0f - Abs(pitch)) * (Dot(w, r) * (m.car_air_damp_pitch * kRotatorUnit));
+    const float damp_u = (1.0f - Abs(c.yaw)) * (Dot(w, u) * (m.car_air_damp_yaw * kRotatorUnit));
+    const float damp_f = (1.0f * Dot(w, f)) * (m.car_air_damp_roll * kRotatorUnit);
+    const Vec3 damping = GameAxesDamping(r, u, f, damp_r, damp_u, damp_f) * 1000.0f;
+    const bool queue_input = TorqueAboveGate(input), queue_damping = TorqueAboveGate(damping);
+    if (queue_input) QueueCarTorque(q, ThroughInverseFrame(frame, input));
+    if (queue_damping) QueueCarTorque(q, ThroughInverseFrame(frame, damping));
+}
+
+// Air control (the reference's ApplyAirControl, raycast_wheels_and_contact_prep.hpp:3048-3241): with
+// `update_air_control`, the rotation control (part one: applied to the angular velocity when not queued, queued as
+// an acceleration when the basis is not authoritative; part two when queued); with `air_throttle`, the throttle
+// (full while boosting) as a force along the game's forward axis.
 RG_HD void CarApplyAirControl(CarState& car, const CarControls& c, const CarConfig& cfg, const CarParams& m,
                               const CarFrame& frame, bool update_air_control, bool air_throttle, CarForceQueue* q) {
-    if (!update_air_control && !air_throttle) return;
-    const float dt = m.tick_dt;
-    constexpr float kRot = 2.0f * Pi() / 65536.0f;
-    constexpr float kTorqueScale = (2.0f * Pi() / 65536.0f) * 1000.0f;
     if (update_air_control) {
-        float pitch_scale = 1.0f;
-        if (car.is_flipping) {
-            pitch_scale = 0.0f;
-        } else if (car.has_flipped && car.internal_flip_time < (m.flip_torque_time + m.flip_pitchlock_extra_time)) {
-            pitch_scale = 0.0f;
-        } else if (car.cal_pre_pitchlock_restart_time >= 0.0f &&
-                   car.cal_pre_pitchlock_restart_time < m.flip_pitchlock_extra_time) {
-            pitch_scale = 0.0f;
-        }
-        const Vec3 damp_w = car.cal_tick_start_ang_vel;
-        if (!q || !car.basis_authoritative) {
-            const Vec3 pitch_dir = -frame.right;
-            const Vec3 yaw_dir = frame.up;
-            const Vec3 roll_dir = -frame.fwd;
-            Vec3 torque(0.0f, 0.0f, 0.0f);
-            if (c.pitch != 0.0f || c.yaw != 0.0f || c.roll != 0.0f) {
-                torque = ((c.pitch * pitch_dir * pitch_scale * m.car_air_control_pitch) +
-                          (c.yaw * yaw_dir * m.car_air_control_yaw)) +
-                         (c.roll * roll_dir * m.car_air_control_roll);
-            }
-            const float damp_p = Dot(pitch_dir, damp_w) * m.car_air_damp_pitch *
-                                 (1.0f - Min(Abs(c.pitch * pitch_scale), 1.0f));
-            const float damp_y = Dot(yaw_dir, damp_w) * m.car_air_damp_yaw * (1.0f - Min(Abs(c.yaw), 1.0f));
-            const float damp_r = Dot(roll_dir, damp_w) * m.car_air_damp_roll;
-            const Vec3 damping = (yaw_dir * damp_y + pitch_dir * damp_p) + roll_dir * damp_r;
-            const Vec3 accel = (torque - damping) * kTorqueScale;
-            if (q) {
-                Vec3 i0, i1, i2, w0, w1, w2;
-                InvInertiaWorldRowsUu(car, cfg, m.car_mass, i0, i1, i2);
-                InverseRows(i0, i1, i2, w0, w1, w2);
-                q->ang_accel = q->ang_accel + MulRows(w0, w1, w2, accel);
-            } else {
-                car.ang_vel += accel * dt;
-            }
-        }
-        if (q) {
-            Vec3 f, r, u;
-            UeQuatAxes(car.rot, f, r, u);
-            const float in_pitch = c.pitch * pitch_scale;
-            const float a_p = (m.car_air_control_pitch * -in_pitch) * kRot;
-            const float a_y = (m.car_air_control_yaw * c.yaw) * kRot;
-            const float a_r = (-m.car_air_control_roll * c.roll) * kRot;
-            const Vec3 t_in((((a_p * r.x + 0.0f) + a_y * u.x) + a_r * f.x) * 1000.0f,
-                            (((a_p * r.y + 0.0f) + a_y * u.y) + a_r * f.y) * 1000.0f,
-                            (((a_p * r.z + 0.0f) + a_y * u.z) + a_r * f.z) * 1000.0f);
-            const float d_p = (1.0f - Abs(in_pitch)) * (Dot(damp_w, r) * (m.car_air_damp_pitch * kRot));
-            const float d_y = (1.0f - Abs(c.yaw)) * (Dot(damp_w, u) * (m.car_air_damp_yaw * kRot));
-            const float d_r = (1.0f * Dot(damp_w, f)) * (m.car_air_damp_roll * kRot);
-            const Vec3 t_damp((((0.0f - d_p * r.x) - d_y * u.x) - d_r * f.x) * 1000.0f,
-                              (((0.0f - d_p * r.y) - d_y * u.y) - d_r * f.y) * 1000.0f,
-                              (((0.0f - d_p * r.z) - d_y * u.z) - d_r * f.z) * 1000.0f);
-            constexpr float kGate = 1.0e-4f;
-            const bool apply_in = Abs(t_in.x) >= kGate || Abs(t_in.y) >= kGate || Abs(t_in.z) >= kGate;
-            const bool apply_damp = Abs(t_damp.x) >= kGate || Abs(t_damp.y) >= kGate || Abs(t_damp.z) >= kGate;
-            if (apply_in || apply_damp) {
-                Vec3 w0, w1, w2;
-                InverseRows(frame.inv_row0, frame.inv_row1, frame.inv_row2, w0, w1, w2);
-                if (apply_in) QueueCarTorque(*q, MulRows(w0, w1, w2, t_in));
-                if (apply_damp) QueueCarTorque(*q, MulRows(w0, w1, w2, t_damp));
-            }
+        const float pitch_gain = PitchLocked(car, m) ? 0.0f : 1.0f;
+        const Vec3 w = car.cal_tick_start_ang_vel;
+        if (q == nullptr) {
+            car.ang_vel += FrameAirAccel(c, frame, w, pitch_gain, m) * m.tick_dt;
+        } else {
+            if (!car.basis_authoritative) q->ang_accel = q->ang_accel + GameAccelAsTorque(car, cfg, m, FrameAirAccel(c, frame, w, pitch_gain, m));
+            QueueGameAirTorques(car, c, frame, w, pitch_gain, m, *q);
         }
     }
     const float throttle = car.is_boosting ? 1.0f : c.throttle;
-    if (throttle != 0.0f && air_throttle) {
-        Vec3 f, r, u;
-        UeQuatAxes(car.rot, f, r, u);
-        const float force = m.car_air_throttle_accel * m.car_mass;
-        const Vec3 force_bt = ((f * force) * throttle) * (1.0f / 50.0f);
-        if (q) {
-            QueueCarForce(*q, force_bt);
-        } else {
-            ApplyCarCentralForce(car, m.car_mass, force_bt * 50.0f, dt);
-        }
+    if (!air_throttle || throttle == 0.0f) return;
+    Vec3 f, r, u;
+    UeQuatAxes(car.rot, f, r, u);
+    const Vec3 force_bt = ((f * (m.car_air_throttle_accel * m.car_mass)) * throttle) * (1.0f / 50.0f);
+    if (q != nullptr) {
+        QueueCarForce(*q, force_bt);
+    } else {
+        ApplyCarCentralForce(car, m.car_mass, force_bt * 50.0f, m.tick_dt);
     }
 }
 
@@ -1461,48 +1564,46 @@ RG_HD float DodgeCancelPitch(const CarState& car, const CarControls& c, float ra
     return -QuantizeAxisInput(-p);
 }
 
-// Flip torque (ApplyFlipTorque, raycast_wheels_and_contact_prep.hpp:3432-3484): a flip ends when its torque time is
-// over; a running flip turns the car by its relative torque (rounded to whole units, pitch reduced by a cancelling
-// pitch input) as an angular acceleration, and, queued, as a torque about the game's quaternion axes through the
-// inverse of the frame's Bullet rows when a component reaches 1e-4.  The vehicle update defers that torque until
-// after air control (`deferred` receives it); without `deferred` it is queued at once.  raw_pitch: the control's pitch before quantization, or 2 for none.
+// The flip's local torque: the relative torque scaled by the flip torque constants and rounded to whole units, its
+// pitch part reduced while a pitch input against it cancels the dodge.
+RG_HD Vec3 FlipLocalTorque(const CarState& car, const CarControls& c, const CarParams& m, float raw_pitch) {
+    const Vec3 rel = car.flip_rel_torque;
+    float pitch = rintf(rel.y * m.flip_torque_y);
+    const float cancel = DodgeCancelPitch(car, c, raw_pitch);
+    const bool cancels = rel.y != 0.0f && cancel != 0.0f && SignOf(rel.y) == SignOf(cancel);
+    if (cancels) pitch *= 1.0f - Min(Abs(cancel), 1.0f);
+    return Vec3(rintf(rel.x * m.flip_torque_x), pitch, 0.0f);
+}
+
+// Flip torque (the reference's ApplyFlipTorque, raycast_wheels_and_contact_prep.hpp:3432-3484).  The flip ends once
+// its torque time is over.  While it runs, its local torque turns the car: straight into the angular velocity when
+// not queued; queued, as an acceleration (basis not authoritative) and as the game-space torque about the
+// quaternion axes through the inverse frame when it passes the gate.  The vehicle update queues that torque after
+// air control: `deferred` receives it then; without `deferred` a nonzero torque is queued at once.  raw_pitch: the
+// control's pitch before quantization, or 2 for none.
 RG_HD void CarApplyFlipTorque(CarState& car, const CarControls& c, const CarConfig& cfg, const CarParams& m,
                               const CarFrame& frame, float raw_pitch, CarForceQueue* q, Vec3* deferred = nullptr) {
-    if (car.is_flipping) car.is_flipping = (car.has_flipped && car.internal_flip_time < m.flip_torque_time) ? 1 : 0;
+    if (car.is_flipping) car.is_flipping = car.has_flipped && car.internal_flip_time < m.flip_torque_time ? 1 : 0;
     if (!car.is_flipping) return;
-    const Vec3 rel = car.flip_rel_torque;
-    float torque_y = rintf(rel.y * m.flip_torque_y);
-    const float cancel = DodgeCancelPitch(car, c, raw_pitch);
-    if (rel.y != 0.0f && cancel != 0.0f && SignOf(rel.y) == SignOf(cancel)) {
-        torque_y *= (1.0f - Min(Abs(cancel), 1.0f));
-    }
-    const Vec3 local(rintf(rel.x * m.flip_torque_x), torque_y, 0.0f);
+    const Vec3 local = FlipLocalTorque(car, c, m, raw_pitch);
     const Vec3 world = RotateCarVector(car, local);
-    if (!q) {
+    if (q == nullptr) {
         car.ang_vel += world * m.tick_dt;
         return;
     }
-    if (!car.basis_authoritative) {
-        Vec3 i0, i1, i2, w0, w1, w2;
-        InvInertiaWorldRowsUu(car, cfg, m.car_mass, i0, i1, i2);
-        InverseRows(i0, i1, i2, w0, w1, w2);
-        q->ang_accel = q->ang_accel + MulRows(w0, w1, w2, world);
-    }
+    if (!car.basis_authoritative) q->ang_accel = q->ang_accel + GameAccelAsTorque(car, cfg, m, world);
     Vec3 f, r, u;
     UeQuatAxes(car.rot, f, r, u);
-    const Vec3 tw(((local.y * r.x + local.x * f.x) + local.z * u.x) + 0.0f * 0.0f,
-                  ((local.y * r.y + local.x * f.y) + local.z * u.y) + 0.0f * 0.0f,
-                  ((local.y * r.z + local.x * f.z) + local.z * u.z) + 0.0f * 0.0f);
-    constexpr float kGate = 1.0e-4f;
-    if (Abs(tw.x) >= kGate || Abs(tw.y) >= kGate || Abs(tw.z) >= kGate) {
-        Vec3 w0, w1, w2;
-        InverseRows(frame.inv_row0, frame.inv_row1, frame.inv_row2, w0, w1, w2);
-        const Vec3 t = MulRows(w0, w1, w2, tw);
-        if (deferred) {
-            *deferred = t;
-        } else if (t.x != 0.0f || t.y != 0.0f || t.z != 0.0f) {
-            QueueCarTorque(*q, t);
-        }
+    // Game order: the pitch (right) term, then roll (forward), then the zero yaw (up) term, then a +0 that clears a
+    // negative zero.
+    const auto one = [&](float rc, float fc, float uc) { return ((local.y * rc + local.x * fc) + local.z * uc) + 0.0f * 0.0f; };
+    const Vec3 game(one(r.x, f.x, u.x), one(r.y, f.y, u.y), one(r.z, f.z, u.z));
+    if (!TorqueAboveGate(game)) return;
+    const Vec3 t = ThroughInverseFrame(frame, game);
+    if (deferred != nullptr) {
+        *deferred = t;
+    } else if (t.x != 0.0f || t.y != 0.0f || t.z != 0.0f) {
+        QueueCarTorque(*q, t);
     }
 }
 

@@ -685,13 +685,13 @@ RG_HD void CarVehicleUpdate(CarState& car, const CarControls& raw, const CarConf
     ws.pre_solve_basis_right = car.tick_start_basis_right;
     ws.pre_solve_basis_up = car.tick_start_basis_up;
     ws.pre_solve_basis_authoritative = 1;
-    const bool set_v3_tick = (pre & kCalPreSetV3Tick) != 0;
-    const uint8_t flipping_at_start = (pre & kCalPreFlippingAtStart) ? 1 : 0;
-    const bool flip_started = (pre & kCalPreFlipStarted) != 0;
-    const bool air_jump_started = (pre & kCalPreAirJumpStarted) != 0;
-    const uint8_t was_jumping = (pre & kCalPreWasJumping) ? 1 : 0;
-    const uint8_t air_jump_used_before = (pre & kCalPreAirJumpUsedBefore) ? 1 : 0;
-    const bool flip_vel_changed = (pre & kCalPreFlipVelChanged) != 0;
+    const bool set_v3_tick = (pre & kPreSupersonicFromSet) != 0;
+    const uint8_t flipping_at_start = (pre & kPreFlippingAtStart) ? 1 : 0;
+    const bool flip_started = (pre & kPreFlipStarted) != 0;
+    const bool air_jump_started = (pre & kPreGroundJumpStarted) != 0;
+    const uint8_t was_jumping = (pre & kPreWasJumping) ? 1 : 0;
+    const uint8_t air_jump_used_before = (pre & kPreHadAirJump) ? 1 : 0;
+    const bool flip_vel_changed = (pre & kPreFlipMovedCar) != 0;
     (void)flip_vel_changed;
     const int prev_contacts = (car.wheels_with_contact[0] != 0) + (car.wheels_with_contact[1] != 0) +
                               (car.wheels_with_contact[2] != 0) + (car.wheels_with_contact[3] != 0);
