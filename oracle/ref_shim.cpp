@@ -629,7 +629,8 @@ int rgref_workload_ball_states(uint64_t seed, long first_arena, long count, doub
 }
 
 // The reference's float kernels (include/rocketgpu/detail/math.hpp:36-492, and its sinf/cosf wrappers that use the
-// host libm beyond the UCRT range): fn 0 sin, 1 cos, 2 atan2(x, y), 3 pow(x, y), 4 acos.
+// host libm beyond the UCRT range): fn 0 sin, 1 cos, 2 atan2(x, y), 3 pow(x, y), 4 acos, 5 SafeAtan2(x, y) (the
+// double atan2 rounded to float, math.hpp:1546-1552).
 int rgref_math(int fn, const float* x, const float* y, float* out, long n) {
     for (long i = 0; i < n; ++i) {
         switch (fn) {
@@ -638,6 +639,7 @@ int rgref_math(int fn, const float* x, const float* y, float* out, long n) {
         case 2: out[i] = RocketGpuUcrtAtan2f(x[i], y[i]); break;
         case 3: out[i] = RocketGpuUcrtPowf(x[i], y[i]); break;
         case 4: out[i] = RocketGpuUcrtAcosf(x[i]); break;
+        case 5: out[i] = rocketgpu::detail::SafeAtan2(x[i], y[i]); break;
         default: return -1;
         }
     }

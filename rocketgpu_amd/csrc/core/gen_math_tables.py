@@ -8,6 +8,8 @@ toward zero:
   exp2_tab[k]  = 2 ** (k / 64)                 k = 0..63    (the powf exp step)
   inv_c[k]     = 512 / (256 + k)               k = 0..256   (1 / c,  c = 1/2 + k/512, the powf log step)
   log_c[k]     = ln((256 + k) / 256)           k = 0..256   (ln(2 c))
+  two_over_pi[k] = bits 8k - 23 .. 8k + 8 of the binary fraction of 2 / pi (32-bit windows)  k = 0..23
+                 (the large-argument reduction of sinf / cosf; pi from Machin's formula)
 
 Run:  python rocketgpu_amd/csrc/core/gen_math_tables.py > rocketgpu_amd/csrc/core/rg_math_tables.inc
 tests/test_math_tables.py re-derives the tables and checks the committed file is what this script prints.
@@ -54,6 +56,16 @@ def fmt(values, per_line=3):
     return "\n".join(out)
 
 
+def d_pi() -> Decimal:
+    # Machin: pi = 16 atan(1/5) - 4 atan(1/239).
+    return 16 * d_atan(Decimal(1) / 5) - 4 * d_atan(Decimal(1) / 239)
+
+
+def two_over_pi_windows():
+    frac = Fraction(2) / Fraction(d_pi())  # 2 / pi to ~260 bits
+    return [int(frac * 2 ** (8 * k + 8)) % 2 ** 32 for k in range(24)]
+
+
 def tables():
     atan_tab = [to_double_toward_zero(d_atan(Decimal(k + 16) / 256)) for k in range(241)]
     exp2_tab = [to_double((Decimal(k) / 64 * Decimal(2).ln()).exp()) for k in range(64)]
@@ -70,6 +82,9 @@ def main():
     print(f"RG_MATH_TABLE(double, kExp2Tab, 64) = {{\n{fmt(exp2_tab)}\n}};")
     print(f"RG_MATH_TABLE(double, kInvCTab, 257) = {{\n{fmt(inv_c)}\n}};")
     print(f"RG_MATH_TABLE(double, kLogCTab, 257) = {{\n{fmt(log_c)}\n}};")
+    w = two_over_pi_windows()
+    rows = ["    " + ", ".join(f"0x{v:08x}u" for v in w[i:i + 6]) + "," for i in range(0, 24, 6)]
+    print("RG_MATH_TABLE(uint32_t, kTwoOverPiBits, 24) = {\n" + "\n".join(rows) + "\n};")
 
 
 if __name__ == "__main__":

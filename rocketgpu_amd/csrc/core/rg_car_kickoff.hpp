@@ -55,7 +55,7 @@ RG_HD void PlaceKickoffCar(CarState& car, const CarSpawnRules& rules, int team, 
         const float back = 250.0f * static_cast<float>((nth - 5) / 4);
         pos = rules.respawn_pos[team][r];
         yaw = rules.respawn_yaw[team][r];
-        pos = pos - Vec3(UcrtCosf(yaw), UcrtSinf(yaw), 0.0f) * back;
+        pos = pos - Vec3(GameCosf(yaw), GameSinf(yaw), 0.0f) * back;
     }
     PlaceCarAtWorldSpawn(car, pos, yaw, spawn_z, boost);
     if (game_spawn && nth < 5) {

@@ -821,7 +821,7 @@ RG_HD Quat FromYaw(float yaw) {
         if (Abs(yaw - e.angle) <= 1e-6f) return Quat(e.w, 0.0f, 0.0f, e.z);
     }
     const float half = 0.5f * yaw;
-    return Quat(UcrtCosf(half), 0.0f, 0.0f, UcrtSinf(half));
+    return Quat(GameCosf(half), 0.0f, 0.0f, GameSinf(half));
 }
 
 // The game's quaternion of a rotation matrix m (rows: the rotated x, y, z axes), UE3's FQuat(FMatrix): the trace

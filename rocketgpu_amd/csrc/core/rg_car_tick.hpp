@@ -424,8 +424,8 @@ RG_HD Vec3 SteeredRight(const WheelRay& w, float steer) {
     const Vec3 up = -w.down;
     const Vec3 right = w.axle;
     const float half = steer * 0.5f;
-    const float s = (half == 0.0f && LengthSq(up) > 0.0f) ? half : (UcrtSinf(half) / Length(up));
-    const float c = half == 0.0f ? 1.0f : UcrtCosf(half);
+    const float s = (half == 0.0f && LengthSq(up) > 0.0f) ? half : (GameSinf(half) / Length(up));
+    const float c = half == 0.0f ? 1.0f : GameCosf(half);
     const Quat q(c, up.x * s, up.y * s, up.z * s);
     Vec3 c0, c1, c2;
     GameQuatToColumns(q, c0, c1, c2);
@@ -965,10 +965,10 @@ RG_HD void IntegrateBasis(Vec3& fwd, Vec3& right, Vec3& up, Quat& q_out, const V
     if (angle < 0.001f) {
         f = dt * 0.5f - ((((dt * dt) * dt) * 0.020833333333f) * angle) * angle;
     } else {
-        f = UcrtSinf((angle * 0.5f) * dt) / angle;
+        f = GameSinf((angle * 0.5f) * dt) / angle;
     }
     const float ax = w.x * f, ay = w.y * f, az = w.z * f;
-    const float c = UcrtCosf((angle * dt) * 0.5f);
+    const float c = GameCosf((angle * dt) * 0.5f);
     const Quat o = GameMatrixToQuat(fwd, right, up);
     const Quat p(((c * o.w - ax * o.x) - ay * o.y) - az * o.z, ((ax * o.w + c * o.x) + ay * o.z) - az * o.y,
                  ((c * o.y + ay * o.w) + az * o.x) - ax * o.z, ((c * o.z + az * o.w) + ax * o.y) - ay * o.x);

@@ -457,11 +457,96 @@ RG_HD float UcrtAcosf(float x) {
     return fmaf(r, s + s, c + c) + (hi + hi);
 }
 
+// ------------------------------------------------------------------------------------------------------------
+// sinf / cosf over the whole float range, as the reference computes them (RocketGpuSinf / RocketGpuCosf,
+// math.hpp:494-586): the UCRT kernels below |x| = 0x4b800456 (sinf) / 0x4f490fdb (cosf); above, the host C
+// library's sinf / cosf, which on the reference's platform is glibc's (sysdeps/ieee754/flt-32/s_sinf.c,
+// s_cosf.c, sincosf.h and sincosf_data.c, glibc 2.28 and later, in the FMA build glibc selects on FMA hardware).
+// That algorithm, restated: reduce |x| modulo pi/2 exactly with a 96-bit window of 2/pi picked by the exponent
+// (reduce_large), then evaluate an even (cos) or odd (sin) double polynomial of the reduced argument, the
+// quadrant choosing which and its sign.  Infinities and NaNs give a NaN.
+
+// glibc's __sincosf_table: the quadrant signs and the polynomial coefficients for sin (s1..s3) and cos (c0..c4),
+// entry 1 with the cos coefficients negated (quadrants 2 and 3).
+struct LibmSincosCoeffs {
+    double sign[4];
+    double c0, c1, c2, c3, c4;
+    double s1, s2, s3;
+};
+struct LibmSincosTable {
+    static constexpr LibmSincosCoeffs k[2] = {
+        {{1.0, -1.0, -1.0, 1.0}, 0x1p0, -0x1.ffffffd0c621cp-2, 0x1.55553e1068f19p-5, -0x1.6c087e89a359dp-10,
+         0x1.99343027bf8c3p-16, -0x1.555545995a603p-3, 0x1.1107605230bc4p-7, -0x1.994eb3774cf24p-13},
+        {{1.0, -1.0, -1.0, 1.0}, -0x1p0, 0x1.ffffffd0c621cp-2, -0x1.55553e1068f19p-5, 0x1.6c087e89a359dp-10,
+         -0x1.99343027bf8c3p-16, -0x1.555545995a603p-3, 0x1.1107605230bc4p-7, -0x1.994eb3774cf24p-13}};
+};
+RG_HD const LibmSincosCoeffs& LibmSincos(int negated_cos) { return LibmSincosTable::k[negated_cos]; }
+
+// x in [-pi/4, pi/4], x2 = x * x: sin x for even n, cos x for odd n (glibc's sinf_poly with FMA).
+RG_HD float LibmSincosPoly(double x, double x2, const LibmSincosCoeffs& p, int n) {
+    if ((n & 1) == 0) {
+        const double x3 = x * x2;
+        const double tail = fma(x2, p.s3, p.s2);
+        const double x7 = x3 * x2;
+        const double head = fma(x3, p.s1, x);
+        return static_cast<float>(fma(x7, tail, head));
+    }
+    const double x4 = x2 * x2;
+    const double tail = fma(x2, p.c4, p.c3);
+    const double head = fma(x2, p.c1, p.c0);
+    const double x6 = x4 * x2;
+    const double mid = fma(x4, p.c2, head);
+    return static_cast<float>(fma(x6, tail, mid));
+}
+
+// |x| modulo pi/2 for |x| >= 2^24 (glibc's reduce_large): the 24-bit significand times the 96 bits of 2/pi that
+// matter at this exponent gives the quadrant n in the top bits and the reduced argument, scaled by pi * 2^-63.
+RG_HD double LibmReduceLarge(uint32_t bits, int* quadrant) {
+    const uint32_t* w = &MathTables::kTwoOverPiBits[(bits >> 26) & 15];
+    const int shift = static_cast<int>((bits >> 23) & 7);
+    const uint32_t m = ((bits & 0xffffffu) | 0x800000u) << shift;
+    uint64_t hi = static_cast<uint32_t>(m * w[0]);
+    const uint64_t mid = static_cast<uint64_t>(m) * w[4];
+    const uint64_t lo = static_cast<uint64_t>(m) * w[8];
+    hi = (lo >> 32) | (hi << 32);
+    hi += mid;
+    const uint64_t n = (hi + (1ull << 61)) >> 62;
+    hi -= n << 62;
+    *quadrant = static_cast<int>(n);
+    return static_cast<double>(static_cast<int64_t>(hi)) * 0x1.921fb54442d18p-62;
+}
+
+RG_HD float GameSinf(float x) {
+    const uint32_t bits = FloatBits(x);
+    if ((bits & 0x7fffffffu) < kUcrtSinfLimitBits) return UcrtSinf(x);
+    if ((bits & 0x7f800000u) == 0x7f800000u) return x - x;  // inf or NaN: NaN
+    int n = 0;
+    const double r = LibmReduceLarge(bits, &n);
+    const int q = n + static_cast<int>(bits >> 31);
+    return LibmSincosPoly(r * LibmSincos(0).sign[q & 3], r * r, LibmSincos((q & 2) ? 1 : 0), n);
+}
+
+RG_HD float GameCosf(float x) {
+    const uint32_t bits = FloatBits(x);
+    if ((bits & 0x7fffffffu) < kUcrtCosfLimitBits) return UcrtCosf(x);
+    if ((bits & 0x7f800000u) == 0x7f800000u) return x - x;
+    int n = 0;
+    const double r = LibmReduceLarge(bits, &n);
+    return LibmSincosPoly(r * LibmSincos(0).sign[n & 3], r * r, LibmSincos((n & 2) ? 1 : 0), n ^ 1);
+}
+
+// atan2 in double rounded to float (SafeAtan2, math.hpp:1546-1552, the car-car culling's angle).  The device's
+// double atan2 and the host C library's agree to within an ulp of double, so the float results agree except when a
+// double result lies within that ulp of a float rounding boundary.
+RG_HD float SafeAtan2(float y, float x) {
+    return static_cast<float>(atan2(static_cast<double>(y), static_cast<double>(x)));
+}
+
 // Rotation by axis/angle as Bullet's btQuaternion::setRotation (math.hpp:876-880).
 RG_HD Quat QuatFromAxisAngle(const Vec3& axis, float angle) {
     const float d = Length(axis);
-    const float s = UcrtSinf(angle * 0.5f) / d;
-    return Quat(UcrtCosf(angle * 0.5f), axis.x * s, axis.y * s, axis.z * s);
+    const float s = GameSinf(angle * 0.5f) / d;
+    return Quat(GameCosf(angle * 0.5f), axis.x * s, axis.y * s, axis.z * s);
 }
 
 // q * v * conj(q) in Bullet's btQuatRotate operation order (math.hpp:882-895).

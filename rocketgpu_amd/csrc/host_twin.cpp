@@ -442,11 +442,12 @@ long rgt_bvh_containment_violations(void* p) {
 int rgt_math(int fn, const float* x, const float* y, float* out, long n) {
     for (long i = 0; i < n; ++i) {
         switch (fn) {
-        case 0: out[i] = core::UcrtSinf(x[i]); break;
-        case 1: out[i] = core::UcrtCosf(x[i]); break;
+        case 0: out[i] = core::GameSinf(x[i]); break;
+        case 1: out[i] = core::GameCosf(x[i]); break;
         case 2: out[i] = core::UcrtAtan2f(x[i], y[i]); break;
         case 3: out[i] = core::UcrtPowf(x[i], y[i]); break;
         case 4: out[i] = core::UcrtAcosf(x[i]); break;
+        case 5: out[i] = core::SafeAtan2(x[i], y[i]); break;
         default: return -1;
         }
     }

@@ -77,9 +77,77 @@ __global__ void __launch_bounds__(64) CarForcePhaseKernel(CarState* cars, const 
     if (queues) queues[k] = q;
 }
 
+// The step core's math functions on the device: fn 0 GameSinf, 1 GameCosf, 2 UcrtAtan2f(x, y), 3 UcrtPowf(x, y),
+// 4 UcrtAcosf, 5 SafeAtan2(x, y).
+__device__ inline float MathFn(int fn, float x, float y) {
+    switch (fn) {
+    case 0: return core::GameSinf(x);
+    case 1: return core::GameCosf(x);
+    case 2: return core::UcrtAtan2f(x, y);
+    case 3: return core::UcrtPowf(x, y);
+    case 4: return core::UcrtAcosf(x);
+    default: return core::SafeAtan2(x, y);
+    }
+}
+
+// splitmix64's output function: the mixing of the math checksums.
+__device__ inline uint64_t Mix64(uint64_t z) {
+    z += 0x9e3779b97f4a7c15ull;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+// Sum over inputs i in [first, first + count) (the float with bits i) of Mix64(i << 32 | result bits), NaN results
+// counted as 0x7fc00000: an order-free checksum of a one-argument function over a range of bit patterns.
+__global__ void __launch_bounds__(256) MathChecksumKernel(int fn, uint64_t first, uint64_t count, uint64_t* sum) {
+    uint64_t acc = 0;
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    for (uint64_t k = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; k < count; k += stride) {
+        const uint64_t i = first + k;
+        const float r = MathFn(fn, core::BitsFloat(static_cast<uint32_t>(i)), 0.0f);
+        const uint32_t b = r != r ? 0x7fc00000u : core::FloatBits(r);
+        acc += Mix64((i << 32) | b);
+    }
+    atomicAdd(reinterpret_cast<unsigned long long*>(sum), static_cast<unsigned long long>(acc));
+}
+
+__global__ void __launch_bounds__(256) MathEvalKernel(int fn, const float* x, const float* y, float* out, int64_t n) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = MathFn(fn, x[i], y[i]);
+}
+
 }  // namespace
 
 extern "C" {
+
+// The checksum of MathChecksumKernel for fn 0, 1 or 4 over bit patterns [first, first + count), into *host_sum.
+// Returns a hipError_t.
+int rgp_math_checksum(int32_t fn, uint64_t first, uint64_t count, uint64_t* host_sum) {
+    if ((fn != 0 && fn != 1 && fn != 4) || host_sum == nullptr || first + count > (1ull << 32)) {
+        return static_cast<int>(hipErrorInvalidValue);
+    }
+    uint64_t* d = nullptr;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&d), sizeof(uint64_t));
+    if (e != hipSuccess) return static_cast<int>(e);
+    e = hipMemset(d, 0, sizeof(uint64_t));
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(MathChecksumKernel, dim3(65536), dim3(256), 0, nullptr, fn, first, count, d);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpy(host_sum, d, sizeof(uint64_t), hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    return static_cast<int>(e);
+}
+
+// out[i] = fn(x[i], y[i]) for i < n (device pointers).  Returns a hipError_t.
+int rgp_math_eval(int32_t fn, const float* x, const float* y, float* out, int64_t n, void* stream) {
+    if (fn < 0 || fn > 5) return static_cast<int>(hipErrorInvalidValue);
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(MathEvalKernel, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0,
+                       static_cast<hipStream_t>(stream), fn, x, y, out, n);
+    return static_cast<int>(hipGetLastError());
+}
 
 // One force phase (0: jump hold, 1: boost, 2: air control, 3: flip torque) on n cars in place (device pointers;
 // configs: one CarConfig per car, read by phases 2 and 3; queues: one force queue per car, or null); flags: for air
