@@ -54,6 +54,86 @@ typedef struct rg_sim_config {
     uint64_t respawn_rng_seed;
 } rg_sim_config;
 
+/* MutatorConfig (rocketgpu.h:708-840) as a flat record: the reference's physics and rules fields in its order
+ * (its pointers, caches and calibration fields omitted), floats then flags.  rg_mutator_config_defaults fills the
+ * reference's defaults.  This build honours the fields its step reads; a config that changes any other field from
+ * its default is refused (RG_EINVAL, the field named by rg_last_error). */
+typedef struct rg_mutator_config {
+    float gravity_z;
+    float ball_radius;
+    float ball_mass;
+    float car_mass;
+    float ball_drag;
+    float ball_friction;
+    float ball_restitution;
+    float ball_max_speed;
+    float ball_max_ang_speed;
+    float car_max_speed;
+    float car_max_ang_speed;
+    float car_air_throttle_accel;
+    float boost_max;
+    float boost_used_per_second;
+    float boost_min_time;
+    float boost_accel_ground;
+    float boost_accel_air;
+    float boost_spawn_amount;
+    float recharge_boost_per_second;
+    float recharge_boost_delay;
+    float jump_accel;
+    float jump_immediate_speed;
+    float jump_min_time;
+    float jump_reset_time_pad;
+    float jump_max_time;
+    float doublejump_max_delay;
+    float flip_torque_time;
+    float flip_torque_min_time;
+    float flip_pitchlock_time;
+    float flip_pitchlock_extra_time;
+    float flip_initial_vel_scale;
+    float flip_torque_x;
+    float flip_torque_y;
+    float flip_forward_impulse_max_speed_scale;
+    float flip_side_impulse_max_speed_scale;
+    float flip_backward_impulse_max_speed_scale;
+    float flip_backward_impulse_scale_x;
+    float supersonic_start_speed;
+    float supersonic_maintain_min_speed;
+    float supersonic_maintain_max_time;
+    float powerslide_rise_rate;
+    float powerslide_fall_rate;
+    float arena_friction;
+    float arena_restitution;
+    float sticky_force;
+    float ball_car_extra_impulse_z_scale;
+    float ball_car_extra_impulse_forward_scale;
+    float ball_car_extra_impulse_max_delta_vel;
+    float ball_hit_extra_force_scale;
+    float bump_force_scale;
+    float goal_base_threshold_y;
+    float bump_cooldown_time;
+    float bump_min_forward_dist;
+    float demo_respawn_time;
+    float boost_pad_cooldown_small;
+    float boost_pad_cooldown_big;
+    float boost_pad_frame_frac;
+    float frame_start_dt;
+    float car_air_control_pitch;
+    float car_air_control_yaw;
+    float car_air_control_roll;
+    float car_air_damp_pitch;
+    float car_air_damp_yaw;
+    float car_air_damp_roll;
+    uint8_t recharge_boost_enabled;
+    uint8_t unlimited_flips;
+    uint8_t unlimited_double_jumps;
+    uint8_t enable_team_demos;
+    uint8_t countdown_on_frame_start;
+    uint8_t game_mode;
+    uint8_t ball_no_rot;
+    uint8_t kickoff_settled;
+    uint8_t reserved[8];
+} rg_mutator_config;
+
 /* A custom arena (ArenaLayout, rocketgpu.h:943-959) given as flat arrays: mesh_count meshes, mesh k owning
  * vertex_counts[k] vertices (xyz floats, uu) and index_counts[k] indices, concatenated in mesh order.  Goals: point,
  * normal (facing into the goal), min, max for team 0 then team 1.  Spawns: kickoff [team][5] and respawn [team][4]
@@ -99,6 +179,14 @@ void rg_batch_destroy(rg_batch* batch);
 int64_t rg_batch_env_count(const rg_batch* batch);
 int64_t rg_batch_car_capacity(const rg_batch* batch);
 
+/* The reference's MutatorConfig defaults. */
+void rg_mutator_config_defaults(rg_mutator_config* out);
+/* set_mutator_config(config)  (rocketgpu.h:1003): the batch's physics from the next call on; settled kickoff rows are
+ * recorded again for the new config.  Also sets the settled-kickoff switch (kickoff_settled). */
+int rg_batch_set_mutator_config(rg_batch* batch, const rg_mutator_config* config);
+/* The batch's current MutatorConfig. */
+int rg_batch_get_mutator_config(const rg_batch* batch, rg_mutator_config* out);
+
 /* set_env_car_count(host_counts, count, stream)  (rocketgpu.h:1023). */
 int rg_batch_set_env_car_count(rg_batch* batch, const int32_t* host_counts, size_t count, void* stream);
 
@@ -114,6 +202,12 @@ int rg_batch_upload_mode(rg_batch* batch, int array_id, const void* host_src, si
                          void* stream);
 /* download_*_states (rocketgpu.h:1036-1053); synchronous like the reference's non-async downloads. */
 int rg_batch_download(rg_batch* batch, int array_id, void* host_dst, size_t count, void* stream);
+/* download_*_async (rocketgpu.h:1050-1053): the copy is queued on `stream` and not waited for; the caller
+ * synchronizes the stream before reading host_dst. */
+int rg_batch_download_async(rg_batch* batch, int array_id, void* host_dst, size_t count, void* stream);
+/* copy_state_from(source, stream)  (rocketgpu.h:1045): every state array of `source`, a batch of the same env count,
+ * car capacity, arena and physics, copied on the device (wheel runtime and internal state included). */
+int rg_batch_copy_state_from(rg_batch* batch, const rg_batch* source, void* stream);
 
 /* boost_pad_capacity()  (rocketgpu.h:1019): env_count x the layout's pad count. */
 int64_t rg_batch_boost_pad_capacity(const rg_batch* batch);

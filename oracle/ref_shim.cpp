@@ -371,6 +371,29 @@ int rgref_set_car_presets(void* p, const uint8_t* presets, long n) {
     } catch (const std::exception& e) { return Fail(e); }
 }
 
+// set_mutator_config with the batch's current MutatorConfig and n float fields replaced by name (gravity_z,
+// ball_drag, ball_friction, ball_restitution, car_mass, jump_accel, boost_accel_ground, car_max_speed,
+// flip_torque_x, car_air_control_pitch).  Returns -1 for an unknown name.
+int rgref_set_mutator_floats(void* p, const char* const* names, const float* values, int n) {
+    try {
+        RefSim& h = *static_cast<RefSim*>(p);
+        MutatorConfig m = h.sim->config().mutator;
+        for (int i = 0; i < n; ++i) {
+            const std::string k = names[i];
+            float* f = k == "gravity_z" ? &m.gravity_z : k == "ball_drag" ? &m.ball_drag
+                     : k == "ball_friction" ? &m.ball_friction : k == "ball_restitution" ? &m.ball_restitution
+                     : k == "car_mass" ? &m.car_mass : k == "jump_accel" ? &m.jump_accel
+                     : k == "boost_accel_ground" ? &m.boost_accel_ground : k == "car_max_speed" ? &m.car_max_speed
+                     : k == "flip_torque_x" ? &m.flip_torque_x : k == "car_air_control_pitch" ? &m.car_air_control_pitch
+                     : nullptr;
+            if (f == nullptr) throw std::invalid_argument("unknown mutator field " + k);
+            *f = values[i];
+        }
+        h.sim->set_mutator_config(m);
+        return 0;
+    } catch (const std::exception& e) { return Fail(e); }
+}
+
 int rgref_reset_to_kickoff(void* p, uint64_t seed) {
     try { static_cast<RefSim*>(p)->sim->reset_to_kickoff(seed); return 0; }
     catch (const std::exception& e) { return Fail(e); }

@@ -41,6 +41,7 @@ def lib():
         L.rgref_create_flags.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_uint64,
                                          ctypes.c_int]
         L.rgref_destroy.argtypes = [ctypes.c_void_p]
+        L.rgref_set_mutator_floats.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
         vp, cp, lg = ctypes.c_void_p, ctypes.c_void_p, ctypes.c_long
         for name, args in [
             ("rgref_set_env_car_count", [vp, cp, lg]),
@@ -151,6 +152,12 @@ class RefSim:
     def set_car_presets(self, presets):
         a = np.ascontiguousarray(presets, dtype=np.uint8)
         self._check(self._L.rgref_set_car_presets(self.h, _ptr(a), a.size))
+
+    def set_mutator_floats(self, **fields):
+        """set_mutator_config with the named float fields of the current MutatorConfig replaced."""
+        names = (ctypes.c_char_p * len(fields))(*[k.encode() for k in fields])
+        vals = np.array(list(fields.values()), np.float32)
+        self._check(self._L.rgref_set_mutator_floats(self.h, names, vals.ctypes.data, len(fields)))
 
     def reset_to_kickoff(self, seed: int = 0):
         self._check(self._L.rgref_reset_to_kickoff(self.h, seed))

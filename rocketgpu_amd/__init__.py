@@ -148,6 +148,12 @@ def lib():
         L.rg_serve_run.argtypes = [vp, ctypes.POINTER(ServeConfig), ctypes.POINTER(ServeReport), vp]
         L.rg_fold_digests.argtypes = [vp, ctypes.c_int64]
         L.rg_car_preset_config.argtypes = [ctypes.c_int32, vp]
+        L.rg_batch_download_async.argtypes = [vp, ctypes.c_int, vp, ctypes.c_size_t, vp]
+        L.rg_batch_copy_state_from.argtypes = [vp, vp, vp]
+        L.rg_mutator_config_defaults.argtypes = [vp]
+        L.rg_mutator_config_defaults.restype = None
+        L.rg_batch_set_mutator_config.argtypes = [vp, vp]
+        L.rg_batch_get_mutator_config.argtypes = [vp, vp]
         L.rg_fold_digests.restype = ctypes.c_uint64
         L.rg_batch_boost_pad_capacity.argtypes = [vp]
         L.rg_batch_boost_pad_capacity.restype = ctypes.c_int64
@@ -209,6 +215,20 @@ def car_preset_config(preset) -> np.ndarray:
     out = np.zeros(ITEMSIZE["car_configs"], dtype=np.uint8)
     _check(lib().rg_car_preset_config(i, out.ctypes.data))
     return out
+
+
+def mutator_config(**fields) -> np.ndarray:
+    """A MutatorConfig record (rocketgpu.h:708-840) with the reference's defaults and `fields` replaced: a one-element
+    array of state_dtype("mutator_config")."""
+    from .views import state_dtype
+
+    rec = np.zeros(1, state_dtype("mutator_config"))
+    lib().rg_mutator_config_defaults(rec.ctypes.data)
+    for k, v in fields.items():
+        if k not in rec.dtype.names:
+            raise KeyError(f"MutatorConfig has no field {k}")
+        rec[0][k] = v
+    return rec
 
 
 def _stream_ptr(stream):
@@ -274,6 +294,19 @@ class BatchSim:
         out = np.zeros((n, ITEMSIZE[name]), dtype=np.uint8)
         _check(lib().rg_batch_download(self._h, RG_ARRAY[name], out.ctypes.data, n, _stream_ptr(stream)))
         return out
+
+    def download_state_array_async(self, name: str, out: np.ndarray, stream=None):
+        """download_*_async (rocketgpu.h:1050-1053): queues the copy of all records of `name` into `out` (a
+        contiguous [count, itemsize] uint8 array, pinned for a true asynchronous copy) on `stream` without waiting;
+        synchronize the stream before reading `out`."""
+        n = self._count(name)
+        if not out.flags["C_CONTIGUOUS"] or out.nbytes < n * ITEMSIZE[name]:
+            raise ValueError("out must be a contiguous array of at least count * itemsize bytes")
+        _check(lib().rg_batch_download_async(self._h, RG_ARRAY[name], out.ctypes.data, n, _stream_ptr(stream)))
+
+    def copy_state_from(self, source: "BatchSim", stream=None):
+        """copy_state_from (rocketgpu.h:1045): every state array of `source` (same shape, arena and physics)."""
+        _check(lib().rg_batch_copy_state_from(self._h, source._h, _stream_ptr(stream)))
 
     def upload_state_array(self, name: str, host_ptr: int, count: int, stream=None):
         """upload_state_array (python/rocketgpu_bindings.cpp:955-983), public-state mode: `count` raw records."""
@@ -368,6 +401,19 @@ class BatchSim:
         _check(lib().rg_batch_step_arenas(self._h, ctypes.c_void_p(device_ids_ptr),
                                           ctypes.c_void_p(device_ticks_ptr or None), count, ticks,
                                           _stream_ptr(stream)))
+
+    def set_mutator_config(self, config=None, **fields):
+        """set_mutator_config (rocketgpu.h:1003): a record from mutator_config(), or the defaults with `fields`
+        replaced.  Fields this build keeps fixed must keep their defaults (RocketGpuError names the field)."""
+        rec = mutator_config(**fields) if config is None else np.ascontiguousarray(config)
+        _check(lib().rg_batch_set_mutator_config(self._h, rec.ctypes.data))
+
+    def get_mutator_config(self) -> np.ndarray:
+        from .views import state_dtype
+
+        rec = np.zeros(1, state_dtype("mutator_config"))
+        _check(lib().rg_batch_get_mutator_config(self._h, rec.ctypes.data))
+        return rec
 
     def step_split_no_alloc(self, ticks: int = 1, stream=None):
         """step(ticks) through the split schedule with its arena lists in the library's fixed device array: allocates

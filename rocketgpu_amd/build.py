@@ -3,6 +3,7 @@
   rocketgpu_amd/_build/librocketgpu_amd.so   the product: gfx950 kernels + BatchSim + C-ABI (hipcc)
   rocketgpu_amd/_build/librg_twin.so         test-only host compile of the same step core (g++)
   rocketgpu_amd/_build/librg_phases.so       test-only gfx950 kernels of single car phases (hipcc)
+  rocketgpu_amd/_build/batch_sim_client      test-only C++ caller of include/rocketgpu_amd/batch_sim.hpp (g++)
   oracle/_ref/librgref.so                    test-only reference CPU build (make -C oracle), when the reference
                                              sources are present
 
@@ -68,6 +69,16 @@ def build_phases():
     os.replace(tmp, PHASES)
 
 
+def build_client():
+    """Test-only C++ program driving the library through the BatchSim header (tests/cpp/batch_sim_client.cpp)."""
+    out = os.path.join(BUILD, "batch_sim_client")
+    rocm = os.environ.get("ROCM_PATH", "/opt/rocm")
+    _run(["g++", "-std=c++17", "-O2", "-D__HIP_PLATFORM_AMD__", f"-I{INCLUDE}", f"-I{rocm}/include", "-o", out + ".tmp",
+          os.path.join(REPO, "tests", "cpp", "batch_sim_client.cpp"), f"-L{BUILD}", "-lrocketgpu_amd",
+          "-Wl,-rpath,$ORIGIN", f"-L{rocm}/lib", "-lamdhip64", f"-Wl,-rpath,{rocm}/lib"])
+    os.replace(out + ".tmp", out)
+
+
 def build_oracle():
     ref = os.environ.get("REFERENCE_DIR", "/root/reference")
     if not os.path.isdir(ref) or shutil.which("make") is None:
@@ -80,11 +91,12 @@ def build_all():
     build_product()
     build_twin()
     build_phases()
+    build_client()
     build_oracle()
 
 
 if __name__ == "__main__":
     what = sys.argv[1:] or ["all"]
     for w in what:
-        {"all": build_all, "product": build_product, "twin": build_twin, "phases": build_phases,
+        {"all": build_all, "product": build_product, "twin": build_twin, "phases": build_phases, "client": build_client,
          "oracle": build_oracle}[w]()

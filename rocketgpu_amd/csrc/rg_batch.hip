@@ -19,6 +19,7 @@
 
 #include "batch_common.hpp"
 #include "car_presets.hpp"
+#include "mutator.hpp"
 #include "core/rg_arena_step.hpp"
 #include "core/rg_car_kickoff.hpp"
 #include "core/rg_env_commands.hpp"
@@ -586,6 +587,7 @@ public:
         mesh_host_ = arena.mesh_host_;
         params_ = arena.params_;
         car_args_ = arena.car_args_;
+        mutator_ = arena.mutator_;
         kickoff_settled_ = kickoff_settled;
         Allocate();
     }
@@ -673,7 +675,26 @@ public:
 
     // MutatorConfig::kickoff_settled (rocketgpu.h:827-832): resets without a countdown put the ball where the
     // game's countdown leaves it.  On by default, as in the reference.
-    void SetKickoffSettled(bool on) { kickoff_settled_ = on; }
+    void SetKickoffSettled(bool on) {
+        kickoff_settled_ = on;
+        mutator_.kickoff_settled = on ? 1 : 0;
+    }
+
+    // set_mutator_config (rocketgpu.h:1003): checked, applied to the step's parameters; the settled kickoff ball
+    // and rows are recorded again under the new physics.
+    void SetMutatorConfig(const rg_mutator_config& m) {
+        CheckMutatorConfig(m);
+        ApplyMutatorConfig(m, params_, car_args_.m);
+        mutator_ = m;
+        kickoff_settled_ = m.kickoff_settled != 0;
+        settled_ready_ = false;
+        kick_groups_.clear();
+        kick_rows_car_.clear();
+        kick_rows_wheels_.clear();
+        kick_dirty_ = true;
+        DestroyGraph();
+    }
+    const rg_mutator_config& MutatorConfig() const { return mutator_; }
 
     // set_physics_event_stream (src/rocketgpu.cpp:2896-2905): all zero disables the stream; otherwise it must cover
     // the whole batch.  Every step form writes to it from then on (a captured graph is captured again).  Arenas
@@ -850,7 +871,8 @@ public:
         }
     }
 
-    void Download(int id, void* dst, size_t n, hipStream_t s) const {
+    // `sync` false: the copy is only queued on `s` (download_*_async, rocketgpu.h:1050-1053); the caller waits on s.
+    void Download(int id, void* dst, size_t n, hipStream_t s, bool sync = true) const {
         switch (id) {
         case RG_ARRAY_BALL_STATES: balls_.Download(static_cast<BallState*>(dst), n, s); break;
         case RG_ARRAY_ENV_STATES: envs_.Download(static_cast<EnvState*>(dst), n, s); break;
@@ -869,7 +891,44 @@ public:
         }
         default: throw InvalidArgument("unknown state array id");
         }
+        if (sync) Check(hipStreamSynchronize(s), "hipStreamSynchronize");
+    }
+
+    // copy_state_from (rocketgpu.h:1045): every state array of `src` (a batch of the same shape on the same arena
+    // and physics) copied on the device, with its car counts, seed and settled-kickoff state.
+    void CopyStateFrom(const BatchSim& src, hipStream_t s) {
+        if (&src == this) return;
+        if (src.cfg_.env_count != cfg_.env_count || src.cfg_.max_cars_per_env != cfg_.max_cars_per_env ||
+            src.params_.pad_count != params_.pad_count || src.GeometryDigest() != GeometryDigest()) {
+            throw InvalidArgument("copy_state_from needs a batch of the same shape, arena and physics");
+        }
+        const auto copy = [&](auto& dst, const auto& from) {
+            using T = typename std::remove_pointer<decltype(dst.data())>::type;
+            if (dst.size()) {
+                Check(hipMemcpyAsync(dst.data(), from.data(), dst.size() * sizeof(T), hipMemcpyDeviceToDevice, s),
+                      "hipMemcpyAsync D2D");
+            }
+        };
+        copy(cars_, src.cars_);
+        copy(controls_, src.controls_);
+        copy(configs_, src.configs_);
+        copy(teams_, src.teams_);
+        copy(wheels_, src.wheels_);
+        copy(balls_, src.balls_);
+        copy(envs_, src.envs_);
+        copy(pads_, src.pads_);
+        copy(car_count_, src.car_count_);
+        copy(internal_, src.internal_);
+        if (src.settled_ready_) copy(settled_, src.settled_);
         Check(hipStreamSynchronize(s), "hipStreamSynchronize");
+        host_counts_ = src.host_counts_;
+        has_cars_ = src.has_cars_;
+        seed_ = src.seed_;
+        kickoff_settled_ = src.kickoff_settled_;
+        settled_ready_ = src.settled_ready_;
+        mutator_ = src.mutator_;
+        kick_dirty_ = true;
+        DestroyGraph();
     }
 
     // ---- Full-state snapshot (checkpoint / resume) -------------------------------------------------------------
@@ -899,14 +958,15 @@ public:
         uint8_t reserved[6];
     };
     static constexpr char kSnapshotMagic[8] = {'R', 'G', 'S', 'N', 'A', 'P', 'S', 'T'};
-    static constexpr uint32_t kSnapshotVersion = 1;
+    static constexpr uint32_t kSnapshotVersion = 2;
 
     size_t SnapshotBodyBytes() const {
         const size_t cars = static_cast<size_t>(car_capacity());
         const size_t envs = static_cast<size_t>(cfg_.env_count);
         return cars * (sizeof(CarState) + sizeof(CarControls) + sizeof(CarConfig) + 1) +
                envs * (sizeof(BallState) + sizeof(EnvState) + sizeof(int32_t) + sizeof(core::ArenaInternal)) +
-               pads_.size() * sizeof(BoostPadState) + sizeof(BallState);
+               pads_.size() * sizeof(BoostPadState) + cars * sizeof(WheelRuntimeState) + sizeof(BallState) +
+               sizeof(uint64_t);
     }
     size_t SnapshotBytes() const { return sizeof(SnapshotHeader) + SnapshotBodyBytes(); }
 
@@ -954,6 +1014,7 @@ public:
         take(pads_);
         take(car_count_);
         take(internal_);
+        take(wheels_);
         Check(hipStreamSynchronize(s), "hipStreamSynchronize");
         BallState settled{};
         if (settled_ready_) {
@@ -962,6 +1023,8 @@ public:
         }
         std::memcpy(at, &settled, sizeof(settled));
         at += sizeof(settled);
+        std::memcpy(at, &seed_, sizeof(seed_));
+        at += sizeof(seed_);
         SnapshotHeader hdr{};
         std::memcpy(hdr.magic, kSnapshotMagic, sizeof(hdr.magic));
         FillHeader(hdr);
@@ -1008,10 +1071,12 @@ public:
             }
         }
         const size_t off_counts = off_internal - envs * sizeof(int32_t);
+        std::vector<int32_t> counts(envs);
         for (size_t e = 0; e < envs; ++e) {
-            int32_t c;
-            std::memcpy(&c, body + off_counts + e * sizeof(int32_t), sizeof(c));
-            if (c < 0 || c > cfg_.max_cars_per_env) throw InvalidArgument("snapshot holds an invalid car count");
+            std::memcpy(&counts[e], body + off_counts + e * sizeof(int32_t), sizeof(int32_t));
+            if (counts[e] < 0 || counts[e] > cfg_.max_cars_per_env) {
+                throw InvalidArgument("snapshot holds an invalid car count");
+            }
         }
         const unsigned char* at = body;
         const auto put = [&](auto& arr) {
@@ -1028,12 +1093,20 @@ public:
         put(pads_);
         put(car_count_);
         put(internal_);
+        put(wheels_);
         kickoff_settled_ = hdr.kickoff_settled != 0;
+        mutator_.kickoff_settled = hdr.kickoff_settled;
         if (hdr.settled_ready) {
             if (settled_.size() == 0) settled_.Resize(1);
             settled_.Upload(reinterpret_cast<const BallState*>(at), 1, s);
         }
+        at += sizeof(BallState);
+        std::memcpy(&seed_, at, sizeof(seed_));
         settled_ready_ = hdr.settled_ready != 0;
+        host_counts_ = counts;
+        has_cars_ = std::any_of(counts.begin(), counts.end(), [](int32_t c) { return c > 0; });
+        kick_dirty_ = true;
+        DestroyGraph();
         Check(hipStreamSynchronize(s), "hipStreamSynchronize");
     }
 
@@ -1683,6 +1756,7 @@ private:
     DeviceArray<EnvCommand> commands_;
     DeviceArray<BallState> settled_;
     bool kickoff_settled_ = true;
+    rg_mutator_config mutator_ = DefaultMutatorConfig();
     bool settled_ready_ = false;
     EventStream events_{};
     DeviceArray<uint64_t> digests_;
@@ -1837,6 +1911,35 @@ int rg_batch_upload_mode(rg_batch* b, int id, const void* src, size_t count, int
 
 int rg_batch_download(rg_batch* b, int id, void* dst, size_t count, void* stream) {
     return Guard([&] { b->sim->Download(id, dst, count, S(stream)); });
+}
+
+int rg_batch_download_async(rg_batch* b, int id, void* dst, size_t count, void* stream) {
+    return Guard([&] { b->sim->Download(id, dst, count, S(stream), false); });
+}
+
+int rg_batch_copy_state_from(rg_batch* b, const rg_batch* src, void* stream) {
+    return Guard([&] {
+        if (!src) throw rg::InvalidArgument("source batch must not be null");
+        b->sim->CopyStateFrom(*src->sim, S(stream));
+    });
+}
+
+void rg_mutator_config_defaults(rg_mutator_config* out) {
+    if (out) *out = rg::DefaultMutatorConfig();
+}
+
+int rg_batch_set_mutator_config(rg_batch* b, const rg_mutator_config* config) {
+    return Guard([&] {
+        if (!config) throw rg::InvalidArgument("config must not be null");
+        b->sim->SetMutatorConfig(*config);
+    });
+}
+
+int rg_batch_get_mutator_config(const rg_batch* b, rg_mutator_config* out) {
+    return Guard([&] {
+        if (!out) throw rg::InvalidArgument("out must not be null");
+        *out = b->sim->MutatorConfig();
+    });
 }
 
 int rg_batch_step(rg_batch* b, int32_t ticks, void* stream) {
@@ -2255,6 +2358,12 @@ field("pos", "<f4", offsetof(rg::CarState, pos), 3);
     RG_F(rg::BoostPadState, pickup_frac_q, "<u2", 0);
     RG_F(rg::BoostPadState, prev_locked_car_id, "<u4", 0);
     RG_F(rg::BoostPadState, cur_locked_car_id, "<u4", 0);
+    // The MutatorConfig record of rg_batch_set_mutator_config (a host record, not a device array).
+    begin("mutator_config", sizeof(rg_mutator_config), false);
+#define RG_MUT_LAYOUT(type, name, def, target) \
+    field(#name, std::is_same<type, float>::value ? "<f4" : "|u1", offsetof(rg_mutator_config, name), 0);
+    RG_MUTATOR_FIELDS(RG_MUT_LAYOUT)
+#undef RG_MUT_LAYOUT
 #undef RG_SUB
 #undef RG_F
     j += "]}}";

@@ -159,3 +159,52 @@ def test_gpu_car_arenas_per_wave_agree(per_wave):
     for key in a:
         assert np.array_equal(a[key], b[key]), key
     assert np.array_equal(ua, ub)
+
+
+def _mixed_batch(n, seed):
+    import rocketgpu_amd as rga
+
+    counts = np.array([2, 4, 6, 3] * (n // 4), np.int32)
+    presets = np.arange(n * 6) % 6
+    teams = (np.arange(n * 6) % 6 >= np.repeat(counts, 6) // 2).astype(np.uint8)
+    cfgs = np.stack([rga.car_preset_config(int(p)) for p in presets])
+    sim = _batch(n, 6, counts, teams, cfgs)
+    sim.reset_to_kickoff(seed)
+    return sim
+
+
+def test_gpu_copy_state_snapshot_and_async_download_with_cars():
+    """copy_state_from, a snapshot save/load and an asynchronous download each carry a car batch (wheel runtime
+    and internal state included): the copies then step exactly as the original."""
+    torch = _torch()
+    import rocketgpu_amd as rga
+
+    n = 64
+    say(f"case: {n} mixed arenas, state copy / snapshot / async download")
+    ctrl = torch.from_numpy(rga.workload_controls(3, 0, n, 6, 64).reshape(64, -1)).cuda()
+    a = _mixed_batch(n, 3)
+    for k in range(4):
+        a.step_with_device_controls(ctrl.data_ptr() + k * 8 * ctrl.shape[1], 8, n * 6)
+    b = _mixed_batch(n, 99)
+    b.copy_state_from(a)
+    c = _mixed_batch(n, 98)
+    c.load_state(a.save_state())
+    for k in range(4, 8):
+        for s in (a, b, c):
+            s.step_with_device_controls(ctrl.data_ptr() + k * 8 * ctrl.shape[1], 8, n * 6)
+    ra, ua = batch_records(a)
+    for other in (b, c):
+        ro, uo = batch_records(other)
+        for key in ra:
+            assert np.array_equal(ra[key], ro[key]), key
+        assert np.array_equal(ua, uo)
+    stream = torch.cuda.Stream()
+    out = torch.empty((n * 6, 880), dtype=torch.uint8, pin_memory=True).numpy()
+    a.download_state_array_async("car_states", out, stream=stream)
+    stream.synchronize()
+    assert np.array_equal(out, ra["cars"])
+    other = rga.BatchSim(n + 1, 6)
+    with pytest.raises(rga.RocketGpuError, match="same shape"):
+        other.copy_state_from(a)
+    for s in (a, b, c, other):
+        s.close()
