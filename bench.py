@@ -256,22 +256,30 @@ def bench_cars(args, rga, np, torch, stream):
 
     # Arenas per wavefront of the car kernel (one lane per arena; fewer arenas per wave, more waves per CU): the
     # tuner times each on a fresh batch over the warm-up window plus five steps and keeps the fastest.
+    # Staging 0 keeps each arena's per-tick working data in lane-private memory, 2 in LDS (one slot per lane).
     sweep = {}
-    per_wave = args.per_wave if args.block else None
-    if per_wave is None:
-        for cand in (64, 32, 16, 8, 4):
+    shape = (args.stage, args.per_wave) if args.block else None
+    if shape is None:
+        key = lambda c: f"stage{c[0]}_perwave{c[1]}"
+        for cand in ((0, 64), (0, 32), (0, 16), (2, 32), (2, 16), (2, 8), (2, 4)):
             tsim = make_car_batch(rga, np, n, args.seed, workload)
-            tsim.set_launch_shape(64, 0, cand)
-            for k in range(args.warmup):
-                tsim.step_with_device_controls(ctrl_ptr(k), T, tsim.car_capacity, stream=stream)
-            sweep[f"perwave{cand}"] = timed(torch, stream, lambda: tsim.step_with_device_controls(
-                ctrl_ptr(args.warmup), T, tsim.car_capacity, stream=stream), 5)
+            tsim.set_launch_shape(64, *cand)
+            try:
+                for k in range(args.warmup):
+                    tsim.step_with_device_controls(ctrl_ptr(k), T, tsim.car_capacity, stream=stream)
+                sweep[key(cand)] = timed(torch, stream, lambda: tsim.step_with_device_controls(
+                    ctrl_ptr(args.warmup), T, tsim.car_capacity, stream=stream), 5)
+            except rga.RocketGpuError as err:  # the LDS slots do not fit this device
+                sweep[key(cand)] = str(err)
+                tsim.close()
+                continue
             tsim.close()
-            if per_wave is None or sweep[f"perwave{cand}"] < sweep[f"perwave{per_wave}"]:
-                per_wave = cand
+            if shape is None or sweep[key(cand)] < sweep[key(shape)]:
+                shape = cand
+    stage, per_wave = shape
 
     sim = make_car_batch(rga, np, n, args.seed, workload)
-    sim.set_launch_shape(64, 0, per_wave)
+    sim.set_launch_shape(64, stage, per_wave)
     cap = sim.car_capacity
     torch.cuda.synchronize()
     for k in range(args.warmup):
@@ -315,7 +323,7 @@ def bench_cars(args, rga, np, torch, stream):
         # and hipGraph replays of both.
         def form(run):
             fsim = make_car_batch(rga, np, n, args.seed, workload)
-            fsim.set_launch_shape(64, 0, per_wave)
+            fsim.set_launch_shape(64, stage, per_wave)
             for _ in range(args.warmup):
                 run(fsim)
             ms = timed(torch, stream, lambda: run(fsim), args.steps)
@@ -341,14 +349,16 @@ def bench_cars(args, rga, np, torch, stream):
                                f"step_with_device_controls T={T}",
                    "arenas": n, "ticks_per_step": T, "cars_per_arena_mix": mix,
                    "cars": int(counts.sum()), "parallelism": "single GPU",
-                   "kernel_launch": f"one lane per arena, {per_wave} arenas per 64-lane wavefront",
-                   "arenas_per_wave_sweep_ms": sweep or None},
+                   "kernel_launch": f"one lane per arena, {per_wave} arenas per 64-lane wavefront, per-tick data "
+                                    f"in {'LDS' if stage == 2 else 'lane-private memory'}",
+                   "launch_shape_sweep_ms": sweep or None},
         "arenas_unsupported_frac_at_end": float((unsupported != 0).mean()),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "limiter": "latency (one lane per arena walks its wheel rays, broadphase and solver serially; "
                                 "dependent loads at low occupancy)",
-                     "kernel": "StepCarArenasKernel", "kernel_avg_ms": avg_kernel_s * 1e3,
+                     "kernel": f"StepCarArenasKernel<{'true' if stage == 2 else 'false'}>",
+                     "kernel_avg_ms": avg_kernel_s * 1e3,
                      "bytes_per_arena_tick": b_arena},
         "forms": forms,
         "cpu_baseline": cpu,

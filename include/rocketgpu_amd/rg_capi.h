@@ -347,6 +347,17 @@ typedef struct rg_serve_config {
     double request_rate_hz;
     double duration_s;
     uint64_t seed;
+    /* Admission by HBM bytes: a launch takes requests while the arena records it will touch
+     * (rg_batch_arena_bytes of each arena's car count) stay within this many bytes; 0: no byte budget. */
+    int64_t launch_budget_bytes;
+    /* Every reset_every-th request of a client (its requests counted from 1) first resets the client's arenas to a
+     * settled kickoff with the spots in slot order (kickoff seed mode 3, a new match); 0: no resets. */
+    int32_t reset_every;
+    /* Nonzero: each request brings its own held controls for the client's cars (rg_workload_request_controls),
+     * written into the batch's control rows on the launch's stream before the step. */
+    int32_t fresh_controls;
+    float p_boost;
+    int32_t reserved;
 } rg_serve_config;
 
 typedef struct rg_serve_report {
@@ -354,6 +365,7 @@ typedef struct rg_serve_report {
     int64_t arena_ticks;
     int64_t launches;
     int64_t arenas_resident;
+    int64_t resident_bytes;
     double seconds;
     double arena_ticks_per_s;
     double p50_ms;
@@ -366,6 +378,17 @@ typedef struct rg_serve_report {
  * was stepped. */
 int rg_serve_run(rg_batch* batch, const rg_serve_config* config, rg_serve_report* report,
                  int32_t* host_ticks_per_arena);
+/* The serving workload, for replaying a client's requests elsewhere: client c's arenas [*first, *first + *count),
+ * and the ticks of its requests in the order they are served (returns how many; at most `capacity` are written). */
+int rg_serve_client_arenas(const rg_serve_config* config, int32_t client, int32_t* first, int32_t* count);
+int64_t rg_serve_client_requests(const rg_serve_config* config, int32_t client, int32_t* out_ticks, int64_t capacity);
+/* The held controls of request `request` (counted from 0) for arenas [first_arena, first_arena + arena_count) with
+ * cars_per_arena car slots each: out receives arena_count * cars_per_arena CarControls. */
+int rg_workload_request_controls(uint64_t seed, int64_t request, int64_t first_arena, int64_t arena_count,
+                                 int32_t cars_per_arena, float p_boost, void* out);
+/* The HBM bytes of one arena's records a step touches with `cars` cars: ball, env, internal state, pads, car count,
+ * and per car its state, wheel state, controls, config and team. */
+int64_t rg_batch_arena_bytes(const rg_batch* batch, int32_t cars);
 
 /* state_layouts()  (python/rocketgpu_bindings.cpp:541-806) as JSON into buf; returns the length needed. */
 int64_t rg_state_layouts_json(char* buf, size_t capacity);

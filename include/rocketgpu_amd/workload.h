@@ -374,6 +374,14 @@ struct ControlStream {
     }
 };
 
+// The held controls car `slot` of arena `arena` gets for request number `request` of its client in the serving
+// workload: the first draw of a control stream seeded by the request.
+inline Controls RequestControls(uint64_t seed, uint64_t request, uint64_t arena, uint64_t slot, float p_boost) {
+    ControlStream s;
+    s.seed(seed ^ (0x9e3779b97f4a7c15ull * (request + 1)), arena, slot, p_boost);
+    return s.next();
+}
+
 // A ball launch somewhere in the open part of PA(s, r): position in the inner box, velocity up to 3000 uu/s,
 // angular velocity up to 5 rad/s per axis.
 struct BallLaunch {

@@ -85,12 +85,14 @@ class ServeConfig(ctypes.Structure):
     _fields_ = [("clients", ctypes.c_int32), ("min_arenas", ctypes.c_int32), ("max_arenas", ctypes.c_int32),
                 ("horizon_count", ctypes.c_int32), ("horizons", ctypes.c_int32 * 4),
                 ("max_arenas_per_launch", ctypes.c_int32), ("max_in_flight", ctypes.c_int32),
-                ("request_rate_hz", ctypes.c_double), ("duration_s", ctypes.c_double), ("seed", ctypes.c_uint64)]
+                ("request_rate_hz", ctypes.c_double), ("duration_s", ctypes.c_double), ("seed", ctypes.c_uint64),
+                ("launch_budget_bytes", ctypes.c_int64), ("reset_every", ctypes.c_int32),
+                ("fresh_controls", ctypes.c_int32), ("p_boost", ctypes.c_float), ("reserved", ctypes.c_int32)]
 
 
 class ServeReport(ctypes.Structure):
     _fields_ = [("requests", ctypes.c_int64), ("arena_ticks", ctypes.c_int64), ("launches", ctypes.c_int64),
-                ("arenas_resident", ctypes.c_int64), ("seconds", ctypes.c_double),
+                ("arenas_resident", ctypes.c_int64), ("resident_bytes", ctypes.c_int64), ("seconds", ctypes.c_double),
                 ("arena_ticks_per_s", ctypes.c_double), ("p50_ms", ctypes.c_double), ("p99_ms", ctypes.c_double),
                 ("mean_ms", ctypes.c_double), ("max_ms", ctypes.c_double)]
 
@@ -146,6 +148,13 @@ def lib():
         L.rg_batch_set_launch_shape.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]
         L.rg_batch_set_split.argtypes = [vp, ctypes.c_int32, ctypes.c_int32]
         L.rg_serve_run.argtypes = [vp, ctypes.POINTER(ServeConfig), ctypes.POINTER(ServeReport), vp]
+        L.rg_serve_client_arenas.argtypes = [ctypes.POINTER(ServeConfig), ctypes.c_int32, vp, vp]
+        L.rg_serve_client_requests.argtypes = [ctypes.POINTER(ServeConfig), ctypes.c_int32, vp, ctypes.c_int64]
+        L.rg_serve_client_requests.restype = ctypes.c_int64
+        L.rg_workload_request_controls.argtypes = [ctypes.c_uint64, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
+                                                   ctypes.c_int32, ctypes.c_float, vp]
+        L.rg_batch_arena_bytes.argtypes = [vp, ctypes.c_int32]
+        L.rg_batch_arena_bytes.restype = ctypes.c_int64
         L.rg_fold_digests.argtypes = [vp, ctypes.c_int64]
         L.rg_car_preset_config.argtypes = [ctypes.c_int32, vp]
         L.rg_batch_download_async.argtypes = [vp, ctypes.c_int, vp, ctypes.c_size_t, vp]
@@ -229,6 +238,27 @@ def mutator_config(**fields) -> np.ndarray:
             raise KeyError(f"MutatorConfig has no field {k}")
         rec[0][k] = v
     return rec
+
+
+def serve_client_plan(cfg: ServeConfig, client: int):
+    """A client's arena range and the ticks of its requests in the order the serving loop serves them."""
+    first, count = ctypes.c_int32(), ctypes.c_int32()
+    _check(lib().rg_serve_client_arenas(ctypes.byref(cfg), client, ctypes.byref(first), ctypes.byref(count)))
+    n = lib().rg_serve_client_requests(ctypes.byref(cfg), client, None, 0)
+    if n < 0:
+        raise RocketGpuError(lib().rg_last_error().decode())
+    ticks = np.zeros(max(n, 1), np.int32)
+    lib().rg_serve_client_requests(ctypes.byref(cfg), client, ticks.ctypes.data, n)
+    return int(first.value), int(count.value), ticks[:n]
+
+
+def request_controls(seed: int, request: int, first_arena: int, arena_count: int, cars_per_arena: int,
+                     p_boost: float = 0.3) -> np.ndarray:
+    """The held controls of a serving request ([arena_count * cars_per_arena, 24] uint8 CarControls records)."""
+    out = np.zeros((arena_count * cars_per_arena, ITEMSIZE["controls"]), np.uint8)
+    _check(lib().rg_workload_request_controls(seed, request, first_arena, arena_count, cars_per_arena, p_boost,
+                                              out.ctypes.data))
+    return out
 
 
 def _stream_ptr(stream):
@@ -446,13 +476,25 @@ class BatchSim:
         """Lanes per workgroup, state staging (0 in place, 1 lane-private copy), arenas per 64-lane wavefront."""
         _check(lib().rg_batch_set_launch_shape(self._h, block, stage, arenas_per_wave))
 
-    def serve(self, clients=256, min_arenas=16, max_arenas=256, horizons=(1, 4, 8), max_arenas_per_launch=65536,
-              max_in_flight=2, request_rate_hz=2000.0, duration_s=2.0, seed=0, ticks_per_arena=False):
-        """Run the serving loop (continuous batching over this batch's arenas); returns the report as a dict and,
-        with ticks_per_arena, the ticks each arena was stepped."""
+    @staticmethod
+    def serve_config(clients=256, min_arenas=16, max_arenas=256, horizons=(1, 4, 8), max_arenas_per_launch=65536,
+                     max_in_flight=2, request_rate_hz=2000.0, duration_s=2.0, seed=0, launch_budget_bytes=0,
+                     reset_every=0, fresh_controls=False, p_boost=0.3) -> ServeConfig:
         h = (ctypes.c_int32 * 4)(*(list(horizons) + [0] * (4 - len(horizons))))
-        cfg = ServeConfig(clients, min_arenas, max_arenas, len(horizons), h, max_arenas_per_launch, max_in_flight,
-                          request_rate_hz, duration_s, seed)
+        return ServeConfig(clients, min_arenas, max_arenas, len(horizons), h, max_arenas_per_launch, max_in_flight,
+                           request_rate_hz, duration_s, seed, launch_budget_bytes, reset_every,
+                           1 if fresh_controls else 0, p_boost, 0)
+
+    def serve(self, clients=256, min_arenas=16, max_arenas=256, horizons=(1, 4, 8), max_arenas_per_launch=65536,
+              max_in_flight=2, request_rate_hz=2000.0, duration_s=2.0, seed=0, ticks_per_arena=False,
+              launch_budget_bytes=0, reset_every=0, fresh_controls=False, p_boost=0.3):
+        """Run the serving loop (continuous batching over this batch's arenas); returns the report as a dict and,
+        with ticks_per_arena, the ticks each arena was stepped.  launch_budget_bytes admits requests by the HBM
+        bytes of the arena records a launch touches; reset_every resets a client's arenas before every n-th of its
+        requests; fresh_controls gives each request its own held controls."""
+        cfg = self.serve_config(clients, min_arenas, max_arenas, horizons, max_arenas_per_launch, max_in_flight,
+                                request_rate_hz, duration_s, seed, launch_budget_bytes, reset_every, fresh_controls,
+                                p_boost)
         rep = ServeReport()
         per_arena = np.zeros(self.env_count, np.int32) if ticks_per_arena else None
         _check(lib().rg_serve_run(self._h, ctypes.byref(cfg), ctypes.byref(rep),

@@ -294,20 +294,9 @@ struct CarStepArgs {
 // car tick (one lane walks all of its queries).  Workgroups are one wavefront; its first a.arenas_per_wave lanes
 // take one arena each (fewer arenas per wave: more waves per CU to hide the dependent loads).  Fields the game
 // reports are refreshed on the call's last tick.
-__global__ void __launch_bounds__(64) StepCarArenasKernel(StepArgs a, CarStepArgs c, int32_t ticks) {
-    if (static_cast<int>(threadIdx.x) >= a.arenas_per_wave) return;
-    const int i = static_cast<int>(blockIdx.x) * a.arenas_per_wave + static_cast<int>(threadIdx.x);
-    if (i >= a.count) return;
-    const int e = a.arena_ids ? a.arena_ids[i] : i;
-    if (e < 0 || e >= a.env_count) return;
-    const int n = a.arena_ticks ? a.arena_ticks[i] : ticks;
-    const int count = a.env_car_count[e];
-    if (count == 0) {
-        core::BallTickScratch scratch;
-        for (int t = 0; t < n; ++t) Tick<false>(a.balls[e], a.envs[e], a.internal[e], a, scratch, core::EventSink{});
-        return;
-    }
-    core::CarArenaTick scratch;
+// Arena e's n car ticks with `scratch` as its per-tick working data.
+__device__ inline void RunCarArenaTicks(const StepArgs& a, const CarStepArgs& c, int e, int n, int count,
+                                        core::CarArenaTick& scratch) {
     const int base = e * a.max_cars;
     const core::PadLayout pads{c.pad_pos, a.params.pad_count};
     for (int t = 0; t < n; ++t) {
@@ -326,6 +315,40 @@ __global__ void __launch_bounds__(64) StepCarArenasKernel(StepArgs a, CarStepArg
         r.car_base = base;
         const core::CarRespawnContext rc{&c.rules, c.seed, e, 0u, 0u, 0};
         core::StepCarArena(r, a.mesh, a.params, c.m, pads, rc, t + 1 == n, scratch);
+    }
+}
+
+__device__ inline void RunBallArenaTicks(const StepArgs& a, int e, int n, core::BallTickScratch& scratch) {
+    for (int t = 0; t < n; ++t) Tick<false>(a.balls[e], a.envs[e], a.internal[e], a, scratch, core::EventSink{});
+}
+
+// kLds: each arena's per-tick working data (CarArenaTick) lives in LDS, one slot per lane, instead of lane-private
+// memory.
+template <bool kLds>
+__global__ void __launch_bounds__(64) StepCarArenasKernel(StepArgs a, CarStepArgs c, int32_t ticks) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    if (static_cast<int>(threadIdx.x) >= a.arenas_per_wave) return;
+    const int i = static_cast<int>(blockIdx.x) * a.arenas_per_wave + static_cast<int>(threadIdx.x);
+    if (i >= a.count) return;
+    const int e = a.arena_ids ? a.arena_ids[i] : i;
+    if (e < 0 || e >= a.env_count) return;
+    const int n = a.arena_ticks ? a.arena_ticks[i] : ticks;
+    const int count = a.env_car_count[e];
+    if constexpr (kLds) {
+        core::CarArenaTick& slot = reinterpret_cast<core::CarArenaTick*>(lds)[threadIdx.x];
+        if (count == 0) {
+            RunBallArenaTicks(a, e, n, slot.ball);
+        } else {
+            RunCarArenaTicks(a, c, e, n, count, slot);
+        }
+    } else {
+        if (count == 0) {
+            core::BallTickScratch scratch;
+            RunBallArenaTicks(a, e, n, scratch);
+            return;
+        }
+        core::CarArenaTick scratch;
+        RunCarArenaTicks(a, c, e, n, count, scratch);
     }
 }
 
@@ -1592,7 +1615,26 @@ private:
         a.arenas_per_wave = per_wave_;
         const CarStepArgs c = CarArgs(controls, stride);
         const int grid = (count + per_wave_ - 1) / per_wave_;
-        hipLaunchKernelGGL(StepCarArenasKernel, dim3(grid), dim3(64), 0, s, a, c, ticks);
+        if (stage_ == kStageLdsScratch) {
+            const size_t lds = sizeof(core::CarArenaTick) * static_cast<size_t>(per_wave_);
+            if (lds > car_lds_allowed_) {
+                int device = 0, limit = 0;
+                Check(hipGetDevice(&device), "hipGetDevice");
+                Check(hipDeviceGetAttribute(&limit, hipDeviceAttributeMaxSharedMemoryPerBlock, device),
+                      "hipDeviceGetAttribute");
+                if (lds > static_cast<size_t>(limit)) {
+                    throw InvalidArgument("car arenas with LDS staging need " + std::to_string(lds) +
+                                          " B of LDS per workgroup; the device allows " + std::to_string(limit));
+                }
+                Check(hipFuncSetAttribute(reinterpret_cast<const void*>(&StepCarArenasKernel<true>),
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, limit),
+                      "hipFuncSetAttribute");
+                car_lds_allowed_ = static_cast<size_t>(limit);
+            }
+            hipLaunchKernelGGL(StepCarArenasKernel<true>, dim3(grid), dim3(64), lds, s, a, c, ticks);
+        } else {
+            hipLaunchKernelGGL(StepCarArenasKernel<false>, dim3(grid), dim3(64), 0, s, a, c, ticks);
+        }
         Check(hipGetLastError(), "StepCarArenasKernel launch");
     }
 
@@ -1762,7 +1804,8 @@ private:
     DeviceArray<uint64_t> digests_;
     DeviceArray<int32_t> split_lists_;   // stages 6 and 7: heavy then light arena ids
     DeviceArray<int32_t> split_counts_;
-    int32_t* no_alloc_scratch_ = nullptr;  // device address of `scratch` on this batch's device
+    int32_t* no_alloc_scratch_ = nullptr;
+    size_t car_lds_allowed_ = 65536;  // dynamic LDS the LDS-staged car kernel may take without a larger attribute  // device address of `scratch` on this batch's device
     int split_rounds_ = kSplitRoundsDefault;
     int split_cap_divisor_ = kSplitCapDivisorDefault;
     int block_ = 64;
@@ -2080,6 +2123,34 @@ int rg_car_preset_config(int32_t preset, void* out) {
         const rg::CarConfig c = rg::CarPresetConfig(preset);
         std::memcpy(out, &c, sizeof(rg::CarConfig));
     });
+}
+
+int rg_workload_request_controls(uint64_t seed, int64_t request, int64_t first_arena, int64_t arena_count,
+                                 int32_t cars_per_arena, float p_boost, void* out) {
+    return Guard([&] {
+        if (request < 0 || first_arena < 0 || arena_count < 0 || cars_per_arena < 0 ||
+            (arena_count * cars_per_arena > 0 && !out)) {
+            throw rg::InvalidArgument("bad request controls arguments");
+        }
+        rgw::Controls* o = static_cast<rgw::Controls*>(out);
+        for (int64_t a = 0; a < arena_count; ++a) {
+            for (int32_t k = 0; k < cars_per_arena; ++k) {
+                o[a * cars_per_arena + k] = rgw::RequestControls(seed, static_cast<uint64_t>(request),
+                                                                 static_cast<uint64_t>(first_arena + a),
+                                                                 static_cast<uint64_t>(k), p_boost);
+            }
+        }
+    });
+}
+
+int64_t rg_batch_arena_bytes(const rg_batch* b, int32_t cars) {
+    if (!b || cars < 0) return -1;
+    const int64_t pads = rg_batch_boost_pad_capacity(b) / rg_batch_env_count(b);
+    return static_cast<int64_t>(sizeof(rg::BallState) + sizeof(rg::EnvState) + sizeof(rg::core::ArenaInternal) +
+                                sizeof(int32_t)) +
+           pads * static_cast<int64_t>(sizeof(rg::BoostPadState)) +
+           cars * static_cast<int64_t>(sizeof(rg::CarState) + sizeof(rg::WheelRuntimeState) +
+                                       sizeof(rg::CarControls) + sizeof(rg::CarConfig) + 1);
 }
 
 uint64_t rg_fold_digests(const uint64_t* d, int64_t n) {

@@ -1,11 +1,14 @@
 // Serving loop: continuous batching of step requests over arenas resident in HBM.
 //
-// Simulated actor clients each own a disjoint range of the batch's arenas and send requests "step my arenas k
-// ticks" with Poisson arrivals.  The loop admits requests whose client is idle into the next launch while the
-// launch's arena budget allows, keeps up to max_in_flight launches on separate streams (their arena sets are
-// disjoint, so launches overlap on the device), and retires a request when the event after its launch completes.
-// Latency = completion - arrival (queueing included).  An arena's result depends only on its own tick count, so a
-// request's result is the same alone and in any mix; tests check that byte for byte.
+// Simulated actor clients each own a disjoint range of the batch's arenas (ball-only or with cars) and send
+// requests "step my arenas k ticks" with Poisson arrivals; a request may bring fresh held controls for the client's
+// cars, and every reset_every-th request of a client first resets its arenas to a kickoff.  The loop admits
+// requests whose client is idle into the next launch while the launch's budgets allow (arenas per launch, and HBM
+// bytes of the arena records the launch touches), keeps up to max_in_flight launches on separate streams (their
+// arena sets are disjoint, so launches overlap on the device), and retires a request when the event after its
+// launch completes.  Latency = completion - arrival (queueing included).  An arena's result depends only on its own
+// requests (ticks, controls, resets), so a request's result is the same alone and in any mix; tests check that
+// byte for byte, and replay served arenas on the reference build.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -19,6 +22,7 @@
 #include <vector>
 
 #include "rocketgpu_amd/rg_capi.h"
+#include "rocketgpu_amd/rg_types.h"
 #include "rocketgpu_amd/workload.h"
 
 extern "C" void rg_set_last_error(const char* message);  // rg_batch.hip
@@ -46,6 +50,10 @@ struct Slot {
     int32_t* host_ticks = nullptr;
     int32_t* dev_ids = nullptr;
     int32_t* dev_ticks = nullptr;
+    rgw::Controls* host_controls = nullptr;  // fresh controls of the launch's arenas, in arena order
+    rg::EnvCommand* dev_commands = nullptr;  // env_count commands, zero except the arenas being reset
+    rg::EnvCommand* host_reset = nullptr;    // one reset command per arena of the largest client
+    std::vector<std::pair<int32_t, int32_t>> reset_ranges;
     std::vector<int32_t> requests;  // indices into the request table
     bool busy = false;
 };
@@ -55,6 +63,49 @@ double Exponential(uint64_t& state, double rate) {
     return -std::log(1.0 - u) / rate;
 }
 
+void CheckConfig(const rg_serve_config* cfg) {
+    if (!cfg) throw std::invalid_argument("null argument");
+    if (cfg->clients <= 0 || cfg->min_arenas <= 0 || cfg->max_arenas < cfg->min_arenas) {
+        throw std::invalid_argument("clients and arenas per client must be positive");
+    }
+    if (cfg->horizon_count <= 0 || cfg->horizon_count > 4) throw std::invalid_argument("1..4 horizons");
+    for (int h = 0; h < cfg->horizon_count; ++h) {
+        if (cfg->horizons[h] <= 0) throw std::invalid_argument("horizons must be positive");
+    }
+    if (!(cfg->request_rate_hz > 0.0) || !(cfg->duration_s > 0.0)) {
+        throw std::invalid_argument("rate and duration must be positive");
+    }
+    if (cfg->max_in_flight <= 0 || cfg->max_in_flight > 8) throw std::invalid_argument("1..8 launches in flight");
+    if (cfg->reset_every < 0 || cfg->launch_budget_bytes < 0) throw std::invalid_argument("negative budget or reset");
+}
+
+// Client c's arena range: counts uniform in [min_arenas, max_arenas], ranges back to back from arena 0.
+void ClientArenas(const rg_serve_config* cfg, std::vector<int32_t>& first, std::vector<int32_t>& count) {
+    uint64_t rng = cfg->seed ^ 0x5e7a11c0ffeeull;
+    first.assign(cfg->clients, 0);
+    count.assign(cfg->clients, 0);
+    int64_t next = 0;
+    for (int c = 0; c < cfg->clients; ++c) {
+        const int span = cfg->max_arenas - cfg->min_arenas + 1;
+        count[c] = cfg->min_arenas + static_cast<int32_t>(rgw::SplitMix64(rng) % static_cast<uint64_t>(span));
+        first[c] = static_cast<int32_t>(next);
+        next += count[c];
+    }
+}
+
+// Client c's requests: Poisson arrivals at rate / clients over the run, each with a horizon drawn from the list.
+template <typename F>
+void ClientArrivals(const rg_serve_config* cfg, int32_t c, F&& add) {
+    const double per_client = cfg->request_rate_hz / cfg->clients;
+    uint64_t s = rgw::StreamSeed(cfg->seed, static_cast<uint64_t>(c), 0, 0x5e);
+    double t = Exponential(s, per_client);
+    while (t < cfg->duration_s) {
+        const int h = static_cast<int>(rgw::SplitMix64(s) % static_cast<uint64_t>(cfg->horizon_count));
+        add(t, cfg->horizons[h]);
+        t += Exponential(s, per_client);
+    }
+}
+
 }  // namespace
 
 extern "C" int rg_serve_run(rg_batch* batch, const rg_serve_config* cfg, rg_serve_report* report,
@@ -62,48 +113,49 @@ extern "C" int rg_serve_run(rg_batch* batch, const rg_serve_config* cfg, rg_serv
     std::vector<Slot> slots;
     int rc = RG_OK;
     try {
-        if (!batch || !cfg || !report) throw std::invalid_argument("null argument");
+        if (!batch || !report) throw std::invalid_argument("null argument");
+        CheckConfig(cfg);
         const int64_t env_count = rg_batch_env_count(batch);
-        if (cfg->clients <= 0 || cfg->min_arenas <= 0 || cfg->max_arenas < cfg->min_arenas) {
-            throw std::invalid_argument("clients and arenas per client must be positive");
-        }
-        if (cfg->horizon_count <= 0 || cfg->horizon_count > 4) throw std::invalid_argument("1..4 horizons");
-        for (int h = 0; h < cfg->horizon_count; ++h) {
-            if (cfg->horizons[h] <= 0) throw std::invalid_argument("horizons must be positive");
-        }
-        if (!(cfg->request_rate_hz > 0.0) || !(cfg->duration_s > 0.0)) {
-            throw std::invalid_argument("rate and duration must be positive");
-        }
-        if (cfg->max_in_flight <= 0 || cfg->max_in_flight > 8) throw std::invalid_argument("1..8 launches in flight");
+        const int32_t max_cars = env_count > 0 ? static_cast<int32_t>(rg_batch_car_capacity(batch) / env_count) : 0;
 
-        // Arena ranges per client.
-        uint64_t rng = cfg->seed ^ 0x5e7a11c0ffeeull;
-        std::vector<int32_t> first(cfg->clients), count(cfg->clients);
-        int64_t next = 0;
-        for (int c = 0; c < cfg->clients; ++c) {
-            const int span = cfg->max_arenas - cfg->min_arenas + 1;
-            count[c] = cfg->min_arenas + static_cast<int32_t>(rgw::SplitMix64(rng) % static_cast<uint64_t>(span));
-            first[c] = static_cast<int32_t>(next);
-            next += count[c];
-        }
+        std::vector<int32_t> first, count;
+        ClientArenas(cfg, first, count);
+        const int64_t next = static_cast<int64_t>(first.back()) + count.back();
         if (next > env_count) throw std::out_of_range("clients own more arenas than the batch holds");
         const int32_t budget = cfg->max_arenas_per_launch;
         if (budget < cfg->max_arenas) throw std::invalid_argument("max_arenas_per_launch below one client's arenas");
 
-        // Arrivals: one Poisson stream per client at rate / clients, merged by time.
-        std::vector<Request> reqs;
-        const double per_client = cfg->request_rate_hz / cfg->clients;
+        // HBM bytes a request touches: the client's arena records at their car counts.
+        std::vector<int32_t> cars(static_cast<size_t>(env_count));
+        CapiCheck(rg_batch_download(batch, RG_ARRAY_ENV_CAR_COUNT, cars.data(), cars.size(), nullptr));
+        std::vector<int64_t> client_bytes(cfg->clients, 0);
+        int64_t resident_bytes = 0;
         for (int c = 0; c < cfg->clients; ++c) {
-            uint64_t s = rgw::StreamSeed(cfg->seed, static_cast<uint64_t>(c), 0, 0x5e);
-            double t = Exponential(s, per_client);
-            while (t < cfg->duration_s) {
-                const int h = static_cast<int>(rgw::SplitMix64(s) % static_cast<uint64_t>(cfg->horizon_count));
-                reqs.push_back(Request{c, cfg->horizons[h], t});
-                t += Exponential(s, per_client);
+            for (int32_t k = 0; k < count[c]; ++k) client_bytes[c] += rg_batch_arena_bytes(batch, cars[first[c] + k]);
+            resident_bytes += client_bytes[c];
+            if (cfg->launch_budget_bytes > 0 && client_bytes[c] > cfg->launch_budget_bytes) {
+                throw std::invalid_argument("launch_budget_bytes below one client's arena records");
             }
+        }
+
+        // Arrivals: one Poisson stream per client, merged by time; each client's requests numbered in order.
+        std::vector<Request> reqs;
+        for (int c = 0; c < cfg->clients; ++c) {
+            ClientArrivals(cfg, c, [&](double t, int32_t ticks) { reqs.push_back(Request{c, ticks, t}); });
         }
         std::stable_sort(reqs.begin(), reqs.end(),
                          [](const Request& a, const Request& b) { return a.arrival_s < b.arrival_s; });
+        std::vector<int64_t> served(cfg->clients, 0);  // requests of each client admitted so far
+
+        // Device addresses of the controls (fresh controls are written there) and the reset commands.
+        rg_device_view view{};
+        CapiCheck(rg_batch_device_view(batch, &view));
+        const bool resets = cfg->reset_every > 0;
+        const bool controls = cfg->fresh_controls != 0 && max_cars > 0;
+        if (resets) {
+            // Record the settled kickoff rows now, outside the timed loop: an empty command list applies nothing.
+            CapiCheck(rg_batch_apply_env_commands_device(batch, view.env_car_count, 0, nullptr));
+        }
 
         slots.resize(static_cast<size_t>(cfg->max_in_flight));
         for (Slot& sl : slots) {
@@ -114,6 +166,26 @@ extern "C" int rg_serve_run(rg_batch* batch, const rg_serve_config* cfg, rg_serv
                      "hipHostMalloc");
             HipCheck(hipMalloc(reinterpret_cast<void**>(&sl.dev_ids), budget * sizeof(int32_t)), "hipMalloc");
             HipCheck(hipMalloc(reinterpret_cast<void**>(&sl.dev_ticks), budget * sizeof(int32_t)), "hipMalloc");
+            if (controls) {
+                HipCheck(hipHostMalloc(reinterpret_cast<void**>(&sl.host_controls),
+                                       static_cast<size_t>(budget) * max_cars * sizeof(rgw::Controls)),
+                         "hipHostMalloc");
+            }
+            if (resets) {
+                HipCheck(hipMalloc(reinterpret_cast<void**>(&sl.dev_commands), env_count * sizeof(rg::EnvCommand)),
+                         "hipMalloc");
+                HipCheck(hipMemset(sl.dev_commands, 0, env_count * sizeof(rg::EnvCommand)), "hipMemset");
+                HipCheck(hipHostMalloc(reinterpret_cast<void**>(&sl.host_reset),
+                                       static_cast<size_t>(cfg->max_arenas) * sizeof(rg::EnvCommand)),
+                         "hipHostMalloc");
+                for (int32_t k = 0; k < cfg->max_arenas; ++k) {
+                    rg::EnvCommand cmd{};
+                    cmd.reset_to_kickoff = 1;
+                    cmd.reset_scoreboard = 1;
+                    cmd.kickoff_seed_mode = 3;
+                    sl.host_reset[k] = cmd;
+                }
+            }
         }
         HipCheck(hipDeviceSynchronize(), "hipDeviceSynchronize");
 
@@ -159,23 +231,34 @@ extern "C" int rg_serve_run(rg_batch* batch, const rg_serve_config* cfg, rg_serv
             for (Slot& sl : slots) {
                 if (sl.busy || ready.empty()) continue;
                 int32_t n = 0;
+                int64_t bytes = 0;
                 std::deque<int32_t> deferred;
+                sl.reset_ranges.clear();
                 while (!ready.empty()) {
                     const int32_t c = ready.front();
                     ready.pop_front();
-                    if (n + count[c] > budget) {
+                    if (n + count[c] > budget ||
+                        (cfg->launch_budget_bytes > 0 && bytes + client_bytes[c] > cfg->launch_budget_bytes)) {
                         deferred.push_back(c);
                         continue;
                     }
                     const int32_t r = pending[c].front();
                     pending[c].pop_front();
                     busy[c] = 1;
+                    const int64_t number = served[c]++;
                     for (int32_t k = 0; k < count[c]; ++k) {
                         sl.host_ids[n + k] = first[c] + k;
                         sl.host_ticks[n + k] = reqs[r].ticks;
                         if (host_ticks_per_arena) host_ticks_per_arena[first[c] + k] += reqs[r].ticks;
                     }
+                    if (controls) {
+                        CapiCheck(rg_workload_request_controls(cfg->seed, number, first[c], count[c], max_cars,
+                                                               cfg->p_boost, sl.host_controls +
+                                                                   static_cast<size_t>(n) * max_cars));
+                    }
+                    if (resets && (number + 1) % cfg->reset_every == 0) sl.reset_ranges.push_back({n, c});
                     n += count[c];
+                    bytes += client_bytes[c];
                     arena_ticks += static_cast<int64_t>(count[c]) * reqs[r].ticks;
                     sl.requests.push_back(r);
                 }
@@ -188,6 +271,37 @@ extern "C" int rg_serve_run(rg_batch* batch, const rg_serve_config* cfg, rg_serv
                                         sl.stream), "hipMemcpyAsync");
                 HipCheck(hipMemcpyAsync(sl.dev_ticks, sl.host_ticks, n * sizeof(int32_t), hipMemcpyHostToDevice,
                                         sl.stream), "hipMemcpyAsync");
+                // Resets first (the settled kickoff of the client's arenas), then the request's controls.
+                if (!sl.reset_ranges.empty()) {
+                    for (const auto& rc : sl.reset_ranges) {
+                        const int32_t c = rc.second;
+                        HipCheck(hipMemcpyAsync(sl.dev_commands + first[c], sl.host_reset,
+                                                count[c] * sizeof(rg::EnvCommand), hipMemcpyHostToDevice, sl.stream),
+                                 "hipMemcpyAsync");
+                    }
+                    CapiCheck(rg_batch_apply_env_commands_device(batch, sl.dev_commands, static_cast<size_t>(env_count),
+                                                                 sl.stream));
+                    for (const auto& rc : sl.reset_ranges) {
+                        const int32_t c = rc.second;
+                        HipCheck(hipMemsetAsync(sl.dev_commands + first[c], 0, count[c] * sizeof(rg::EnvCommand),
+                                                sl.stream),
+                                 "hipMemsetAsync");
+                    }
+                }
+                if (controls) {
+                    // The launch's arenas are whole client ranges: copy each client's rows to its arena slots.
+                    int32_t at = 0;
+                    for (const int32_t r : sl.requests) {
+                        const int32_t c = reqs[r].client;
+                        HipCheck(hipMemcpyAsync(static_cast<rgw::Controls*>(view.controls) +
+                                                    static_cast<size_t>(first[c]) * max_cars,
+                                                sl.host_controls + static_cast<size_t>(at) * max_cars,
+                                                static_cast<size_t>(count[c]) * max_cars * sizeof(rgw::Controls),
+                                                hipMemcpyHostToDevice, sl.stream),
+                                 "hipMemcpyAsync");
+                        at += count[c];
+                    }
+                }
                 CapiCheck(rg_batch_step_arenas(batch, sl.dev_ids, sl.dev_ticks, n, 0, sl.stream));
                 HipCheck(hipEventRecord(sl.done, sl.stream), "hipEventRecord");
                 sl.busy = true;
@@ -226,6 +340,7 @@ extern "C" int rg_serve_run(rg_batch* batch, const rg_serve_config* cfg, rg_serv
         report->mean_ms = latency.empty() ? 0.0 : sum / latency.size() * 1e3;
         report->max_ms = latency.empty() ? 0.0 : latency.back() * 1e3;
         report->arenas_resident = next;
+        report->resident_bytes = resident_bytes;
     } catch (const std::invalid_argument& e) {
         rg_set_last_error(e.what());
         rc = RG_EINVAL;
@@ -242,8 +357,43 @@ extern "C" int rg_serve_run(rg_batch* batch, const rg_serve_config* cfg, rg_serv
         if (sl.dev_ticks) (void)hipFree(sl.dev_ticks);
         if (sl.host_ids) (void)hipHostFree(sl.host_ids);
         if (sl.host_ticks) (void)hipHostFree(sl.host_ticks);
+        if (sl.host_controls) (void)hipHostFree(sl.host_controls);
+        if (sl.host_reset) (void)hipHostFree(sl.host_reset);
+        if (sl.dev_commands) (void)hipFree(sl.dev_commands);
         if (sl.done) (void)hipEventDestroy(sl.done);
         if (sl.stream) (void)hipStreamDestroy(sl.stream);
     }
     return rc;
+}
+
+extern "C" int rg_serve_client_arenas(const rg_serve_config* cfg, int32_t client, int32_t* first, int32_t* count) {
+    try {
+        CheckConfig(cfg);
+        if (client < 0 || client >= cfg->clients || !first || !count) throw std::invalid_argument("bad client");
+        std::vector<int32_t> f, c;
+        ClientArenas(cfg, f, c);
+        *first = f[client];
+        *count = c[client];
+        return RG_OK;
+    } catch (const std::exception& e) {
+        rg_set_last_error(e.what());
+        return RG_EINVAL;
+    }
+}
+
+extern "C" int64_t rg_serve_client_requests(const rg_serve_config* cfg, int32_t client, int32_t* out_ticks,
+                                            int64_t capacity) {
+    try {
+        CheckConfig(cfg);
+        if (client < 0 || client >= cfg->clients) throw std::invalid_argument("bad client");
+        int64_t n = 0;
+        ClientArrivals(cfg, client, [&](double, int32_t ticks) {
+            if (out_ticks && n < capacity) out_ticks[n] = ticks;
+            ++n;
+        });
+        return n;
+    } catch (const std::exception& e) {
+        rg_set_last_error(e.what());
+        return -1;
+    }
 }

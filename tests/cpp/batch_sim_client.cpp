@@ -14,6 +14,8 @@
 
 namespace {
 
+constexpr uint64_t kFnvBasis = 0xcbf29ce484222325ull;  // FNV-1a 64 offset basis
+
 uint64_t Fnv1a(uint64_t h, const void* data, size_t n) {
     const unsigned char* p = static_cast<const unsigned char*>(data);
     for (size_t i = 0; i < n; ++i) {
@@ -26,8 +28,8 @@ uint64_t Fnv1a(uint64_t h, const void* data, size_t n) {
 }  // namespace
 
 int main(int argc, char** argv) {
-    if (argc != 4) {
-        std::fprintf(stderr, "usage: batch_sim_client <arenas> <ticks> <seed>\n");
+    if (argc != 4 && argc != 5) {
+        std::fprintf(stderr, "usage: batch_sim_client <arenas> <ticks> <seed> [kickoff dump file]\n");
         return 2;
     }
     const int n = std::atoi(argv[1]), ticks = std::atoi(argv[2]);
@@ -65,6 +67,29 @@ int main(int argc, char** argv) {
         }
         if (!refused) return 1;
         sim.reset_to_kickoff(seed);
+        const auto stage_digest = [&](const char* what) {
+            std::vector<rocketgpu_amd::CarState> c(cap);
+            std::vector<rocketgpu_amd::BallState> b(n);
+            sim.download_car_states(c.data(), cap);
+            sim.download_ball_states(b.data(), n);
+            std::printf("stage %s %016llx\n", what,
+                        static_cast<unsigned long long>(Fnv1a(Fnv1a(kFnvBasis, c.data(),
+                                                                    cap * sizeof(c[0])),
+                                                              b.data(), n * sizeof(b[0]))));
+        };
+        stage_digest("kickoff");
+        if (argc > 4) {
+            // Dump the kickoff records (car states, then ball states) for a byte comparison.
+            std::vector<rocketgpu_amd::CarState> c(cap);
+            std::vector<rocketgpu_amd::BallState> b(n);
+            sim.download_car_states(c.data(), cap);
+            sim.download_ball_states(b.data(), n);
+            if (FILE* f = std::fopen(argv[4], "wb")) {
+                std::fwrite(c.data(), sizeof(c[0]), cap, f);
+                std::fwrite(b.data(), sizeof(b[0]), n, f);
+                std::fclose(f);
+            }
+        }
 
         std::vector<rocketgpu_amd::CarControls> controls(static_cast<size_t>(ticks) * cap);
         if (rg_workload_controls(seed, 0, n, 4, ticks, 0.3f, controls.data()) != RG_OK) return 1;
@@ -82,6 +107,8 @@ int main(int argc, char** argv) {
             const int k = ticks - t < 8 ? ticks - t : 8;
             sim.step_with_device_controls(d_controls + static_cast<size_t>(t) * cap, k, static_cast<int32_t>(cap),
                                           stream);
+            sim.synchronize(stream);
+            stage_digest("step");
         }
         sim.synchronize(stream);
 
@@ -94,10 +121,10 @@ int main(int argc, char** argv) {
         sim.download_ball_states_async(balls.data(), n, stream);
         copy.download_car_states(cars_copy.data(), cap, stream);
         sim.synchronize(stream);
-        const uint64_t h = Fnv1a(Fnv1a(1469598103934665603ull, cars.data(), cap * sizeof(cars[0])), balls.data(),
+        const uint64_t h = Fnv1a(Fnv1a(kFnvBasis, cars.data(), cap * sizeof(cars[0])), balls.data(),
                                  n * sizeof(balls[0]));
-        const uint64_t hc = Fnv1a(1469598103934665603ull, cars_copy.data(), cap * sizeof(cars[0]));
-        const uint64_t hs = Fnv1a(1469598103934665603ull, cars.data(), cap * sizeof(cars[0]));
+        const uint64_t hc = Fnv1a(kFnvBasis, cars_copy.data(), cap * sizeof(cars[0]));
+        const uint64_t hs = Fnv1a(kFnvBasis, cars.data(), cap * sizeof(cars[0]));
         std::printf("copy_state_from %s\n", hc == hs ? "equal" : "DIFFERENT");
         (void)hipFree(d_controls);
         (void)hipStreamDestroy(stream);

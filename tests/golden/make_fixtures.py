@@ -32,6 +32,7 @@ FIXTURES = {
     "ball_arena_96_t1000": (96, 1000, 100, 7, 1.0, 256.0),
     "ball_arena_1024_t3000": (1024, 3000, 500, 11, 1.0, 256.0),
     "ball_arena_small_scale_64_t1500": (64, 1500, 500, 13, 0.8, 192.0),
+    "ball_arena_256_t10000": (256, 10000, 2000, 17, 1.0, 256.0),
 }
 
 

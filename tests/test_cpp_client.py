@@ -38,7 +38,7 @@ def test_cpp_client_matches_python_binding():
     lines = run.stdout.strip().splitlines()
     assert lines[0] == "mutator gravity_z -600.0"
     assert lines[1].startswith("refused: mutator field sticky_force")
-    assert lines[2] == "copy_state_from equal"
+    assert "copy_state_from equal" in lines
     digest = lines[-1].split()[1]
 
     counts = np.array([2 + 2 * (e % 2) for e in range(n)], np.int32)
@@ -51,10 +51,18 @@ def test_cpp_client_matches_python_binding():
     sim.set_car_configs(cfgs)
     sim.set_mutator_config(gravity_z=-600.0)
     sim.reset_to_kickoff(seed)
+
+    def stage():
+        cars = sim.download_state_array("car_states").tobytes()
+        balls = sim.download_state_array("ball_states").tobytes()
+        return f"{_fnv(balls, _fnv(cars)):016x}"
+
+    stages = [stage()]
     ctrl = torch.from_numpy(rga.workload_controls(seed, 0, n, 4, ticks).reshape(ticks, -1)).cuda()
     for t in range(0, ticks, 8):
         sim.step_with_device_controls(ctrl.data_ptr() + t * ctrl.shape[1], min(8, ticks - t), cap)
-    cars = sim.download_state_array("car_states").tobytes()
-    balls = sim.download_state_array("ball_states").tobytes()
+        stages.append(stage())
     sim.close()
-    assert f"{_fnv(balls, _fnv(cars)):016x}" == digest
+    cpp_stages = [line.split()[2] for line in lines if line.startswith("stage ")]
+    assert cpp_stages == stages
+    assert stages[-1] == digest

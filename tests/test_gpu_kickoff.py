@@ -133,13 +133,14 @@ def test_gpu_car_env_commands_match_reference(n_env, cars, mixed):
     ref.close()
 
 
-@pytest.mark.parametrize("per_wave", [32, 8, 4])
-def test_gpu_car_arenas_per_wave_agree(per_wave):
-    """The car kernel's arenas-per-wavefront launch knob changes nothing in the results."""
+@pytest.mark.parametrize("stage,per_wave", [(0, 32), (0, 8), (0, 4), (2, 32), (2, 8), (2, 4)])
+def test_gpu_car_arenas_per_wave_agree(stage, per_wave):
+    """The car kernel's launch knobs (arenas per wavefront; per-tick data in lane-private memory or LDS) change
+    nothing in the results."""
     torch = _torch()
     import rocketgpu_amd as rga
 
-    say(f"case: 96 mixed arenas, arenas per wave 64 vs {per_wave}")
+    say(f"case: 96 mixed arenas, arenas per wave 64 vs stage {stage}, {per_wave} per wave")
     n = 96
     counts = np.array([2, 4, 6] * (n // 3), np.int32)
     presets = np.arange(n * 6) % 6
@@ -147,9 +148,9 @@ def test_gpu_car_arenas_per_wave_agree(per_wave):
     cfgs = np.stack([rga.car_preset_config(int(p)) for p in presets])
     ctrl = torch.from_numpy(rga.workload_controls(9, 0, n, 6, 64).reshape(64, -1)).cuda()
     out = []
-    for pw in (64, per_wave):
+    for st, pw in ((0, 64), (stage, per_wave)):
         sim = _batch(n, 6, counts, teams, cfgs)
-        sim.set_launch_shape(64, 0, pw)
+        sim.set_launch_shape(64, st, pw)
         sim.reset_to_kickoff(9)
         for k in range(8):
             sim.step_with_device_controls(ctrl.data_ptr() + k * 8 * ctrl.shape[1], 8, n * 6)
