@@ -15,7 +15,7 @@
 // unsupported word (rg_car_tick.hpp lists the bits).
 #pragma once
 
-#include "rg_car_tick.hpp"
+#include "rg_car_world.hpp"
 
 namespace rg {
 namespace core {
@@ -32,6 +32,12 @@ struct CarArenaTick {
     Vec3 predict_ang[kMaxArenaCars];
     ManifoldEntry entries[kMaxManifoldEntries];
     int entry_count;
+    // Car-mesh contacts: each car's solved velocity change and split pseudo-velocities, and the rows of the car
+    // being processed.
+    Vec3 world_dlin[kMaxArenaCars], world_dang[kMaxArenaCars];
+    Vec3 world_push[kMaxArenaCars], world_turn[kMaxArenaCars];
+    uint8_t world_vel[kMaxArenaCars], world_split[kMaxArenaCars];
+    CarWorldRows world_rows;
 };
 
 // Where an arena's records live for one tick.
@@ -929,14 +935,85 @@ RG_HD void StepCarArena(CarArenaRefs& a, const ArenaMeshView& mesh, const StepPa
     if (awake_rows && dispatched) {
         BallMeshContacts<SerialWalk, false>(ball, p, st, mesh, t.ball, ext_lin_uu, solver_vel_bt, dt, EventSink{});
     }
-    // Car-mesh contacts (screen only).
+    // Car-mesh contacts: each car against the meshes its broadphase pairs list (and those holding retained points),
+    // its rows solved on their own (they touch no other body).
+    bool car_vel_rows = false, car_split_rows = false;
     for (int i = 0; i < count; ++i) {
-        if (t.pre[i].is_demoed || a.cars[i].just_respawned) continue;
-        if (CarNearMesh(t.pose[i], mesh, 0.2f)) unsupported |= kUnsupportedCarWorld;
+        t.world_vel[i] = t.world_split[i] = 0;
+        CarWorldManifold* retained = st.car_world[i];
+        if (t.pre[i].is_demoed || a.cars[i].just_respawned) {
+            for (int k = 0; k < kCarWorldManifoldsPerCar; ++k) retained[k].count = 0;
+            continue;
+        }
+        uint32_t mesh_bits = 0u;
+        for (int e = 0; e < t.entry_count; ++e) {
+            const uint32_t key = t.entries[e].key;
+            const int sid = KeyStatic(key);
+            if (KeyKind(key) == 2 && KeyBodyA(key) == i && sid != kPredictiveStaticId && sid >= 0 && sid < 32) {
+                mesh_bits |= 1u << sid;
+            }
+        }
+        bool held = false;
+        for (int k = 0; k < kCarWorldManifoldsPerCar; ++k) held = held || retained[k].count > 0;
+        if (mesh_bits == 0u && !held) continue;
+        CarState& car = a.cars[i];
+        if (!car.basis_authoritative) {
+            unsupported |= kUnsupportedCarWorld;
+            continue;
+        }
+        CarWorldRows& wr = t.world_rows;
+        CarWorldContacts(car, a.cfgs[i], m, mesh, mesh_bits, retained, dt, wr);
+        if (wr.hit) {
+            car.world_contact.has_contact = 1;
+            car.world_contact.contact_normal = wr.latest_normal;
+        }
+        const float inv_mass = 1.0f / m.car_mass;
+        if (wr.nn > 0 || wr.nf > 0) {
+            SolveCarWorldVelocity(wr, inv_mass, t.world_dlin[i], t.world_dang[i]);
+            t.world_vel[i] = 1;
+            car_vel_rows = true;
+        }
+        if (wr.ns > 0) {
+            SolveCarWorldSplit(wr, inv_mass, t.world_push[i], t.world_turn[i]);
+            t.world_split[i] = 1;
+            car_split_rows = true;
+        }
     }
     if (t.ball.predictive_hit) BallPredictiveRows(ball, p, st, t.ball, ext_lin_uu, solver_vel_bt, dt);
-    SolveBallVelocity(ball, t.ball, solver_vel_bt);
+    // Velocity solve.  With rows anywhere in the arena every body takes its accumulated change (zero for bodies
+    // without rows), as the arena-wide solve writes them all back.
+    const bool ball_vel_rows = t.ball.rows.n_vel > 0 || t.ball.rows.n_fric > 0;
+    if (ball_vel_rows) {
+        SolveBallVelocity(ball, t.ball, solver_vel_bt);
+    } else if (car_vel_rows) {
+        SetBallVelBt(ball, solver_vel_bt + Vec3(0.0f, 0.0f, 0.0f));
+        ball.ang_vel += Vec3(0.0f, 0.0f, 0.0f);
+    }
+    if (ball_vel_rows || car_vel_rows) {
+        for (int i = 0; i < count; ++i) {
+            const Vec3 dl = t.world_vel[i] ? t.world_dlin[i] : Vec3(0.0f, 0.0f, 0.0f);
+            const Vec3 da = t.world_vel[i] ? t.world_dang[i] : Vec3(0.0f, 0.0f, 0.0f);
+            SetCarVelBt(a.cars[i], a.cars[i].vel_bt + dl);
+            a.cars[i].ang_vel += da;
+        }
+    }
+    // Split impulses.
     SolveBallSplit(ball, t.ball, dt);
+    if (t.ball.rows.n_split > 0 || car_split_rows) {
+        for (int i = 0; i < count; ++i) {
+            CarState& car = a.cars[i];
+            if (!t.world_split[i]) {
+                SetCarPosBt(car, car.pos_bt + Vec3(0.0f, 0.0f, 0.0f) * dt);
+                continue;
+            }
+            const Vec3 push = t.world_push[i], turn = t.world_turn[i];
+            const bool nonzero = push.x != 0.0f || push.y != 0.0f || push.z != 0.0f || turn.x != 0.0f ||
+                                 turn.y != 0.0f || turn.z != 0.0f;
+            if (!nonzero) continue;
+            SetCarPosBt(car, car.pos_bt + push * dt);
+            IntegrateBasis(car.basis_forward, car.basis_right, car.basis_up, car.rot, turn * 0.1f, dt);
+        }
+    }
     if (!sleeping) SetBallVelBt(ball, ball.vel_bt + UuToBtDiv(ext_lin_uu));
     if (awake_rows) {
         const Vec3 v = ball.vel_bt;

@@ -107,6 +107,26 @@ struct BallMeshManifold {
     BallManifoldPoint pts[4];
 };
 
+// A car's persistent contact manifold against one mesh (Bullet's btPersistentManifold between the hitbox and the
+// mesh shape): up to four points, each the car-body-frame point, the mesh point and the normal when it was found,
+// and its distance at the last refresh (bt).
+struct CarWorldPoint {
+    Vec3 local_a;
+    Vec3 point_b;
+    Vec3 normal;
+    float distance;
+};
+
+struct CarWorldManifold {
+    uint8_t count;
+    uint8_t mesh;
+    uint8_t pad[2];
+    CarWorldPoint pts[4];
+};
+
+constexpr int kCarWorldManifoldsPerCar = 2;
+constexpr int kMaxArenaCarSlots = kMaxBodies - 1;
+
 // Internal per-arena state that persists across ticks (the parts of the reference's WorldSolverScratch the ball
 // path carries).
 struct ArenaInternal {
@@ -118,6 +138,8 @@ struct ArenaInternal {
     // Car arenas: the physics the tick met that this build does not step (kUnsupported* bits), sticky until reset.
     uint32_t unsupported;
     BallMeshManifold ball_mesh[kMaxArenaMeshes];
+    // Car arenas: each car slot's hitbox-vs-mesh manifolds (two slots, each bound to one mesh while it holds points).
+    CarWorldManifold car_world[kMaxArenaCarSlots][kCarWorldManifoldsPerCar];
 };
 
 // ------------------------------------------------------------------------------------------------------------

@@ -76,6 +76,9 @@ struct CarParams {
     // Read by the flip torque.
     float flip_torque_x = 260.0f;
     float flip_torque_y = 224.0f;
+    // Read by the car's contact rows against the arena mesh.
+    float arena_friction = 0.30f;
+    float arena_restitution = 0.30f;
 };
 
 // ------------------------------------------------------------------------------------------------------------

@@ -1379,6 +1379,12 @@ private:
         for (int m = 0; m < core::kMaxArenaMeshes; ++m) {
             if (st.ball_mesh[m].count > 4) return false;
         }
+        for (int c = 0; c < core::kMaxArenaCarSlots; ++c) {
+            for (int k = 0; k < core::kCarWorldManifoldsPerCar; ++k) {
+                const core::CarWorldManifold& w = st.car_world[c][k];
+                if (w.count > 4 || (w.count > 0 && w.mesh >= meshes)) return false;
+            }
+        }
         return true;
     }
 

@@ -70,8 +70,9 @@ def _setup(n_env, cars, seed, presets=None):
     return R, ref, sim, T, h
 
 
-@pytest.mark.parametrize("n_env,cars,ticks,seed,mixed", [(96, 2, 600, 2024, False), (48, 6, 240, 77, True)])
-def test_gpu_cars_match_twin_and_reference(n_env, cars, ticks, seed, mixed):
+@pytest.mark.parametrize("n_env,cars,ticks,seed,mixed,min_frac",
+                         [(96, 2, 600, 2024, False, 0.95), (48, 6, 240, 77, True, 0.75)])
+def test_gpu_cars_match_twin_and_reference(n_env, cars, ticks, seed, mixed, min_frac):
     torch = _torch()
     import rocketgpu_amd as rga
 
@@ -99,7 +100,7 @@ def test_gpu_cars_match_twin_and_reference(n_env, cars, ticks, seed, mixed):
         if t % 100 == 99:
             say(f"  tick {t + 1}: {int(live.sum())} of {n_env} arenas unflagged")
     say(f"  arena-ticks compared with the reference: {compared} of {n_env * ticks}")
-    assert compared >= 0.3 * n_env * ticks
+    assert compared >= min_frac * n_env * ticks
     T.rgt_cars_destroy(h)
     sim.close()
     ref.close()
