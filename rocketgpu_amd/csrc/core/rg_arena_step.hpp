@@ -15,7 +15,7 @@
 // unsupported word (rg_car_tick.hpp lists the bits).
 #pragma once
 
-#include "rg_car_world.hpp"
+#include "rg_car_car.hpp"
 
 namespace rg {
 namespace core {
@@ -38,6 +38,11 @@ struct CarArenaTick {
     Vec3 world_push[kMaxArenaCars], world_turn[kMaxArenaCars];
     uint8_t world_vel[kMaxArenaCars], world_split[kMaxArenaCars];
     CarWorldRows world_rows;
+    // Car-car contacts: each car's tick-start velocity (uu), the pair rows, and the mesh rows of the cars that
+    // touch another car (solved jointly with the pair rows).
+    Vec3 start_vel[kMaxArenaCars];
+    CarPairRows pair_rows;
+    CarGroupWorldRows group_rows;
 };
 
 // Where an arena's records live for one tick.
@@ -705,10 +710,6 @@ RG_HD bool CarNearMesh(const CarProxyPose& pose, const ArenaMeshView& mesh, floa
     return near;
 }
 
-RG_HD float CarBoundingRadius(const CarConfig& cfg) {
-    return Length(cfg.hitbox_size * 0.5f) + Length(cfg.hitbox_offset) + 5.0f;
-}
-
 // ------------------------------------------------------------------------------------------------------------
 // The tick.
 
@@ -750,7 +751,7 @@ RG_HD void StepCarArena(CarArenaRefs& a, const ArenaMeshView& mesh, const StepPa
         rc.team = a.teams[i];
         rc.car_id = static_cast<uint32_t>(a.car_base + i + 1);
         rc.env_tick = env.tick;
-        CarPreTick(a.cars[i], a.controls[i], a.cfgs[i], m, &a.wheels[i], rc);
+        CarPreTick(a.cars[i], a.controls[i], a.cfgs[i], m, &a.wheels[i], rc, &t.start_vel[i]);
         t.predict_vel[i] = a.cars[i].vel_bt;
         t.predict_ang[i] = a.cars[i].ang_vel;
     }
@@ -908,20 +909,65 @@ RG_HD void StepCarArena(CarArenaRefs& a, const ArenaMeshView& mesh, const StepPa
     B.pose = t.pose;
     B.body_count = MinI(count + 1, kMaxBodies);
     ArenaBroadphase(st, mesh, t, B, predictive, sleeping && !vehicle_woke);
-    // Body pairs the tick would have to resolve.
+    // Ball-car pairs the tick would have to resolve.
     for (int i = 0; i < count; ++i) {
         const CarState& ci = a.cars[i];
-        const bool live_i = !ci.is_demoed && !ci.just_respawned;
-        for (int j = i + 1; j < count; ++j) {
-            const CarState& cj = a.cars[j];
-            const bool live = live_i && !cj.is_demoed && !cj.just_respawned;
-            const float rr = CarBoundingRadius(a.cfgs[j]) + CarBoundingRadius(a.cfgs[i]);
-            if (live && LengthSq(ci.pos - cj.pos) < rr * rr) unsupported |= kUnsupportedCarCar;
-        }
         if (t.proxy[i + 1].valid && !ci.just_respawned &&
             AabbOverlap(t.proxy[i + 1].child_min, t.proxy[i + 1].child_max, t.proxy[0].child_min,
                         t.proxy[0].child_max)) {
             unsupported |= kUnsupportedBallCar;
+        }
+    }
+    // Car-car contacts: the pairs in the contact-pair list's order, then the other pairs in slot order, each skipped
+    // while its manifold is empty and its cars' bounding spheres are apart (or a car is out of play).
+    CarPairRows& pr = t.pair_rows;
+    pr.nn = pr.nf = pr.ns = 0;
+    pr.cars = 0u;
+    if (m.enable_car_car && count >= 2) {
+        uint32_t skip = 0u;
+        for (int hi = 1; hi < count; ++hi) {
+            for (int lo = 0; lo < hi; ++lo) {
+                const CarState& cl = a.cars[lo];
+                const CarState& ch = a.cars[hi];
+                const bool live = !cl.is_demoed && !cl.just_respawned && !ch.is_demoed && !ch.just_respawned;
+                const float reach = CarBoundingRadius(a.cfgs[hi]) + CarBoundingRadius(a.cfgs[lo]);
+                const bool apart = !(LengthSq(cl.pos - ch.pos) < reach * reach);
+                if (st.car_pair[CarPairSlot(lo, hi)].count == 0 && (apart || !live)) skip |= 1u << CarPairSlot(lo, hi);
+            }
+        }
+        int8_t order_lo[kCarPairSlots], order_hi[kCarPairSlots];
+        int n_order = 0;
+        uint32_t listed = 0u;
+        for (int e = 0; e < t.entry_count; ++e) {
+            const uint32_t key = t.entries[e].key;
+            if (KeyKind(key) != 3) continue;
+            const int lo = KeyBodyA(key), hi = KeyBodyB(key);
+            if (lo < 0 || hi >= count || lo >= hi) continue;
+            const uint32_t bit = 1u << CarPairSlot(lo, hi);
+            if (listed & bit) continue;
+            listed |= bit;
+            if (skip & bit) continue;
+            order_lo[n_order] = static_cast<int8_t>(lo);
+            order_hi[n_order++] = static_cast<int8_t>(hi);
+        }
+        for (int lo = 0; lo < count; ++lo) {
+            for (int hi = lo + 1; hi < count; ++hi) {
+                const uint32_t bit = 1u << CarPairSlot(lo, hi);
+                if ((listed | skip) & bit) continue;
+                order_lo[n_order] = static_cast<int8_t>(lo);
+                order_hi[n_order++] = static_cast<int8_t>(hi);
+            }
+        }
+        for (int k = 0; k < n_order; ++k) {
+            const int i = order_lo[k], j = order_hi[k];
+            // Car a is the one whose broadphase handle is the larger (j on a tie).
+            const bool i_first = st.bp.sweep.body_handle[i + 1] <= st.bp.sweep.body_handle[j + 1];
+            const int ia = i_first ? j : i, ib = i_first ? i : j;
+            CarPairSide sa{&a.cars[ia], &a.cfgs[ia], &t.pre[ia], t.start_vel[ia],
+                           static_cast<uint32_t>(a.car_base + ia + 1), a.teams[ia], ia};
+            CarPairSide sb{&a.cars[ib], &a.cfgs[ib], &t.pre[ib], t.start_vel[ib],
+                           static_cast<uint32_t>(a.car_base + ib + 1), a.teams[ib], ib};
+            CarPairContact(sa, sb, st.car_pair[CarPairSlot(i, j)], m, dt, pr, unsupported);
         }
     }
     // Ball-mesh contacts.
@@ -935,9 +981,12 @@ RG_HD void StepCarArena(CarArenaRefs& a, const ArenaMeshView& mesh, const StepPa
     if (awake_rows && dispatched) {
         BallMeshContacts<SerialWalk, false>(ball, p, st, mesh, t.ball, ext_lin_uu, solver_vel_bt, dt, EventSink{});
     }
-    // Car-mesh contacts: each car against the meshes its broadphase pairs list (and those holding retained points),
-    // its rows solved on their own (they touch no other body).
+    // Car-mesh contacts: each car against the meshes its broadphase pairs list (and those holding retained points).
+    // A car without car-car rows has rows that touch no other body: they are solved on their own.  The rows of the
+    // cars with car-car rows are kept for the joint solve.
     bool car_vel_rows = false, car_split_rows = false;
+    CarGroupWorldRows& gr = t.group_rows;
+    gr.nn = gr.nf = gr.ns = 0;
     for (int i = 0; i < count; ++i) {
         t.world_vel[i] = t.world_split[i] = 0;
         CarWorldManifold* retained = st.car_world[i];
@@ -968,6 +1017,10 @@ RG_HD void StepCarArena(CarArenaRefs& a, const ArenaMeshView& mesh, const StepPa
             car.world_contact.contact_normal = wr.latest_normal;
         }
         const float inv_mass = 1.0f / m.car_mass;
+        if ((pr.cars >> i) & 1u) {
+            AppendGroupWorldRows(wr, i, gr, unsupported);
+            continue;
+        }
         if (wr.nn > 0 || wr.nf > 0) {
             SolveCarWorldVelocity(wr, inv_mass, t.world_dlin[i], t.world_dang[i]);
             t.world_vel[i] = 1;
@@ -977,6 +1030,36 @@ RG_HD void StepCarArena(CarArenaRefs& a, const ArenaMeshView& mesh, const StepPa
             SolveCarWorldSplit(wr, inv_mass, t.world_push[i], t.world_turn[i]);
             t.world_split[i] = 1;
             car_split_rows = true;
+        }
+    }
+    // The joint solve of the cars with car-car rows.
+    if (pr.cars != 0u) {
+        Vec3 dl[kMaxArenaCars], da[kMaxArenaCars];
+        for (int i = 0; i < count; ++i) dl[i] = da[i] = Vec3(0.0f, 0.0f, 0.0f);
+        const float inv_mass = 1.0f / m.car_mass;
+        if (gr.nn + gr.nf + pr.nn + pr.nf > 0) {
+            SolveGroupVelocity(t.entries, t.entry_count, gr, pr, inv_mass, dl, da);
+            car_vel_rows = true;
+            for (int i = 0; i < count; ++i) {
+                if (!((pr.cars >> i) & 1u)) continue;
+                t.world_dlin[i] = dl[i];
+                t.world_dang[i] = da[i];
+                t.world_vel[i] = 1;
+            }
+        }
+        if (gr.ns + pr.ns > 0) {
+            for (int i = 0; i < count; ++i) dl[i] = da[i] = Vec3(0.0f, 0.0f, 0.0f);
+            SolveGroupSplit(t.entries, t.entry_count, gr, pr, inv_mass, dl, da);
+            car_split_rows = true;
+            uint32_t touched = 0u;
+            for (int r = 0; r < gr.ns; ++r) touched |= 1u << gr.s_car[r];
+            for (int r = 0; r < pr.ns; ++r) touched |= (1u << pr.s[r].a) | (1u << pr.s[r].b);
+            for (int i = 0; i < count; ++i) {
+                if (!((touched >> i) & 1u)) continue;
+                t.world_push[i] = dl[i];
+                t.world_turn[i] = da[i];
+                t.world_split[i] = 1;
+            }
         }
     }
     if (t.ball.predictive_hit) BallPredictiveRows(ball, p, st, t.ball, ext_lin_uu, solver_vel_bt, dt);

@@ -127,6 +127,25 @@ struct CarWorldManifold {
 constexpr int kCarWorldManifoldsPerCar = 2;
 constexpr int kMaxArenaCarSlots = kMaxBodies - 1;
 
+// The persistent contact manifold of two cars (rocketgpu.h CarCarContact): up to four points, each the point on
+// car a and the point on car b in their body frames, and the normal from a toward b when it was found (bt).  The
+// world points and distances are recomputed from these at every refresh, so only these are kept between ticks.
+struct CarPairPoint {
+    Vec3 local_a;
+    Vec3 local_b;
+    Vec3 normal;
+};
+
+struct CarPairManifold {
+    uint8_t count;
+    uint8_t pad[3];
+    CarPairPoint pts[4];
+};
+
+// One manifold per pair of car slots lo < hi, at hi * (hi - 1) / 2 + lo.
+constexpr int kCarPairSlots = kMaxArenaCarSlots * (kMaxArenaCarSlots - 1) / 2;
+RG_HD constexpr int CarPairSlot(int lo, int hi) { return hi * (hi - 1) / 2 + lo; }
+
 // Internal per-arena state that persists across ticks (the parts of the reference's WorldSolverScratch the ball
 // path carries).
 struct ArenaInternal {
@@ -140,6 +159,8 @@ struct ArenaInternal {
     BallMeshManifold ball_mesh[kMaxArenaMeshes];
     // Car arenas: each car slot's hitbox-vs-mesh manifolds (two slots, each bound to one mesh while it holds points).
     CarWorldManifold car_world[kMaxArenaCarSlots][kCarWorldManifoldsPerCar];
+    // Car arenas: each pair of car slots' hitbox-vs-hitbox manifold.
+    CarPairManifold car_pair[kCarPairSlots];
 };
 
 // ------------------------------------------------------------------------------------------------------------

@@ -79,6 +79,12 @@ struct CarParams {
     // Read by the car's contact rows against the arena mesh.
     float arena_friction = 0.30f;
     float arena_restitution = 0.30f;
+    // Read by car-car contacts and bumps.
+    uint8_t enable_car_car = 1;
+    uint8_t enable_demos = 1;
+    uint8_t enable_team_demos = 0;
+    float bump_force_scale = 1.0f;
+    float bump_cooldown_time = 0.25f;
 };
 
 // ------------------------------------------------------------------------------------------------------------
@@ -1044,7 +1050,7 @@ RG_HD bool CarPreTickSupported(const CarState& car, const CarConfig& cfg, const 
 
 // One car's pre-tick (StepEnvPreTickCar).  `ws` may be null (no wheel runtime state).
 RG_HD void CarPreTick(CarState& car, const CarControls& controls, const CarConfig& cfg, const CarParams& m,
-                      WheelRuntimeState* ws, const CarRespawnContext& rc) {
+                      WheelRuntimeState* ws, const CarRespawnContext& rc, Vec3* start_vel_out = nullptr) {
     const float dt = m.tick_dt;
     // 1. start.
     car.ball_contact_this_tick = 0;
@@ -1095,6 +1101,7 @@ RG_HD void CarPreTick(CarState& car, const CarControls& controls, const CarConfi
                                      car.tick_start_basis_up);
     const Quat start_rot = car.rot;
     const Vec3 start_vel_uu = car.vel;
+    if (start_vel_out) *start_vel_out = start_vel_uu;
     FlipZDampPrepass(car, m, dt);
     car.cal_tick_start_ang_vel = car.ang_vel;
     const Vec3 vel_before = car.vel_bt;

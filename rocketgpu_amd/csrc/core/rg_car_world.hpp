@@ -329,6 +329,7 @@ struct CarWorldRows {
     CarWorldRow n[kMaxCarWorldRows];
     CarWorldFrictionRow f[kMaxCarWorldRows];
     CarWorldSplitRow s[kMaxCarWorldRows];
+    int8_t n_mesh[kMaxCarWorldRows], s_mesh[kMaxCarWorldRows];  // the mesh of each non-penetration / split row
     int nn, nf, ns;
     uint8_t hit;         // a manifold held a point this tick
     Vec3 latest_normal;  // the normal recorded as the car's world contact
@@ -549,7 +550,10 @@ RG_HD void CarWorldContacts(CarState& car, const CarConfig& cfg, const CarParams
             }
             n = PointsFromManifold(rm, f.body, pts);
         }
+        const int nn0 = out.nn, ns0 = out.ns;
         EmitRows(car, cfg, m, pts, n, dt, out);
+        for (int r = nn0; r < out.nn; ++r) out.n_mesh[r] = static_cast<int8_t>(mi);
+        for (int r = ns0; r < out.ns; ++r) out.s_mesh[r] = static_cast<int8_t>(mi);
     }
     if (out.hit) {
         out.latest_normal = last_normal;

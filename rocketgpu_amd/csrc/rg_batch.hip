@@ -594,6 +594,8 @@ public:
         params_.enable_scoring = cfg.enable_scoring;
         car_args_ = CarStepArgs{};
         car_args_.m.tick_dt = cfg.tick_dt;
+        car_args_.m.enable_car_car = cfg.enable_car_car_collisions ? 1 : 0;
+        car_args_.m.enable_demos = cfg.enable_demos ? 1 : 0;
         car_args_.m.frame_frac = params_.frame_frac;
         car_args_.rules = SpawnRules(rules);
         for (int k = 0; k < rules.pad_count && k < rgw::kMaxPads; ++k) {
@@ -1384,6 +1386,9 @@ private:
                 const core::CarWorldManifold& w = st.car_world[c][k];
                 if (w.count > 4 || (w.count > 0 && w.mesh >= meshes)) return false;
             }
+        }
+        for (int k = 0; k < core::kCarPairSlots; ++k) {
+            if (st.car_pair[k].count > 4) return false;
         }
         return true;
     }

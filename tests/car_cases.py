@@ -49,3 +49,25 @@ def batch_records(sim):
     return ({"cars": get("car_states"), "wheels": get("wheel_states"), "balls": get("ball_states"),
              "envs": get("env_states"), "pads": get("boost_pad_states")},
             get("unsupported").view(np.uint32).ravel())
+
+
+_CONTROLS = np.dtype([("throttle", "<f4"), ("steer", "<f4"), ("pitch", "<f4"), ("yaw", "<f4"), ("roll", "<f4"),
+                      ("jump", "u1"), ("boost", "u1"), ("handbrake", "u1"), ("pad", "u1")])
+
+
+def teammate_chase_controls(car_records, n_env, cars):
+    """Controls [n_env * cars, 24] u8 that steer every car at full throttle toward the next car of its own team
+    (slots 0..cars/2-1 Blue, the rest Orange), so that teammates drive into each other: car-car contacts and bumps
+    without the ball (which sits between the teams) and below supersonic speed (no boost)."""
+    from rocketgpu_amd import views
+
+    c = np.ascontiguousarray(car_records).view(views.state_dtype("car_states")).reshape(n_env, cars)
+    half = cars // 2
+    target = np.array([(k + 1) % half + (k // half) * half for k in range(cars)])
+    d = c["pos"][:, target] - c["pos"]
+    fwd = c["basis_forward"]
+    angle = np.arctan2(fwd[..., 0] * d[..., 1] - fwd[..., 1] * d[..., 0], (fwd[..., :2] * d[..., :2]).sum(-1))
+    out = np.zeros(n_env * cars, _CONTROLS)
+    out["throttle"] = 1.0
+    out["steer"] = np.clip(3.0 * angle, -1.0, 1.0).ravel()
+    return np.ascontiguousarray(out.view(np.uint8).reshape(n_env * cars, _CONTROLS.itemsize))

@@ -13,7 +13,7 @@ import sys
 import numpy as np
 import pytest
 
-from car_cases import arena_mismatches, reference_at_kickoff, reference_records
+from car_cases import arena_mismatches, reference_at_kickoff, reference_records, teammate_chase_controls
 from conftest import oracle_or_skip, twin_cars
 
 pytestmark = pytest.mark.gpu
@@ -127,6 +127,44 @@ def test_gpu_cars_multi_tick_launch_matches_reference():
     say(f"  {int(live.sum())} of {n_env} arenas unflagged after {launches * T_ticks} ticks")
     assert not live[bad].any()
     assert live.sum() >= n_env // 4
+    T.rgt_cars_destroy(h)
+    sim.close()
+    ref.close()
+
+
+def test_gpu_cars_teammates_collide_match_twin_and_reference():
+    """Teammates steered into each other (hitbox-hitbox contacts, bumps, the joint solve of touching cars): the GPU
+    equals the host twin on every tick and the reference while unflagged."""
+    torch = _torch()
+
+    n_env, cars, ticks, seed = 16, 4, 1000, 5
+    say(f"case: teammate chase, {n_env} arenas x {cars} cars, {ticks} ticks, seed {seed}")
+    R, ref, sim, T, h = _setup(n_env, cars, seed, np.arange(n_env * cars) % 6)
+    cap = n_env * cars
+    theirs = reference_records(R, ref, n_env, cars)
+    ever_flagged = np.zeros(n_env, bool)
+    for t in range(ticks):
+        ctl = teammate_chase_controls(theirs["cars"], n_env, cars)
+        dev = torch.from_numpy(ctl).to("cuda")
+        sim.step_with_device_controls(dev.data_ptr(), 1, cap)
+        ref.step_with_controls(ctl, 1, cap)
+        assert T.rgt_cars_upload(h, 4, ctl.ctypes.data, cap) == 0
+        assert T.rgt_cars_step(h, 1) == 0
+        gpu, uns = _gpu_records(sim)
+        diff_twin = arena_mismatches(gpu, _twin_records(T, h, n_env, cars), n_env, cars)
+        assert diff_twin.size == 0, f"tick {t}: GPU differs from the host twin in arenas {diff_twin[:10]}"
+        theirs = reference_records(R, ref, n_env, cars)
+        bad = arena_mismatches(gpu, theirs, n_env, cars)
+        live = uns == 0
+        assert not live[bad].any(), f"tick {t}: unflagged arenas {bad[live[bad]][:10]} differ from the reference"
+        ever_flagged |= ~live
+        if t % 250 == 249:
+            say(f"  tick {t + 1}: {int(live.sum())} of {n_env} arenas unflagged")
+    from rocketgpu_amd import views
+
+    bumps = gpu["cars"].view(views.state_dtype("car_states"))["bump_event_bumps"].reshape(n_env, -1).sum(axis=1)
+    say(f"  bumps in arenas never flagged: {int(bumps[~ever_flagged].sum())}")
+    assert bumps[~ever_flagged].sum() >= 20
     T.rgt_cars_destroy(h)
     sim.close()
     ref.close()

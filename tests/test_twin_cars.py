@@ -8,7 +8,7 @@ difference.  The test also requires most arena-ticks to be compared, so it canno
 import numpy as np
 import pytest
 
-from car_cases import arena_mismatches, reference_at_kickoff, reference_records
+from car_cases import arena_mismatches, reference_at_kickoff, reference_records, teammate_chase_controls
 from conftest import oracle_or_skip, twin_cars
 from rocketgpu_amd import views
 
@@ -72,3 +72,39 @@ def test_twin_cars_mixed_bodies_match_reference(cars):
     n_env, ticks = 6, 200
     compared, _, _ = _run(n_env, cars, ticks, seed=11 + cars, presets=np.arange(n_env * cars) % 6)
     assert compared >= 0.7 * n_env * ticks, f"only {compared} of {n_env * ticks} arena-ticks compared"
+
+
+def test_twin_cars_teammates_collide_match_reference():
+    """Teammates steered into each other: hitbox-hitbox contacts, the pair manifolds, bumps and the joint solve of
+    the touching cars, compared with the reference on every tick while unflagged."""
+    R = oracle_or_skip()
+    T = twin_cars()
+    n_env, cars, ticks, seed = 16, 4, 1000, 5
+    presets = np.arange(n_env * cars) % 6
+    ref, counts, teams, cfgs = reference_at_kickoff(R, n_env, cars, seed, presets)
+    h = T.rgt_cars_create(n_env, cars, 1.0, 256.0, seed)
+    theirs = reference_records(R, ref, n_env, cars)
+    for which, arr in ((6, counts), (7, teams), (5, cfgs), (0, theirs["cars"]), (8, theirs["wheels"]),
+                       (1, theirs["balls"]), (2, theirs["envs"]), (3, theirs["pads"])):
+        a = np.ascontiguousarray(arr)
+        assert T.rgt_cars_upload(h, which, a.ctypes.data, a.shape[0]) == 0
+    ever_flagged = np.zeros(n_env, bool)
+    compared = 0
+    for t in range(ticks):
+        ctl = teammate_chase_controls(theirs["cars"], n_env, cars)
+        assert T.rgt_cars_upload(h, 4, ctl.ctypes.data, ctl.shape[0]) == 0
+        ref.step_with_controls(ctl, 1, ctl.shape[0])
+        assert T.rgt_cars_step(h, 1) == 0
+        mine, uns = _twin_records(T, h, n_env, cars)
+        theirs = reference_records(R, ref, n_env, cars)
+        bad = arena_mismatches(mine, theirs, n_env, cars)
+        live = uns == 0
+        assert not live[bad].any(), f"tick {t}: arenas {bad[live[bad]]} differ from the reference while unflagged"
+        ever_flagged |= ~live
+        compared += int(live.sum())
+    bumps = mine["cars"].view(views.state_dtype("car_states"))["bump_event_bumps"].reshape(n_env, -1).sum(axis=1)
+    T.rgt_cars_destroy(h)
+    ref.close()
+    # Arenas that were never flagged went through many bumps (each bump is one car-car contact's outcome).
+    assert bumps[~ever_flagged].sum() >= 20, f"bumps in unflagged arenas: {bumps[~ever_flagged].sum()}"
+    assert compared >= 0.4 * n_env * ticks
