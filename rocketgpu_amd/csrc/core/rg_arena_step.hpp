@@ -38,11 +38,9 @@ struct CarArenaTick {
     Vec3 world_push[kMaxArenaCars], world_turn[kMaxArenaCars];
     uint8_t world_vel[kMaxArenaCars], world_split[kMaxArenaCars];
     CarWorldRows world_rows;
-    // Car-car contacts: each car's tick-start velocity (uu), the pair rows, and the mesh rows of the cars that
-    // touch another car (solved jointly with the pair rows).
+    // Car-car contacts: each car's tick-start velocity (uu) and the pair rows.
     Vec3 start_vel[kMaxArenaCars];
     CarPairRows pair_rows;
-    CarGroupWorldRows group_rows;
 };
 
 // Where an arena's records live for one tick.
@@ -938,8 +936,9 @@ RG_HD void StepCarArena(CarArenaRefs& a, const ArenaMeshView& mesh, const StepPa
         int8_t order_lo[kCarPairSlots], order_hi[kCarPairSlots];
         int n_order = 0;
         uint32_t listed = 0u;
-        for (int e = 0; e < t.entry_count; ++e) {
-            const uint32_t key = t.entries[e].key;
+        // The contact-pair list as the broadphase keeps it (its island sort comes after the car-car stage).
+        for (int e = 0; e < st.bp.manifold_count; ++e) {
+            const uint32_t key = st.bp.manifolds[e];
             if (KeyKind(key) != 3) continue;
             const int lo = KeyBodyA(key), hi = KeyBodyB(key);
             if (lo < 0 || hi >= count || lo >= hi) continue;
@@ -985,8 +984,9 @@ RG_HD void StepCarArena(CarArenaRefs& a, const ArenaMeshView& mesh, const StepPa
     // A car without car-car rows has rows that touch no other body: they are solved on their own.  The rows of the
     // cars with car-car rows are kept for the joint solve.
     bool car_vel_rows = false, car_split_rows = false;
-    CarGroupWorldRows& gr = t.group_rows;
-    gr.nn = gr.nf = gr.ns = 0;
+    CarWorldRows& wr = t.world_rows;
+    wr.nn = wr.nf = wr.ns = 0;
+    wr.full = 0;
     for (int i = 0; i < count; ++i) {
         t.world_vel[i] = t.world_split[i] = 0;
         CarWorldManifold* retained = st.car_world[i];
@@ -1010,35 +1010,38 @@ RG_HD void StepCarArena(CarArenaRefs& a, const ArenaMeshView& mesh, const StepPa
             unsupported |= kUnsupportedCarWorld;
             continue;
         }
-        CarWorldRows& wr = t.world_rows;
+        const int n0 = wr.nn, f0 = wr.nf, s0 = wr.ns;
         CarWorldContacts(car, a.cfgs[i], m, mesh, mesh_bits, retained, dt, wr);
+        for (int r = n0; r < wr.nn; ++r) wr.n_car[r] = static_cast<int8_t>(i);
         if (wr.hit) {
             car.world_contact.has_contact = 1;
             car.world_contact.contact_normal = wr.latest_normal;
         }
+        // Rows of a car that touches another car stay for the joint solve.
+        if ((pr.cars >> i) & 1u) continue;
         const float inv_mass = 1.0f / m.car_mass;
-        if ((pr.cars >> i) & 1u) {
-            AppendGroupWorldRows(wr, i, gr, unsupported);
-            continue;
-        }
-        if (wr.nn > 0 || wr.nf > 0) {
-            SolveCarWorldVelocity(wr, inv_mass, t.world_dlin[i], t.world_dang[i]);
+        if (wr.nn > n0 || wr.nf > f0) {
+            SolveCarWorldVelocity(wr, n0, f0, inv_mass, m.arena_friction, t.world_dlin[i], t.world_dang[i]);
             t.world_vel[i] = 1;
             car_vel_rows = true;
         }
-        if (wr.ns > 0) {
-            SolveCarWorldSplit(wr, inv_mass, t.world_push[i], t.world_turn[i]);
+        if (wr.ns > s0) {
+            SolveCarWorldSplit(wr, s0, inv_mass, t.world_push[i], t.world_turn[i]);
             t.world_split[i] = 1;
             car_split_rows = true;
         }
+        wr.nn = n0;
+        wr.nf = f0;
+        wr.ns = s0;
     }
+    if (wr.full) unsupported |= (pr.cars != 0u ? kUnsupportedCarCar : kUnsupportedCarWorld);
     // The joint solve of the cars with car-car rows.
     if (pr.cars != 0u) {
         Vec3 dl[kMaxArenaCars], da[kMaxArenaCars];
         for (int i = 0; i < count; ++i) dl[i] = da[i] = Vec3(0.0f, 0.0f, 0.0f);
         const float inv_mass = 1.0f / m.car_mass;
-        if (gr.nn + gr.nf + pr.nn + pr.nf > 0) {
-            SolveGroupVelocity(t.entries, t.entry_count, gr, pr, inv_mass, dl, da);
+        if (wr.nn + wr.nf + pr.nn + pr.nf > 0) {
+            SolveGroupVelocity(t.entries, t.entry_count, wr, pr, inv_mass, m.arena_friction, dl, da);
             car_vel_rows = true;
             for (int i = 0; i < count; ++i) {
                 if (!((pr.cars >> i) & 1u)) continue;
@@ -1047,13 +1050,16 @@ RG_HD void StepCarArena(CarArenaRefs& a, const ArenaMeshView& mesh, const StepPa
                 t.world_vel[i] = 1;
             }
         }
-        if (gr.ns + pr.ns > 0) {
+        if (wr.ns + pr.ns > 0) {
             for (int i = 0; i < count; ++i) dl[i] = da[i] = Vec3(0.0f, 0.0f, 0.0f);
-            SolveGroupSplit(t.entries, t.entry_count, gr, pr, inv_mass, dl, da);
+            SolveGroupSplit(t.entries, t.entry_count, wr, pr, inv_mass, dl, da);
             car_split_rows = true;
             uint32_t touched = 0u;
-            for (int r = 0; r < gr.ns; ++r) touched |= 1u << gr.s_car[r];
-            for (int r = 0; r < pr.ns; ++r) touched |= (1u << pr.s[r].a) | (1u << pr.s[r].b);
+            for (int r = 0; r < wr.ns; ++r) touched |= 1u << wr.n_car[wr.s[r].normal_row];
+            for (int r = 0; r < pr.ns; ++r) {
+                const CarPairRow& nr = pr.n[pr.s[r].normal_row];
+                touched |= (1u << nr.a) | (1u << nr.b);
+            }
             for (int i = 0; i < count; ++i) {
                 if (!((touched >> i) & 1u)) continue;
                 t.world_push[i] = dl[i];

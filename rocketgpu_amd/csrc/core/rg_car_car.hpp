@@ -620,18 +620,25 @@ struct CarPairRow {
     Vec3 rxa, rxb;    // contact offset x direction, for a (with dir) and for b (with -dir)
     Vec3 anga, angb;  // world inverse inertia times those
     float jac, rhs, applied;
-    float friction;   // friction rows
-    int8_t a, b;      // car slots
+    int8_t a, b;        // car slots
     int8_t normal_row;  // friction rows: their non-penetration row
     int8_t pad;
 };
 
-constexpr int kMaxPairRows = 12;
+// A pair's split row: the direction terms and denominator of its point's non-penetration row.
+struct CarPairSplitRow {
+    float rhs;
+    float applied;
+    int normal_row;
+};
+
+constexpr int kMaxPairRows = 8;
+constexpr float kCarCarFriction = 0.09f;
 
 struct CarPairRows {
-    CarPairRow n[kMaxPairRows];  // non-penetration
-    CarPairRow f[kMaxPairRows];  // friction
-    CarPairRow s[kMaxPairRows];  // split impulse
+    CarPairRow n[kMaxPairRows];       // non-penetration
+    CarPairRow f[kMaxPairRows];       // friction
+    CarPairSplitRow s[kMaxPairRows];  // split impulse
     int nn, nf, ns;
     uint32_t cars;  // slots with pair rows
 };
@@ -688,6 +695,7 @@ RG_HD void EmitPairRows(CarState& a, const CarConfig& ca, const CarPreContact& p
         const Vec3 angb = MulRows(ib0, ib1, ib2, rxb);
         const float den_b = inv_m + Dot(na, Cross(-angb, rb));
         const float den = den_a + den_b;
+        int normal_idx = -1;
         if (!(den <= 1.0e-6f)) {
             if (out.nn >= kMaxPairRows) {
                 unsupported |= kUnsupportedCarCar;
@@ -707,11 +715,10 @@ RG_HD void EmitPairRows(CarState& a, const CarConfig& ca, const CarPreContact& p
                 r.jac = jac;
                 r.rhs = (rest - (v1 + v2)) * jac;
                 r.applied = 0.0f;
-                r.friction = 0.0f;
                 r.a = static_cast<int8_t>(slot_a);
                 r.b = static_cast<int8_t>(slot_b);
                 r.normal_row = -1;
-                const int normal_idx = out.nn++;
+                normal_idx = out.nn++;
                 // Friction along the sliding direction at the contact (with the external impulses).
                 const Vec3 bva = base_lin_a + Cross(pa.ang_vel + ext_ang_a, ra);
                 const Vec3 bvb = base_lin_b + Cross(pb.ang_vel + ext_ang_b, rb);
@@ -742,7 +749,6 @@ RG_HD void EmitPairRows(CarState& a, const CarConfig& ca, const CarPreContact& p
                         const float t2 = Dot(tb, base_lin_b) + Dot(rxtb, pb.ang_vel);
                         fr.rhs = -(t1 + t2) * fr.jac;
                         fr.applied = 0.0f;
-                        fr.friction = 0.09f;
                         fr.a = static_cast<int8_t>(slot_a);
                         fr.b = static_cast<int8_t>(slot_b);
                         fr.normal_row = static_cast<int8_t>(normal_idx);
@@ -767,25 +773,19 @@ RG_HD void EmitPairRows(CarState& a, const CarConfig& ca, const CarPreContact& p
                     const float ujac = 1.0f / uden;
                     const float urhs = ((pen_uu * 0.8f) / dt) * ujac;
                     if (urhs > 0.0f && ujac > 0.0f) {
+                        // In Bullet units the row's denominator equals the non-penetration row's (the same terms,
+                        // b's with both signs flipped), so it shares that row's direction terms and inverse.
                         const float pen_bt = Max(0.0f, -Dot(q.pos_b - q.pos_a, npt));
                         const float den_bt = den_a + (inv_m + Dot(npt, Cross(angb, rb)));
                         const float jac_bt = den_bt > 1.0e-8f ? 1.0f / den_bt : 0.0f;
-                        if (!(jac_bt > 0.0f && pen_bt > 0.0f) || out.ns >= kMaxPairRows) {
+                        if (!(jac_bt > 0.0f && pen_bt > 0.0f) || normal_idx < 0 || out.ns >= kMaxPairRows ||
+                            FloatBits(jac_bt) != FloatBits(out.n[normal_idx].jac)) {
                             unsupported |= kUnsupportedCarCar;
                         } else {
-                            CarPairRow& s = out.s[out.ns++];
-                            s.dir = na;
-                            s.rxa = rxa;
-                            s.rxb = rxb;
-                            s.anga = anga;
-                            s.angb = angb;
-                            s.jac = jac_bt;
+                            CarPairSplitRow& s = out.s[out.ns++];
                             s.rhs = ((pen_bt * 0.8f) / dt) * jac_bt;
                             s.applied = 0.0f;
-                            s.friction = 0.0f;
-                            s.a = static_cast<int8_t>(slot_a);
-                            s.b = static_cast<int8_t>(slot_b);
-                            s.normal_row = -1;
+                            s.normal_row = normal_idx;
                         }
                     }
                 }
@@ -921,54 +921,19 @@ RG_HD void CarPairContact(const CarPairSide& sa, const CarPairSide& sb, CarPairM
 // ------------------------------------------------------------------------------------------------------------
 // The joint solve of the cars that touch each other.
 
-constexpr int kMaxGroupWorldRows = 16;
-
-// Mesh rows of the cars in the joint solve, each with its car slot and mesh.
-struct CarGroupWorldRows {
-    CarWorldRow n[kMaxGroupWorldRows];
-    CarWorldFrictionRow f[kMaxGroupWorldRows];
-    CarWorldSplitRow s[kMaxGroupWorldRows];
-    int8_t n_car[kMaxGroupWorldRows], n_mesh[kMaxGroupWorldRows];
-    int8_t s_car[kMaxGroupWorldRows], s_mesh[kMaxGroupWorldRows];
-    int nn, nf, ns;
-};
-
-// Appends one car's mesh rows (friction rows keep pointing at their non-penetration rows).
-RG_HD void AppendGroupWorldRows(const CarWorldRows& r, int slot, CarGroupWorldRows& g, uint32_t& unsupported) {
-    if (g.nn + r.nn > kMaxGroupWorldRows || g.nf + r.nf > kMaxGroupWorldRows || g.ns + r.ns > kMaxGroupWorldRows) {
-        unsupported |= kUnsupportedCarCar;
-        return;
-    }
-    const int base = g.nn;
-    for (int i = 0; i < r.nn; ++i) {
-        g.n[g.nn] = r.n[i];
-        g.n_car[g.nn] = static_cast<int8_t>(slot);
-        g.n_mesh[g.nn] = r.n_mesh[i];
-        ++g.nn;
-    }
-    for (int i = 0; i < r.nf; ++i) {
-        g.f[g.nf] = r.f[i];
-        g.f[g.nf].normal_row = r.f[i].normal_row + base;
-        ++g.nf;
-    }
-    for (int i = 0; i < r.ns; ++i) {
-        g.s[g.ns] = r.s[i];
-        g.s_car[g.ns] = static_cast<int8_t>(slot);
-        g.s_mesh[g.ns] = r.s_mesh[i];
-        ++g.ns;
-    }
-}
-
 // Visit order of the joint rows: the arena's contact-pair list (a car-mesh entry brings that car's rows for that
 // mesh, a car-car entry that pair's rows), then the rows no entry claimed, mesh rows before pair rows.
 struct GroupOrder {
-    uint8_t kind[kMaxGroupWorldRows + kMaxPairRows];  // 0 mesh row, 1 pair row
-    uint8_t row[kMaxGroupWorldRows + kMaxPairRows];
+    uint8_t kind[kMaxCarWorldRows + kMaxPairRows];  // 0 mesh row, 1 pair row
+    uint8_t row[kMaxCarWorldRows + kMaxPairRows];
     int count;
 };
 
-RG_HD void BuildGroupOrder(const ManifoldEntry* entries, int entry_count, int n_world, const int8_t* w_car,
-                           const int8_t* w_mesh, int n_pair, const CarPairRow* pair, GroupOrder& o) {
+// Rows are named through the non-penetration row they belong to: w_of(r) / p_of(r) give the mesh row r's and the
+// pair row r's non-penetration row.
+template <typename WorldOf, typename PairOf>
+RG_HD void BuildGroupOrder(const ManifoldEntry* entries, int entry_count, int n_world, const CarWorldRows& w,
+                           WorldOf w_of, int n_pair, const CarPairRows& p, PairOf p_of, GroupOrder& o) {
     uint32_t wused = 0u, pused = 0u;
     o.count = 0;
     for (int e = 0; e < entry_count; ++e) {
@@ -979,7 +944,7 @@ RG_HD void BuildGroupOrder(const ManifoldEntry* entries, int entry_count, int n_
             const int sid = KeyStatic(key);
             if (sid < 0 || sid >= 100) continue;
             for (int r = 0; r < n_world; ++r) {
-                if (((wused >> r) & 1u) || w_car[r] != car || w_mesh[r] != sid) continue;
+                if (((wused >> r) & 1u) || w.n_car[w_of(r)] != car || w.n_mesh[w_of(r)] != sid) continue;
                 wused |= 1u << r;
                 o.kind[o.count] = 0;
                 o.row[o.count++] = static_cast<uint8_t>(r);
@@ -988,7 +953,8 @@ RG_HD void BuildGroupOrder(const ManifoldEntry* entries, int entry_count, int n_
             const int lo = KeyBodyA(key), hi = KeyBodyB(key);
             for (int r = 0; r < n_pair; ++r) {
                 if ((pused >> r) & 1u) continue;
-                const int plo = MinI(pair[r].a, pair[r].b), phi = MaxI(pair[r].a, pair[r].b);
+                const CarPairRow& pn = p.n[p_of(r)];
+                const int plo = MinI(pn.a, pn.b), phi = MaxI(pn.a, pn.b);
                 if (plo != lo || phi != hi) continue;
                 pused |= 1u << r;
                 o.kind[o.count] = 1;
@@ -1062,9 +1028,9 @@ RG_HD bool SolveWorldVelocityRow(CarWorldRow& row, float inv_mass, Vec3& dl, Vec
     return true;
 }
 
-RG_HD bool SolveWorldFrictionRow(CarWorldFrictionRow& row, float total, float inv_mass, Vec3& dl, Vec3& da) {
+RG_HD bool SolveWorldFrictionRow(CarWorldFrictionRow& row, float max_impulse, float total, float inv_mass, Vec3& dl,
+                                  Vec3& da) {
     if (total <= 0.0f) return false;
-    const float max_impulse = row.friction * total;
     float delta = row.rhs;
     const float dv = Dot(row.tangent, dl) + Dot(row.rxt, da);
     delta = delta - dv * row.jac;
@@ -1087,12 +1053,12 @@ RG_HD bool SolveWorldFrictionRow(CarWorldFrictionRow& row, float total, float in
 
 // Velocity solve of the joint rows: non-penetration rows in visit order, then friction rows in the order of their
 // non-penetration rows (rows without one last, mesh before pair), ten iterations or until one changes nothing.
-// dl / da: the velocity changes per car slot.
-RG_HD void SolveGroupVelocity(const ManifoldEntry* entries, int entry_count, CarGroupWorldRows& g, CarPairRows& p,
-                              float inv_mass, Vec3* dl, Vec3* da) {
+// g holds only the touching cars' mesh rows.  dl / da: the velocity changes per car slot.
+RG_HD void SolveGroupVelocity(const ManifoldEntry* entries, int entry_count, CarWorldRows& g, CarPairRows& p,
+                              float inv_mass, float mesh_friction, Vec3* dl, Vec3* da) {
     GroupOrder on;
-    BuildGroupOrder(entries, entry_count, g.nn, g.n_car, g.n_mesh, p.nn, p.n, on);
-    uint8_t fk[2 * kMaxGroupWorldRows + 2 * kMaxPairRows], fr[2 * kMaxGroupWorldRows + 2 * kMaxPairRows];
+    BuildGroupOrder(entries, entry_count, g.nn, g, [](int r) { return r; }, p.nn, p, [](int r) { return r; }, on);
+    uint8_t fk[kMaxCarWorldRows + kMaxPairRows], fr[kMaxCarWorldRows + kMaxPairRows];
     int fcount = 0;
     uint32_t wfu = 0u, pfu = 0u;
     for (int k = 0; k < on.count; ++k) {
@@ -1138,12 +1104,13 @@ RG_HD void SolveGroupVelocity(const ManifoldEntry* entries, int entry_count, Car
             if (fk[k] == 0) {
                 CarWorldFrictionRow& row = g.f[r];
                 const int c = g.n_car[row.normal_row];
-                changed = SolveWorldFrictionRow(row, g.n[row.normal_row].applied, inv_mass, dl[c], da[c]) || changed;
+                changed = SolveWorldFrictionRow(row, mesh_friction * g.n[row.normal_row].applied,
+                                                g.n[row.normal_row].applied, inv_mass, dl[c], da[c]) || changed;
             } else {
                 CarPairRow& row = p.f[r];
                 const float total = p.n[row.normal_row].applied;
                 if (total <= 0.0f) continue;
-                changed = SolvePairVelocityRow(row, inv_mass, dl, da, row.friction * total, true) || changed;
+                changed = SolvePairVelocityRow(row, inv_mass, dl, da, kCarCarFriction * total, true) || changed;
             }
         }
         if (!changed) break;
@@ -1151,56 +1118,42 @@ RG_HD void SolveGroupVelocity(const ManifoldEntry* entries, int entry_count, Car
 }
 
 // Split-impulse solve of the joint rows in visit order; push / turn: pseudo-velocities per car slot.
-RG_HD void SolveGroupSplit(const ManifoldEntry* entries, int entry_count, CarGroupWorldRows& g, CarPairRows& p,
+RG_HD void SolveGroupSplit(const ManifoldEntry* entries, int entry_count, CarWorldRows& g, CarPairRows& p,
                            float inv_mass, Vec3* push, Vec3* turn) {
     GroupOrder o;
-    BuildGroupOrder(entries, entry_count, g.ns, g.s_car, g.s_mesh, p.ns, p.s, o);
+    BuildGroupOrder(entries, entry_count, g.ns, g, [&](int r) { return g.s[r].normal_row; }, p.ns, p,
+                    [&](int r) { return p.s[r].normal_row; }, o);
     for (int iter = 0; iter < 10; ++iter) {
         bool changed = false;
         for (int k = 0; k < o.count; ++k) {
             const int r = o.row[k];
             if (o.kind[k] == 0) {
-                CarWorldSplitRow& row = g.s[r];
-                if (row.rhs == 0.0f) continue;
-                const int c = g.s_car[r];
-                const float before = row.applied;
-                float delta = row.rhs;
-                delta = delta - (Dot(row.normal, push[c]) + Dot(row.rxn, turn[c])) * row.jac;
-                const float sum = row.applied + delta;
-                if (sum < 0.0f) {
-                    delta = 0.0f - row.applied;
-                    row.applied = 0.0f;
-                } else {
-                    row.applied = sum;
-                }
-                if (delta != 0.0f) {
-                    push[c] += (row.normal * inv_mass) * delta;
-                    turn[c] += row.ang * delta;
-                }
-                changed = changed || delta != 0.0f || FloatBits(before) != FloatBits(row.applied);
-            } else {
-                CarPairRow& row = p.s[r];
-                if (row.rhs == 0.0f) continue;
-                const int a = row.a, b = row.b;
-                const float before = row.applied;
-                float delta = row.rhs;
-                delta -= (Dot(-row.dir, push[b]) + Dot(row.rxb, turn[b])) * row.jac;
-                delta -= (Dot(row.dir, push[a]) + Dot(row.rxa, turn[a])) * row.jac;
-                const float sum = row.applied + delta;
-                if (sum < 0.0f) {
-                    delta = 0.0f - row.applied;
-                    row.applied = 0.0f;
-                } else {
-                    row.applied = sum;
-                }
-                if (delta != 0.0f) {
-                    push[a] += (row.dir * inv_mass) * delta;
-                    turn[a] += row.anga * delta;
-                    push[b] += (-row.dir * inv_mass) * delta;
-                    turn[b] += row.angb * delta;
-                }
-                changed = changed || delta != 0.0f || FloatBits(before) != FloatBits(row.applied);
+                const int c = g.n_car[g.s[r].normal_row];
+                changed = SolveCarWorldSplitRow(g, g.s[r], inv_mass, push[c], turn[c]) || changed;
+                continue;
             }
+            CarPairSplitRow& row = p.s[r];
+            if (row.rhs == 0.0f) continue;
+            const CarPairRow& nr = p.n[row.normal_row];
+            const int a = nr.a, b = nr.b;
+            const float before = row.applied;
+            float delta = row.rhs;
+            delta -= (Dot(-nr.dir, push[b]) + Dot(nr.rxb, turn[b])) * nr.jac;
+            delta -= (Dot(nr.dir, push[a]) + Dot(nr.rxa, turn[a])) * nr.jac;
+            const float sum = row.applied + delta;
+            if (sum < 0.0f) {
+                delta = 0.0f - row.applied;
+                row.applied = 0.0f;
+            } else {
+                row.applied = sum;
+            }
+            if (delta != 0.0f) {
+                push[a] += (nr.dir * inv_mass) * delta;
+                turn[a] += nr.anga * delta;
+                push[b] += (-nr.dir * inv_mass) * delta;
+                turn[b] += nr.angb * delta;
+            }
+            changed = changed || delta != 0.0f || FloatBits(before) != FloatBits(row.applied);
         }
         if (!changed) break;
     }

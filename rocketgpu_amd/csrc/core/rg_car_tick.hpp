@@ -23,6 +23,7 @@
 
 #include "rg_ball_step.hpp"
 #include "rg_car_phases.hpp"
+#include "rg_gjk.hpp"
 
 namespace rg {
 namespace core {
@@ -32,7 +33,7 @@ enum : uint32_t {
     kUnsupportedCarWorld = 1u << 0,        // a car's hitbox may touch a mesh triangle
     kUnsupportedBallCar = 1u << 1,         // the ball's and a car's boxes overlap
     kUnsupportedCarCar = 1u << 2,          // two cars' boxes overlap
-    kUnsupportedWheelOnCar = 1u << 3,      // a wheel ray may reach another car
+    kUnsupportedWheelOnCar = 1u << 3,      // unused: wheel rays are cast against the other cars
     kUnsupportedCarPredictive = 1u << 4,   // a car's swept center crosses a triangle within the tick
     kUnsupportedRespawnSpot = 1u << 5,     // a demolished car picks its respawn spot this tick
     kUnsupportedSetWheelNormals = 1u << 6, // an uploaded car carries set-state wheel normals
@@ -148,7 +149,7 @@ constexpr float kSuspensionStiffness = 500.0f;
 constexpr float kSuspensionCompression = 25.0f;
 constexpr float kSuspensionRelaxation = 40.0f;
 
-enum : uint8_t { kGroundNone = 0, kGroundMesh = 1, kGroundBall = 2 };
+enum : uint8_t { kGroundNone = 0, kGroundMesh = 1, kGroundBall = 2, kGroundCar = 3 };
 
 // One wheel for one tick: its ray, what the ray found, and the friction terms the drive set-up leaves for the
 // friction impulse.  Wheel order: 0 front left-of-offset, 1 front mirrored, 2 back, 3 back mirrored.
@@ -172,6 +173,7 @@ struct WheelRay {
     int32_t static_id;
     uint8_t ground;      // kGround*
     uint8_t front, contact, on_world;
+    int8_t ground_car;   // the car slot under the wheel (kGroundCar)
 };
 
 // Bone height of the suspension, rounded to 0.01 uu (raycast_wheels_and_contact_prep.hpp:458-464).
@@ -235,6 +237,7 @@ RG_HD void InitWheel(WheelRay& w, const CarState& car, const CarConfig& cfg, con
     w.fric_fwd = Vec3(1.0f, 0.0f, 0.0f);
     w.fric_roll = w.fric_lat = w.fric_long = w.fric_scale = 0.0f;
     w.static_id = -1;
+    w.ground_car = -1;
     w.ground = kGroundNone;
     w.contact = 0;
     w.on_world = 0;
@@ -372,6 +375,188 @@ RG_HD void WheelPushback(const CarState& car, float mass, const CarKin& k, Wheel
     w.pushback = impulse / 4.0f;
 }
 
+// The hitbox size Bullet is given: the published size when the config carries it, grown by the game's 0.025 bt
+// collision skin (BulletHitboxSize, math.hpp:1554-1560).
+RG_HD Vec3 BulletHitboxSizeUu(const CarConfig& cfg) {
+    const Vec3 sized = cfg.hitbox_published + Vec3(2.5f, 2.5f, 2.5f);
+    const bool has = sized.x == cfg.hitbox_size.x && sized.y == cfg.hitbox_size.y && sized.z == cfg.hitbox_size.z;
+    const Vec3 published = has ? cfg.hitbox_published : cfg.hitbox_size - Vec3(2.5f, 2.5f, 2.5f);
+    const Vec3 half_bt = (published * 0.02f) * 0.5f + Vec3(0.025f, 0.025f, 0.025f);
+    return (half_bt * 50.0f) * 2.0f;
+}
+
+// Bullet half extents of the hitbox box shape, the collision margin taken off and put back
+// (BulletBoxHalfExtentWithMarginBt, host_validation_and_contact_debug.hpp:43-54).
+RG_HD Vec3 HitboxHalfBt(const CarConfig& cfg) {
+    const Vec3 half = (BulletHitboxSizeUu(cfg) * (1.0f / 50.0f)) / 2.0f;
+    const float safe = 0.1f * Min(half.x, Min(half.y, half.z));
+    const float margin = safe < 0.04f ? safe : 0.04f;
+    return (half - Vec3(margin, margin, margin)) + Vec3(margin, margin, margin);
+}
+
+// Closest points of a point and a box with a collision margin by Bullet's GJK pair detector (Voronoi simplex, the two
+// shapes shifted by their midpoint, the normal turned to point from the box's center toward the point); the result
+// is kept when nearer than `dist` (WheelGjkPointBoxClosest, raycast_wheels_and_contact_prep.hpp:4758-4855).
+RG_HD bool PointBoxClosest(const Vec3& point, const Vec3& origin, const Vec3& f, const Vec3& r, const Vec3& u,
+                           const Vec3& core, float box_margin, bool& has, float& dist, Vec3& normal) {
+    const Vec3 offset = (point + origin) * 0.5f;
+    const Vec3 la = point - offset;
+    const Vec3 lb = origin - offset;
+    const float margin = 0.0f + box_margin;
+    Vec3 axis(0.0f, 1.0f, 0.0f);
+    float sq = 1.0e18f;
+    gjk::Simplex sx;
+    gjk::SimplexReset(sx);
+    bool check = false;
+    int iter = 0;
+    const Vec3 pw = Vec3(0.0f, 0.0f, 0.0f) + la;
+    while (true) {
+        const Vec3 sb(Dot(f, axis), Dot(r, axis), Dot(u, axis));
+        const Vec3 ql(sb.x >= 0.0f ? core.x : -core.x, sb.y >= 0.0f ? core.y : -core.y,
+                      sb.z >= 0.0f ? core.z : -core.z);
+        const Vec3 qw((f.x * ql.x + r.x * ql.y) + u.x * ql.z + lb.x, (f.y * ql.x + r.y * ql.y) + u.y * ql.z + lb.y,
+                      (f.z * ql.x + r.z * ql.y) + u.z * ql.z + lb.z);
+        const Vec3 w = pw - qw;
+        const float delta = Dot(axis, w);
+        if (delta > 0.0f && delta * delta > sq * 1.0e18f) {
+            check = true;
+            break;
+        }
+        if (gjk::SimplexHas(sx, w)) {
+            check = true;
+            break;
+        }
+        if (sq - delta <= sq * 1.0e-6f) {
+            check = true;
+            break;
+        }
+        gjk::SimplexAdd(sx, w, pw, qw);
+        if (!gjk::SimplexUpdate(sx)) {
+            check = true;
+            break;
+        }
+        const Vec3 next = sx.near_v;
+        if (LengthSq(next) < 1.0e-6f) {
+            axis = next;
+            check = true;
+            break;
+        }
+        const float prev = sq;
+        sq = LengthSq(next);
+        if (prev - sq <= 1.1920929e-7f * prev) {
+            check = true;
+            break;
+        }
+        axis = next;
+        if (iter++ > 1000) break;
+        if (sx.n == 4) break;
+    }
+    if (!check) return has;
+    gjk::SimplexUpdate(sx);
+    const float len_sq = LengthSq(axis);
+    if (!(len_sq > 1.1920929e-7f * 1.1920929e-7f)) return has;
+    const float rlen = 1.0f / sqrtf(len_sq);
+    Vec3 n = axis * rlen;
+    const float d = (1.0f / rlen) - margin;
+    if (!(d < 0.0f || d * d < 1.0e18f)) return has;
+    const Vec3 hm = core + Vec3(box_margin, box_margin, box_margin);
+    const Vec3 ext((hm.x * Abs(f.x) + hm.y * Abs(r.x)) + hm.z * Abs(u.x),
+                   (hm.x * Abs(f.y) + hm.y * Abs(r.y)) + hm.z * Abs(u.y),
+                   (hm.x * Abs(f.z) + hm.y * Abs(r.z)) + hm.z * Abs(u.z));
+    const Vec3 zero(0.0f, 0.0f, 0.0f);
+    const Vec3 ca = ((la + zero) + (la - zero)) * 0.5f;
+    const Vec3 cb = ((lb - ext) + (lb + ext)) * 0.5f;
+    if (Dot(ca - cb, n) < 0.0f) n = n * -1.0f;
+    if (d < dist) {
+        has = true;
+        normal = n;
+        dist = d;
+    }
+    return has;
+}
+
+// A wheel's ray against another car's hitbox (CastWheelRayAgainstCarHitboxGjkCast,
+// raycast_wheels_and_contact_prep.hpp:4857-4979): a sphere and slab rejection, then Bullet's GJK convex cast of the
+// ray's point against the box (advance by distance over approach speed until within 0.001 bt).  On a hit: the
+// distance along the ray in uu, the point (bt) and the box's normal there.
+RG_HD bool WheelRayCarHit(const Vec3& src, const Vec3& dir, float len_uu, float len_bt, const CarState& other,
+                          const CarConfig& ocfg, float& t_uu, Vec3& point, Vec3& normal) {
+    const Vec3 to = src + dir * len_bt;
+    const Vec3 half = BulletHitboxSizeUu(ocfg) * (0.5f * (1.0f / 50.0f));
+    const float margin = Min(0.1f * Min(half.x, Min(half.y, half.z)), 0.04f);
+    const Vec3 inner = half - Vec3(margin, margin, margin);
+    Vec3 f, r, u;
+    CarBasis(other, f, r, u);
+    const Vec3 off = ocfg.hitbox_offset * (1.0f / 50.0f);
+    const Vec3 center = other.pos_bt + ((f * off.x + r * off.y) + u * off.z);
+    {
+        const float rr = Length(half) + len_bt + 1.0e-3f;
+        if (LengthSq(center - src) > rr * rr) return false;
+    }
+    {
+        const Vec3 rf = src - other.pos_bt, rt = to - other.pos_bt;
+        const Vec3 lf(Dot(rf, f), Dot(rf, r), Dot(rf, u));
+        const Vec3 lt(Dot(rt, f), Dot(rt, r), Dot(rt, u));
+        const Vec3 ext = inner + Vec3(margin, margin, margin);
+        const Vec3 bmin = off - ext, bmax = off + ext;
+        if ((lf.x < bmin.x && lt.x < bmin.x) || (lf.x > bmax.x && lt.x > bmax.x) || (lf.y < bmin.y && lt.y < bmin.y) ||
+            (lf.y > bmax.y && lt.y > bmax.y) || (lf.z < bmin.z && lt.z < bmin.z) || (lf.z > bmax.z && lt.z > bmax.z)) {
+            return false;
+        }
+        Vec3 ld = lt - lf;
+        const float ll = Length(ld);
+        if (!(ll > 0.0f)) return false;
+        ld = ld * (1.0f / ll);
+        const float lambda_max = Dot(ld, lt - lf);
+        const float inv[3] = {ld.x == 0.0f ? 1.0e30f : 1.0f / ld.x, ld.y == 0.0f ? 1.0e30f : 1.0f / ld.y,
+                              ld.z == 0.0f ? 1.0e30f : 1.0f / ld.z};
+        const float from[3] = {lf.x, lf.y, lf.z};
+        const float lo[3] = {bmin.x, bmin.y, bmin.z};
+        const float hi[3] = {bmax.x, bmax.y, bmax.z};
+        float tmin = ((inv[0] < 0.0f ? hi[0] : lo[0]) - from[0]) * inv[0];
+        float tmax = ((inv[0] < 0.0f ? lo[0] : hi[0]) - from[0]) * inv[0];
+        for (int k = 1; k < 3; ++k) {
+            const float t0 = ((inv[k] < 0.0f ? hi[k] : lo[k]) - from[k]) * inv[k];
+            const float t1 = ((inv[k] < 0.0f ? lo[k] : hi[k]) - from[k]) * inv[k];
+            if (tmin > t1 || t0 > tmax) return false;
+            if (t0 > tmin) tmin = t0;
+            if (t1 < tmax) tmax = t1;
+        }
+        if (!(tmin < lambda_max && tmax > 0.0f)) return false;
+    }
+    const Vec3 ray(to.x - src.x - (center.x - center.x), to.y - src.y - (center.y - center.y),
+                   to.z - src.z - (center.z - center.z));
+    bool has = false;
+    float dist = 1.0e18f;
+    Vec3 n(0.0f, 0.0f, 0.0f);
+    if (!PointBoxClosest(src, center, f, r, u, inner, margin, has, dist, n)) return false;
+    float lambda = 0.0f;
+    int iter = 0;
+    while (true) {
+        if (dist <= 0.001f) {
+            if (-0.0f <= Dot(ray, n)) return false;
+            break;
+        }
+        if (++iter > 64) return false;
+        const float next = lambda - dist / Dot(ray, n);
+        if (next > 1.0f || next < 0.0f || next <= lambda) return false;
+        const float s = 1.0f - next;
+        const Vec3 pa(next * to.x + s * src.x, next * to.y + s * src.y, next * to.z + s * src.z);
+        const Vec3 bo(next * center.x + s * center.x, next * center.y + s * center.y, next * center.z + s * center.z);
+        if (!PointBoxClosest(pa, bo, f, r, u, inner, margin, has, dist, n)) return false;
+        lambda = next;
+        if (!(0.0f <= dist)) break;
+    }
+    const float n2 = LengthSq(n);
+    if (!(n2 > 1.0e-4f) || !(lambda < 1.0f)) return false;
+    const float inv_len = 1.0f / sqrtf(n2);
+    const float s = 1.0f - lambda;
+    t_uu = lambda * len_uu;
+    normal = Vec3(n.x * inv_len, n.y * inv_len, n.z * inv_len);
+    point = Vec3(s * src.x + lambda * to.x, s * src.y + lambda * to.y, s * src.z + lambda * to.z);
+    return true;
+}
+
 // The wheel's ray against the arena, the other cars and the ball; nearest wins (RayCastWheelAgainstArena, :641-844,
 // for a custom arena: meshes only, no analytic planes).  `others_near`: the other bodies whose last swept boxes the
 // car's rays reach (bit 0 ball, bit 1 + j car j); a car there cannot be cast here yet, which the arena reports.
@@ -387,14 +572,38 @@ RG_HD void CastWheel(WheelRay& w, const CarState& car, const CarState* cars, con
     float best_t = mh.hit ? mh.fraction * len_uu : 0.0f;
     Vec3 p(0.0f, 0.0f, 0.0f), n = mh.normal;
     if (mh.hit) p = (1.0f - mh.fraction) * w.from + mh.fraction * (w.from + delta);
-    // Other cars: any within reach of the ray would need the ray-vs-hitbox cast (CastWheelRayAgainstCars, :329-418).
-    for (int j = 0; j < count; ++j) {
-        if (j == self || ((others_near >> (j + 1)) & 1u) == 0u || cars[j].is_demoed) continue;
-        const CarConfig& oc = cfgs[j];
-        const float reach = 0.5f * (Abs(oc.hitbox_size.x) + Abs(oc.hitbox_size.y) + Abs(oc.hitbox_size.z)) +
-                            Abs(oc.hitbox_offset.x) + Abs(oc.hitbox_offset.y) + Abs(oc.hitbox_offset.z);
-        const float rr = reach * (1.0f / 50.0f) + len_bt + 1.0e-2f;
-        if (!(LengthSq(cars[j].pos_bt - w.from) > rr * rr)) unsupported |= kUnsupportedWheelOnCar;
+    // Other cars within reach of the ray: the nearest hitbox hit (CastWheelRayAgainstCars, :329-418), taken when
+    // nearer than the mesh's.  A car that gives no contact response (demolished this tick) voids the wheel's contact.
+    int car_hit = -1;
+    bool no_response = false;
+    {
+        float car_t = 0.0f;
+        Vec3 car_p(0.0f, 0.0f, 0.0f), car_n(0.0f, 0.0f, 0.0f);
+        for (int j = 0; j < count; ++j) {
+            if (j == self || ((others_near >> (j + 1)) & 1u) == 0u || cars[j].is_demoed) continue;
+            const CarConfig& oc = cfgs[j];
+            const float reach = 0.5f * (Abs(oc.hitbox_size.x) + Abs(oc.hitbox_size.y) + Abs(oc.hitbox_size.z)) +
+                                Abs(oc.hitbox_offset.x) + Abs(oc.hitbox_offset.y) + Abs(oc.hitbox_offset.z);
+            const float rr = reach * (1.0f / 50.0f) + len_bt + 1.0e-2f;
+            if (LengthSq(cars[j].pos_bt - w.from) > rr * rr) continue;
+            float t = 0.0f;
+            Vec3 hp, hn;
+            if (!WheelRayCarHit(w.from, w.down, len_uu, len_bt, cars[j], oc, t, hp, hn)) continue;
+            if (min_hit_sq > 0.0f && LengthSq(hp - w.from) < min_hit_sq) continue;
+            if (car_hit < 0 || t < car_t) {
+                car_hit = j;
+                car_t = t;
+                car_p = hp;
+                car_n = hn;
+                no_response = cars[j].demo_no_contact_response != 0;
+            }
+        }
+        if (car_hit >= 0 && (kind == kGroundNone || car_t < best_t)) {
+            kind = kGroundCar;
+            best_t = car_t;
+            p = car_p;
+            n = car_n;
+        }
     }
     if (others_near & 1u) {
         Vec3 bn, bp;
@@ -410,12 +619,64 @@ RG_HD void CastWheel(WheelRay& w, const CarState& car, const CarState* cars, con
             }
         }
     }
-    if (kind == kGroundNone) {
+    if (kind == kGroundNone || (kind == kGroundCar && no_response)) {
         WheelMiss(w);
         return;
     }
     const int32_t mesh_id = kind == kGroundMesh ? static_cast<int32_t>((mesh.tris[mh.tri].flags >> 8) & 0xffu) : -1;
     WheelHit(car, w, p, n, static_cast<uint8_t>(kind), kind == kGroundMesh, mesh_id);
+    w.ground_car = static_cast<int8_t>(kind == kGroundCar ? car_hit : -1);
+}
+
+// A wheel resting on another car pushes it (WheelRayPhase, raycast_wheels_and_contact_prep.hpp:2197-2219): when
+// sunk below its threshold the wheel's pushback is a quarter of the two-body contact impulse that a Bullet solver
+// step would give (ResolveCarCarWheelPushbackMagnitudeBtInputs, host_validation_and_contact_debug.hpp:461-484),
+// applied in full to the other car at the hit, and every such wheel also gives the other car a unit impulse along
+// -normal there.
+RG_HD void WheelPushCar(const CarState& car, const CarKin& k, float mass, WheelRay& w, CarState& other,
+                        const CarConfig& ocfg) {
+    constexpr float kSolverDt = 1.0f / 60.0f;
+    const Vec3 n = w.normal, cp = w.hit_point;
+    const float thresh = (w.rest * (1.0f / 50.0f) + w.radius * (1.0f / 50.0f)) - 0.05f;
+    Vec3 o0, o1, o2;
+    if (other.restored_inv_inertia_valid) {
+        o0 = other.restored_inv_inertia_bt[0];
+        o1 = other.restored_inv_inertia_bt[1];
+        o2 = other.restored_inv_inertia_bt[2];
+    } else {
+        InvInertiaWorldRows(other.tick_start_basis_forward, other.tick_start_basis_right, other.tick_start_basis_up,
+                            InvInertiaLocalBt(ocfg, mass), o0, o1, o2);
+    }
+    w.pushback = 0.0f;
+    if (w.trace_len < thresh) {
+        const float depth = w.trace_len - thresh;
+        const Vec3 v1 = car.vel_bt + Cross(car.ang_vel, cp - car.pos_bt);
+        const Vec3 v2 = other.vel_bt + Cross(other.ang_vel, cp - other.pos_bt);
+        const float rel_vel = Dot(n, v1 - v2);
+        const auto denom = [&](const Vec3& pos, const Vec3& r0, const Vec3& r1, const Vec3& r2) {
+            const Vec3 rel = cp - pos;
+            const Vec3 c0 = Cross(rel, n);
+            const Vec3 ang(r0.x * c0.x + r1.x * c0.y + r2.x * c0.z, r0.y * c0.x + r1.y * c0.y + r2.y * c0.z,
+                           r0.z * c0.x + r1.z * c0.y + r2.z * c0.z);
+            return 1.0f / mass + Dot(n, Cross(ang, rel));
+        };
+        const float d0 = denom(car.pos_bt, k.inv_row0, k.inv_row1, k.inv_row2);
+        const float d1 = denom(other.pos_bt, o0, o1, o2);
+        float impulse = 0.0f;
+        if (!(d0 + d1 <= 0.0f)) {
+            const float jac = 1.0f / (d0 + d1);
+            const float sum = (0.2f * -depth / kSolverDt) * jac + -rel_vel * jac;
+            impulse = sum > 0.0f ? sum : 0.0f;
+        }
+        w.pushback = impulse / 4.0f;
+    }
+    const auto apply = [&](const Vec3& impulse) {
+        SetCarVelBt(other, other.vel_bt + impulse * (1.0f / mass));
+        other.vel_touched_solve_stage = 1;
+        other.ang_vel += MulRows(o0, o1, o2, Cross(cp - other.pos_bt, impulse));
+    };
+    if (w.pushback > 0.0f) apply(-n * (w.pushback * 4.0f));
+    apply(-n);
 }
 
 // The right column of a wheel's frame turned by the steer angle about the car's up axis
@@ -659,7 +920,7 @@ struct VehicleWorld {
     const ArenaMeshView* mesh;
     BallState* ball;
     float ball_radius;
-    const CarState* cars;     // the arena's cars
+    CarState* cars;           // the arena's cars (a wheel on another car pushes it)
     const CarConfig* cfgs;
     int count;
     int self;
@@ -719,6 +980,9 @@ RG_HD void CarVehicleUpdate(CarState& car, const CarControls& raw, const CarConf
                   world.ball_radius, world.others_near, min_hit_sq, unsupported);
         if (w[i].contact) w[i].pushback = 0.0f;
         if (w[i].on_world) WheelPushback(car, m.car_mass, k, w[i]);
+        if (w[i].contact && !w[i].on_world && w[i].ground == kGroundCar) {
+            WheelPushCar(car, k, m.car_mass, w[i], world.cars[w[i].ground_car], world.cfgs[w[i].ground_car]);
+        }
         if (w[i].contact && !w[i].on_world && w[i].ground == kGroundBall) {
             // A wheel on the ball pushes it with a unit impulse along -normal.
             BallState& b = *world.ball;
@@ -997,25 +1261,6 @@ RG_HD void CarFinish(CarState& car, const CarPreContact& pc, float dt) {
 // Broadphase proxy of a car (ComputeBulletBodyProxy, car branch, arena_geometry.hpp:3910-4033): the hitbox's box now
 // (insertion box) and the union of the compound's box now and after one predicted step (swept box), each grown by
 // the contact threshold.
-
-// The hitbox size Bullet is given: the published size when the config carries it, grown by the game's 0.025 bt
-// collision skin (BulletHitboxSize, math.hpp:1554-1560).
-RG_HD Vec3 BulletHitboxSizeUu(const CarConfig& cfg) {
-    const Vec3 sized = cfg.hitbox_published + Vec3(2.5f, 2.5f, 2.5f);
-    const bool has = sized.x == cfg.hitbox_size.x && sized.y == cfg.hitbox_size.y && sized.z == cfg.hitbox_size.z;
-    const Vec3 published = has ? cfg.hitbox_published : cfg.hitbox_size - Vec3(2.5f, 2.5f, 2.5f);
-    const Vec3 half_bt = (published * 0.02f) * 0.5f + Vec3(0.025f, 0.025f, 0.025f);
-    return (half_bt * 50.0f) * 2.0f;
-}
-
-// Bullet half extents of the hitbox box shape, the collision margin taken off and put back
-// (BulletBoxHalfExtentWithMarginBt, host_validation_and_contact_debug.hpp:43-54).
-RG_HD Vec3 HitboxHalfBt(const CarConfig& cfg) {
-    const Vec3 half = (BulletHitboxSizeUu(cfg) * (1.0f / 50.0f)) / 2.0f;
-    const float safe = 0.1f * Min(half.x, Min(half.y, half.z));
-    const float margin = safe < 0.04f ? safe : 0.04f;
-    return (half - Vec3(margin, margin, margin)) + Vec3(margin, margin, margin);
-}
 
 struct CarProxyPose {
     Vec3 axis[3];   // hitbox axes now

@@ -23,7 +23,8 @@ namespace rg {
 namespace core {
 
 constexpr int kCarWorldRowsPerMesh = 4;
-constexpr int kMaxCarWorldRows = 16;
+// Rows held at once: one car's (two manifolds of four points), or those of the cars that touch each other.
+constexpr int kMaxCarWorldRows = 8;
 
 // Contact-breaking threshold of a car's hitbox compound: its bounding sphere (box center offset plus half diagonal)
 // times Bullet's 0.02 factor (CarContactBreakingThresholdBt, arena_geometry.hpp:6787-6798).
@@ -312,27 +313,27 @@ struct CarWorldFrictionRow {
     float jac;
     float rhs;
     float applied;
-    float friction;
     int normal_row;
 };
 
+// A split row shares its point's non-penetration row's direction terms and denominator.
 struct CarWorldSplitRow {
-    Vec3 normal;
-    Vec3 rxn;
-    Vec3 ang;
-    float jac;
     float rhs;
     float applied;
+    int normal_row;
 };
 
+// Rows of cars against the meshes.  A car without car-car rows has its rows solved and dropped before the next car's
+// are made; the rows of cars with car-car rows stay for the joint solve.
 struct CarWorldRows {
     CarWorldRow n[kMaxCarWorldRows];
     CarWorldFrictionRow f[kMaxCarWorldRows];
     CarWorldSplitRow s[kMaxCarWorldRows];
-    int8_t n_mesh[kMaxCarWorldRows], s_mesh[kMaxCarWorldRows];  // the mesh of each non-penetration / split row
+    int8_t n_car[kMaxCarWorldRows], n_mesh[kMaxCarWorldRows];  // the car slot and mesh of each non-penetration row
     int nn, nf, ns;
-    uint8_t hit;         // a manifold held a point this tick
-    Vec3 latest_normal;  // the normal recorded as the car's world contact
+    uint8_t full;        // a point found no room for its rows
+    uint8_t hit;         // a manifold of the last car held a point this tick
+    Vec3 latest_normal;  // the normal recorded as the last car's world contact
 };
 
 // Rows for a car's manifold points (emit_candidates): non-penetration with the mesh restitution (no velocity
@@ -366,14 +367,17 @@ RG_HD void EmitRows(CarState& car, const CarConfig& cfg, const CarParams& m, con
         }
         out.hit = 1;
         out.latest_normal = p.normal;
+        ++emitted;
+        if (out.nn >= kMaxCarWorldRows) {
+            out.full = 1;
+            continue;
+        }
         const Vec3 ang = MulRows(i0, i1, i2, Cross(p.rel_bt, p.normal));
-        if (pen_uu > 0.0f && dt > 0.0f && out.ns < kMaxCarWorldRows) {
+        const float jac = 1.0f / (inv_mass + Dot(p.normal, Cross(ang, p.rel_bt)));
+        if (pen_uu > 0.0f && dt > 0.0f) {
             CarWorldSplitRow& s = out.s[out.ns++];
-            s.normal = p.normal;
-            s.rxn = Cross(p.rel_bt, p.normal);
-            s.ang = ang;
-            s.jac = 1.0f / (inv_mass + Dot(p.normal, Cross(ang, p.rel_bt)));
-            s.rhs = ((-p.distance * 0.8f) / dt) * s.jac;
+            s.normal_row = out.nn;
+            s.rhs = ((-p.distance * 0.8f) / dt) * jac;
             s.applied = 0.0f;
         }
         // Sliding direction at the contact with the external impulses applied.
@@ -394,19 +398,19 @@ RG_HD void EmitRows(CarState& car, const CarConfig& cfg, const CarParams& m, con
             tangent = Vec3(-p.normal.y * k, p.normal.x * k, 0.0f);
         }
         const Vec3 tang_ang = MulRows(i0, i1, i2, Cross(p.rel_bt, tangent));
-        if (out.nn < kMaxCarWorldRows) {
+        {
             CarWorldRow& r = out.n[out.nn];
             r.normal = p.normal;
             r.rxn = Cross(p.rel_bt, p.normal);
             r.ang = ang;
-            r.jac = 1.0f / (inv_mass + Dot(p.normal, Cross(ang, p.rel_bt)));
+            r.jac = jac;
             const float rest_vn = Dot(p.normal, car.vel_bt + Cross(car.ang_vel, p.rel_bt));
             const float rhs_vn = Dot(p.normal, vel_ext) + Dot(r.rxn, ang_ext);
             const float restitution = Max(m.arena_restitution * -rest_vn, 0.0f);
             r.rhs = (restitution - rhs_vn) * r.jac;
             r.applied = 0.0f;
             const int normal_idx = out.nn++;
-            if (m.arena_friction > 0.0f && out.nf < kMaxCarWorldRows) {
+            if (m.arena_friction > 0.0f) {
                 CarWorldFrictionRow& fr = out.f[out.nf++];
                 fr.tangent = tangent;
                 fr.rxt = Cross(p.rel_bt, tangent);
@@ -415,11 +419,9 @@ RG_HD void EmitRows(CarState& car, const CarConfig& cfg, const CarParams& m, con
                 const float v1 = Dot(tangent, vel_ext) + Dot(fr.rxt, car.ang_vel);
                 fr.rhs = (0.0f - (v1 + 0.0f)) * fr.jac;
                 fr.applied = 0.0f;
-                fr.friction = m.arena_friction;
                 fr.normal_row = normal_idx;
             }
         }
-        ++emitted;
     }
 }
 
@@ -427,9 +429,9 @@ RG_HD void EmitRows(CarState& car, const CarConfig& cfg, const CarParams& m, con
 // points.  Returns rows for the solver; updates the retained manifolds.
 RG_HD void CarWorldContacts(CarState& car, const CarConfig& cfg, const CarParams& m, const ArenaMeshView& mesh,
                             uint32_t mesh_bits, CarWorldManifold* retained, float dt, CarWorldRows& out) {
-    out.nn = out.nf = out.ns = 0;
     out.hit = 0;
     out.latest_normal = Vec3(0.0f, 0.0f, 1.0f);
+    const int car_n0 = out.nn;
     uint32_t visit = mesh_bits;
     for (int k = 0; k < kCarWorldManifoldsPerCar; ++k) {
         if (retained[k].count > 0 && retained[k].mesh < 32) visit |= 1u << retained[k].mesh;
@@ -550,17 +552,16 @@ RG_HD void CarWorldContacts(CarState& car, const CarConfig& cfg, const CarParams
             }
             n = PointsFromManifold(rm, f.body, pts);
         }
-        const int nn0 = out.nn, ns0 = out.ns;
+        const int nn0 = out.nn;
         EmitRows(car, cfg, m, pts, n, dt, out);
         for (int r = nn0; r < out.nn; ++r) out.n_mesh[r] = static_cast<int8_t>(mi);
-        for (int r = ns0; r < out.ns; ++r) out.s_mesh[r] = static_cast<int8_t>(mi);
     }
     if (out.hit) {
         out.latest_normal = last_normal;
         // The recorded normal: the normalized sum of the rows' normals (one mesh: a single ordering key).
         Vec3 sum(0.0f, 0.0f, 0.0f);
-        for (int r = 0; r < out.nn; ++r) sum += out.n[r].normal;
-        if (out.nn > 0) {
+        for (int r = car_n0; r < out.nn; ++r) sum += out.n[r].normal;
+        if (out.nn > car_n0) {
             const float len_sq = Dot(sum, sum);
             if (len_sq >= 1e-8f) {
                 const Vec3 nn = sum * (1.0f / sqrtf(len_sq));
@@ -570,14 +571,16 @@ RG_HD void CarWorldContacts(CarState& car, const CarConfig& cfg, const CarParams
     }
 }
 
-// Velocity solve of one car's rows: normal rows, then friction rows (each bounded by its normal row's impulse), in
-// order, until an iteration changes nothing.  Returns the accumulated linear and angular velocity changes.
-RG_HD void SolveCarWorldVelocity(CarWorldRows& r, float inv_mass, Vec3& d_lin, Vec3& d_ang) {
+// Velocity solve of one car's rows (non-penetration rows from n0, friction rows from f0): non-penetration rows, then
+// friction rows (each bounded by `friction` times its non-penetration row's impulse), in order, until an iteration
+// changes nothing.  Returns the accumulated linear and angular velocity changes.
+RG_HD void SolveCarWorldVelocity(CarWorldRows& r, int n0, int f0, float inv_mass, float friction, Vec3& d_lin,
+                                 Vec3& d_ang) {
     d_lin = Vec3(0.0f, 0.0f, 0.0f);
     d_ang = Vec3(0.0f, 0.0f, 0.0f);
     for (int iter = 0; iter < 10; ++iter) {
         bool changed = false;
-        for (int i = 0; i < r.nn; ++i) {
+        for (int i = n0; i < r.nn; ++i) {
             CarWorldRow& row = r.n[i];
             const float dv = Dot(row.normal, d_lin) + Dot(row.rxn, d_ang);
             float delta = row.rhs;
@@ -599,11 +602,11 @@ RG_HD void SolveCarWorldVelocity(CarWorldRows& r, float inv_mass, Vec3& d_lin, V
             d_ang += row.ang * delta;
             changed = true;
         }
-        for (int i = 0; i < r.nf; ++i) {
+        for (int i = f0; i < r.nf; ++i) {
             CarWorldFrictionRow& row = r.f[i];
             const float total = r.n[row.normal_row].applied;
             if (total <= 0.0f) continue;
-            const float max_impulse = row.friction * total;
+            const float max_impulse = friction * total;
             float delta = row.rhs;
             const float dv = Dot(row.tangent, d_lin) + Dot(row.rxt, d_ang);
             delta = delta - dv * row.jac;
@@ -630,34 +633,37 @@ RG_HD void SolveCarWorldVelocity(CarWorldRows& r, float inv_mass, Vec3& d_lin, V
     }
 }
 
-// Split-impulse solve of one car's penetrating rows: pseudo-velocities that push the car out of the mesh without
-// adding momentum.
-RG_HD void SolveCarWorldSplit(CarWorldRows& r, float inv_mass, Vec3& push, Vec3& turn) {
+// One split row of a car against a mesh: returns whether it changed anything.
+RG_HD bool SolveCarWorldSplitRow(const CarWorldRows& r, CarWorldSplitRow& row, float inv_mass, Vec3& push, Vec3& turn) {
+    if (row.rhs == 0.0f) return false;
+    const CarWorldRow& nr = r.n[row.normal_row];
+    const float before = row.applied;
+    float delta = row.rhs;
+    const float dv_lin = Dot(nr.normal, push);
+    const float dv_ang = Dot(nr.rxn, turn);
+    delta = delta - (dv_lin + dv_ang) * nr.jac;
+    const float sum = row.applied + delta;
+    if (sum < 0.0f) {
+        delta = 0.0f - row.applied;
+        row.applied = 0.0f;
+    } else {
+        row.applied = sum;
+    }
+    if (delta != 0.0f) {
+        push += (nr.normal * inv_mass) * delta;
+        turn += nr.ang * delta;
+    }
+    return delta != 0.0f || FloatBits(before) != FloatBits(row.applied);
+}
+
+// Split-impulse solve of one car's penetrating rows (from s0): pseudo-velocities that push the car out of the mesh
+// without adding momentum.
+RG_HD void SolveCarWorldSplit(CarWorldRows& r, int s0, float inv_mass, Vec3& push, Vec3& turn) {
     push = Vec3(0.0f, 0.0f, 0.0f);
     turn = Vec3(0.0f, 0.0f, 0.0f);
     for (int iter = 0; iter < 10; ++iter) {
         bool changed = false;
-        for (int i = 0; i < r.ns; ++i) {
-            CarWorldSplitRow& row = r.s[i];
-            if (row.rhs == 0.0f) continue;
-            const float before = row.applied;
-            float delta = row.rhs;
-            const float dv_lin = Dot(row.normal, push);
-            const float dv_ang = Dot(row.rxn, turn);
-            delta = delta - (dv_lin + dv_ang) * row.jac;
-            const float sum = row.applied + delta;
-            if (sum < 0.0f) {
-                delta = 0.0f - row.applied;
-                row.applied = 0.0f;
-            } else {
-                row.applied = sum;
-            }
-            if (delta != 0.0f) {
-                push += (row.normal * inv_mass) * delta;
-                turn += row.ang * delta;
-            }
-            changed = changed || delta != 0.0f || FloatBits(before) != FloatBits(row.applied);
-        }
+        for (int i = s0; i < r.ns; ++i) changed = SolveCarWorldSplitRow(r, r.s[i], inv_mass, push, turn) || changed;
         if (!changed) break;
     }
 }

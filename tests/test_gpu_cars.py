@@ -71,7 +71,7 @@ def _setup(n_env, cars, seed, presets=None):
 
 
 @pytest.mark.parametrize("n_env,cars,ticks,seed,mixed,min_frac",
-                         [(96, 2, 600, 2024, False, 0.95), (48, 6, 240, 77, True, 0.75)])
+                         [(96, 2, 600, 2024, False, 0.95), (48, 6, 240, 77, True, 0.9)])
 def test_gpu_cars_match_twin_and_reference(n_env, cars, ticks, seed, mixed, min_frac):
     torch = _torch()
     import rocketgpu_amd as rga
@@ -133,11 +133,11 @@ def test_gpu_cars_multi_tick_launch_matches_reference():
 
 
 def test_gpu_cars_teammates_collide_match_twin_and_reference():
-    """Teammates steered into each other (hitbox-hitbox contacts, bumps, the joint solve of touching cars): the GPU
-    equals the host twin on every tick and the reference while unflagged."""
+    """Teammates steered into each other (hitbox-hitbox contacts, bumps, wheels on other cars, the joint solve of
+    touching cars): the GPU equals the host twin on every tick and the reference while unflagged."""
     torch = _torch()
 
-    n_env, cars, ticks, seed = 16, 4, 1000, 5
+    n_env, cars, ticks, seed = 16, 6, 1000, 9
     say(f"case: teammate chase, {n_env} arenas x {cars} cars, {ticks} ticks, seed {seed}")
     R, ref, sim, T, h = _setup(n_env, cars, seed, np.arange(n_env * cars) % 6)
     cap = n_env * cars
@@ -164,7 +164,8 @@ def test_gpu_cars_teammates_collide_match_twin_and_reference():
 
     bumps = gpu["cars"].view(views.state_dtype("car_states"))["bump_event_bumps"].reshape(n_env, -1).sum(axis=1)
     say(f"  bumps in arenas never flagged: {int(bumps[~ever_flagged].sum())}")
-    assert bumps[~ever_flagged].sum() >= 20
+    assert bumps[~ever_flagged].sum() >= 100
+    assert (~ever_flagged).sum() >= n_env // 2
     T.rgt_cars_destroy(h)
     sim.close()
     ref.close()

@@ -71,15 +71,15 @@ def test_twin_cars_1v1_octane_match_reference():
 def test_twin_cars_mixed_bodies_match_reference(cars):
     n_env, ticks = 6, 200
     compared, _, _ = _run(n_env, cars, ticks, seed=11 + cars, presets=np.arange(n_env * cars) % 6)
-    assert compared >= 0.7 * n_env * ticks, f"only {compared} of {n_env * ticks} arena-ticks compared"
+    assert compared >= 0.9 * n_env * ticks, f"only {compared} of {n_env * ticks} arena-ticks compared"
 
 
 def test_twin_cars_teammates_collide_match_reference():
-    """Teammates steered into each other: hitbox-hitbox contacts, the pair manifolds, bumps and the joint solve of
-    the touching cars, compared with the reference on every tick while unflagged."""
+    """Teammates steered into each other: hitbox-hitbox contacts, the pair manifolds, bumps, wheels resting on other
+    cars and the joint solve of the touching cars, compared with the reference on every tick while unflagged."""
     R = oracle_or_skip()
     T = twin_cars()
-    n_env, cars, ticks, seed = 16, 4, 1000, 5
+    n_env, cars, ticks, seed = 16, 6, 1000, 9
     presets = np.arange(n_env * cars) % 6
     ref, counts, teams, cfgs = reference_at_kickoff(R, n_env, cars, seed, presets)
     h = T.rgt_cars_create(n_env, cars, 1.0, 256.0, seed)
@@ -106,5 +106,5 @@ def test_twin_cars_teammates_collide_match_reference():
     T.rgt_cars_destroy(h)
     ref.close()
     # Arenas that were never flagged went through many bumps (each bump is one car-car contact's outcome).
-    assert bumps[~ever_flagged].sum() >= 20, f"bumps in unflagged arenas: {bumps[~ever_flagged].sum()}"
-    assert compared >= 0.4 * n_env * ticks
+    assert bumps[~ever_flagged].sum() >= 100, f"bumps in unflagged arenas: {bumps[~ever_flagged].sum()}"
+    assert compared >= 0.9 * n_env * ticks
