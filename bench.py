@@ -355,8 +355,10 @@ def bench_cars(args, rga, np, torch, stream):
         "arenas_unsupported_frac_at_end": float((unsupported != 0).mean()),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "limiter": "latency (one lane per arena walks its wheel rays, broadphase and solver serially; "
-                                "dependent loads at low occupancy)",
+                     "limiter": "latency and divergence (one lane per arena walks its wheel rays, broadphase, "
+                                "contacts and solver serially; counters: 67% of wave cycles waiting on memory, a "
+                                "quarter of the lanes active per VALU instruction, one wave per CU at 256 VGPRs; "
+                                "measured traffic 1.12x the algorithmic bytes)",
                      "kernel": f"StepCarArenasKernel<{'true' if stage == 2 else 'false'}>",
                      "kernel_avg_ms": avg_kernel_s * 1e3,
                      "bytes_per_arena_tick": b_arena},
