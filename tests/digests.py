@@ -41,3 +41,39 @@ def ball_env_digests(balls: np.ndarray) -> np.ndarray:
     cols = np.concatenate([b[:, BALL_POS:BALL_POS + 12], b[:, BALL_VEL:BALL_VEL + 12],
                            b[:, BALL_ANG_VEL:BALL_ANG_VEL + 12], rot], axis=1)
     return fnv_rows(cols)
+
+
+# CarState field offsets (bytes) of the fields the digest reads (rocketgpu_amd.views car_states).
+CAR_POS, CAR_ROT, CAR_VEL, CAR_ANG_VEL = 0, 12, 28, 40
+
+
+def _fnv_rows_from(rows: np.ndarray, h: np.ndarray) -> np.ndarray:
+    """FNV-1a 64 of each row of a [R, B] uint8 array, continuing from per-row states h."""
+    h = h.copy()
+    with np.errstate(over="ignore"):
+        for j in range(rows.shape[1]):
+            h ^= rows[:, j].astype(np.uint64)
+            h *= FNV_PRIME
+    return h
+
+
+def arena_body_digests(balls: np.ndarray, cars: np.ndarray, counts: np.ndarray, max_cars: int) -> np.ndarray:
+    """Per-arena body digests of car arenas: the ball's pos, vel, ang_vel, rot, then each of the arena's cars' in
+    slot order, as one FNV-1a 64 stream.  balls [N, 288] u8, cars [N * max_cars, 880] u8, counts [N]."""
+    b = np.ascontiguousarray(balls, dtype=np.uint8)
+    c = np.ascontiguousarray(cars, dtype=np.uint8).reshape(b.shape[0], max_cars, -1)
+    ball_cols = np.concatenate([b[:, BALL_POS:BALL_POS + 12], b[:, BALL_VEL:BALL_VEL + 12],
+                                b[:, BALL_ANG_VEL:BALL_ANG_VEL + 12], b[:, BALL_ROT:BALL_ROT + 16]], axis=1)
+    h = _fnv_rows_from(ball_cols, np.full(b.shape[0], FNV_OFFSET, dtype=np.uint64))
+    for k in range(max_cars):
+        ck = c[:, k]
+        cols = np.concatenate([ck[:, CAR_POS:CAR_POS + 12], ck[:, CAR_VEL:CAR_VEL + 12],
+                               ck[:, CAR_ANG_VEL:CAR_ANG_VEL + 12], ck[:, CAR_ROT:CAR_ROT + 16]], axis=1)
+        h = np.where(k < np.asarray(counts), _fnv_rows_from(cols, h), h)
+    return h
+
+
+def chain_step(chain: np.ndarray, digests: np.ndarray) -> np.ndarray:
+    """One step of the per-arena running chain: FNV-1a 64 over the 8 little-endian bytes of the new digest."""
+    d = np.ascontiguousarray(digests, dtype=np.uint64)
+    return _fnv_rows_from(d.view(np.uint8).reshape(-1, 8), np.asarray(chain, dtype=np.uint64))

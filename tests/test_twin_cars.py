@@ -108,3 +108,49 @@ def test_twin_cars_teammates_collide_match_reference():
     # Arenas that were never flagged went through many bumps (each bump is one car-car contact's outcome).
     assert bumps[~ever_flagged].sum() >= 100, f"bumps in unflagged arenas: {bumps[~ever_flagged].sum()}"
     assert compared >= 0.9 * n_env * ticks
+
+
+def test_twin_cars_mixed_long_run_prefix_matches_oracle_chains():
+    """The first 64 arenas of the 4,096-arena, 10,000-tick oracle fixture, stepped by the host twin from the
+    reference's kickoff for 2,000 ticks: every arena unflagged up to a checkpoint has the oracle's running body-digest
+    chain there (arenas are independent, and their kickoff and car ids depend only on the seed and the arena index)."""
+    import os
+
+    from digests import FNV_OFFSET, arena_body_digests, chain_step
+    import rocketgpu_amd as rga
+
+    R = oracle_or_skip()
+    T = twin_cars()
+    fx = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "cars_mixed_4096_t10000.npz"))
+    n, seed, period, every = 64, int(fx["seed"]), int(fx["period"]), int(fx["every"])
+    counts, presets, teams = fx["counts"][:n], fx["presets"][:n * 6], fx["teams"][:n * 6]
+    ref = R.RefSim(n, 6, respawn_seed=seed)
+    ref.set_env_car_count(counts)
+    ref.set_teams(teams)
+    ref.set_car_presets(presets)
+    ref.reset_to_kickoff(seed)
+    rec = reference_records(R, ref, n, 6)
+    h = T.rgt_cars_create(n, 6, 1.0, 256.0, seed)
+    for which, arr in ((6, counts), (7, teams), (5, ref.download_car_configs(n * 6)), (0, rec["cars"]),
+                       (8, rec["wheels"]), (1, rec["balls"]), (2, rec["envs"]), (3, rec["pads"])):
+        a = np.ascontiguousarray(arr)
+        assert T.rgt_cars_upload(h, which, a.ctypes.data, a.shape[0]) == 0
+    ref.close()
+    controls = rga.workload_controls(seed, 0, n, 6, period)
+    chain = chain_step(np.full(n, FNV_OFFSET, dtype=np.uint64),
+                       arena_body_digests(rec["balls"], rec["cars"], counts, 6))
+    assert np.array_equal(chain, fx["chain"][0][:n])
+    flagged = np.zeros(n, bool)
+    for t in range(2 * every):
+        ctl = np.ascontiguousarray(controls[t % period])
+        assert T.rgt_cars_upload(h, 4, ctl.ctypes.data, ctl.shape[0]) == 0
+        assert T.rgt_cars_step(h, 1) == 0
+        mine, uns = _twin_records(T, h, n, 6)
+        flagged |= uns != 0
+        chain = chain_step(chain, arena_body_digests(mine["balls"], mine["cars"], counts, 6))
+        if (t + 1) % every == 0:
+            ref_chain = fx["chain"][(t + 1) // every][:n]
+            bad = np.nonzero(~flagged & (chain != ref_chain))[0]
+            assert bad.size == 0, f"tick {t + 1}: unflagged arenas {bad} differ from the oracle"
+    T.rgt_cars_destroy(h)
+    assert (~flagged).sum() >= n // 2
