@@ -361,7 +361,9 @@ def bench_cars(args, rga, np, torch, stream):
                                 "measured traffic 1.12x the algorithmic bytes)",
                      "kernel": f"StepCarArenasKernel<{'true' if stage == 2 else 'false'}>",
                      "kernel_avg_ms": avg_kernel_s * 1e3,
-                     "bytes_per_arena_tick": b_arena},
+                     "bytes_per_arena_tick": b_arena,
+                     "traffic_gbs": traffic / avg_kernel_s / 1e9 if traffic else None,
+                     "traffic_frac": traffic / avg_kernel_s / 1e9 / HBM_PEAK_GBS if traffic else None},
         "forms": forms,
         "cpu_baseline": cpu,
     }

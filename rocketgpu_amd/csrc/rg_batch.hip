@@ -841,6 +841,7 @@ public:
         car_count_.Upload(counts, n, s);
         host_counts_.assign(counts, counts + n);
         has_cars_ = std::any_of(host_counts_.begin(), host_counts_.end(), [](int32_t c) { return c > 0; });
+        RebuildCarOrder(s);
         DestroyGraph();
         Check(hipStreamSynchronize(s), "hipStreamSynchronize");
     }
@@ -948,6 +949,7 @@ public:
         Check(hipStreamSynchronize(s), "hipStreamSynchronize");
         host_counts_ = src.host_counts_;
         has_cars_ = src.has_cars_;
+        RebuildCarOrder(s);
         seed_ = src.seed_;
         kickoff_settled_ = src.kickoff_settled_;
         settled_ready_ = src.settled_ready_;
@@ -1130,6 +1132,7 @@ public:
         settled_ready_ = hdr.settled_ready != 0;
         host_counts_ = counts;
         has_cars_ = std::any_of(counts.begin(), counts.end(), [](int32_t c) { return c > 0; });
+        RebuildCarOrder(s);
         kick_dirty_ = true;
         DestroyGraph();
         Check(hipStreamSynchronize(s), "hipStreamSynchronize");
@@ -1617,10 +1620,27 @@ private:
         Check(hipGetLastError(), "ApplyEnvCommandsCarsKernel launch");
     }
 
+    // The plain launch's lane order: arenas by car count, most cars first (index order within a count), so that the
+    // arenas sharing a wavefront run the same per-car loops.  Each arena's result does not depend on its lane.
+    void RebuildCarOrder(hipStream_t s) {
+        const int n = cfg_.env_count;
+        std::vector<int32_t> order(static_cast<size_t>(n));
+        for (int i = 0; i < n; ++i) order[static_cast<size_t>(i)] = i;
+        std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
+            return host_counts_[static_cast<size_t>(x)] > host_counts_[static_cast<size_t>(y)];
+        });
+        if (car_order_.size() != static_cast<size_t>(n)) car_order_.Resize(static_cast<size_t>(n));
+        car_order_.Upload(order.data(), static_cast<size_t>(n), s);
+        Check(hipStreamSynchronize(s), "hipStreamSynchronize");
+    }
+
     void LaunchCars(const int32_t* ids, const int32_t* ticks_list, int32_t count, int32_t ticks,
                     const CarControls* controls, int64_t stride, hipStream_t s) {
         StepArgs a = BaseArgs();
-        a.arena_ids = ids;
+        a.arena_ids = ids == nullptr && ticks_list == nullptr && count == cfg_.env_count &&
+                              car_order_.size() == static_cast<size_t>(cfg_.env_count)
+                          ? car_order_.data()
+                          : ids;
         a.arena_ticks = ticks_list;
         a.count = count;
         a.arenas_per_wave = per_wave_;
@@ -1779,6 +1799,7 @@ private:
     DeviceArray<EnvState> envs_;
     DeviceArray<core::ArenaInternal> internal_;
     DeviceArray<int32_t> car_count_;
+    DeviceArray<int32_t> car_order_;  // plain-launch lane order (RebuildCarOrder)
     DeviceArray<CarState> cars_;
     DeviceArray<CarControls> controls_;
     DeviceArray<WheelRuntimeState> wheels_;
