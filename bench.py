@@ -254,14 +254,15 @@ def bench_cars(args, rga, np, torch, stream):
     def ctrl_ptr(k):
         return ctrl.data_ptr() + ((k * T) % CTRL_TICKS) * row_bytes
 
-    # Arenas per wavefront of the car kernel (one lane per arena; fewer arenas per wave, more waves per CU): the
+    # Arenas per wavefront of the car kernel (fewer arenas per wave, more waves per CU): the
     # tuner times each on a fresh batch over the warm-up window plus five steps and keeps the fastest.
-    # Staging 0 keeps each arena's per-tick working data in lane-private memory, 2 in LDS (one slot per lane).
+    # Staging 0 keeps each arena's per-tick working data in lane-private memory, 2 in LDS (one slot per lane); 4 gives
+    # each arena 2 lanes (32 arenas per wave) or 4 (16 or 8), which split its wheel rays' tree walks.
     sweep = {}
     shape = (args.stage, args.per_wave) if args.block else None
     if shape is None:
         key = lambda c: f"stage{c[0]}_perwave{c[1]}"
-        for cand in ((0, 64), (0, 32), (0, 16), (2, 32), (2, 16), (2, 8), (2, 4)):
+        for cand in ((0, 64), (0, 32), (0, 16), (4, 32), (4, 16), (4, 8), (2, 32), (2, 16), (2, 8), (2, 4)):
             tsim = make_car_batch(rga, np, n, args.seed, workload)
             tsim.set_launch_shape(64, *cand)
             try:
@@ -307,6 +308,8 @@ def bench_cars(args, rga, np, torch, stream):
     arena_ticks = n * T * args.steps
     value = arena_ticks / wall
     b_arena = float(np.mean([bytes_per_arena_tick(c) for c in counts]))
+    # Lanes per arena of the car kernel: 2 or 4 at stage 4 (32 or at most 16 arenas per wave), else 1.
+    lanes = (2 if per_wave == 32 else 4) if stage == 4 else 1
     achieved = n * T * b_arena / avg_kernel_s / 1e9
     traffic = None
     traffic_file = args.traffic or os.path.join(REPO, "profiles", "step_car_arenas_16384_traffic.json")
@@ -349,17 +352,21 @@ def bench_cars(args, rga, np, torch, stream):
                                f"step_with_device_controls T={T}",
                    "arenas": n, "ticks_per_step": T, "cars_per_arena_mix": mix,
                    "cars": int(counts.sum()), "parallelism": "single GPU",
-                   "kernel_launch": f"one lane per arena, {per_wave} arenas per 64-lane wavefront, per-tick data "
-                                    f"in {'LDS' if stage == 2 else 'lane-private memory'}",
+                   "kernel_launch": f"{lanes} lane{'s' if lanes > 1 else ''} per arena, {per_wave} arenas per "
+                                    f"64-lane wavefront, per-tick data in "
+                                    f"{'LDS' if stage == 2 else 'lane-private memory'}"
+                                    + (", the arena's wheel-ray tree walks split between its lanes" if lanes > 1
+                                       else ""),
                    "launch_shape_sweep_ms": sweep or None},
         "arenas_unsupported_frac_at_end": float((unsupported != 0).mean()),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "limiter": "latency and divergence (one lane per arena walks its wheel rays, broadphase, "
-                                "contacts and solver serially; counters: 67% of wave cycles waiting on memory, a "
-                                "quarter of the lanes active per VALU instruction, one wave per CU at 256 VGPRs; "
-                                "measured traffic 1.12x the algorithmic bytes)",
-                     "kernel": f"StepCarArenasKernel<{'true' if stage == 2 else 'false'}>",
+                     "limiter": "latency and divergence (each arena's broadphase, contacts, solver and vehicle "
+                                "dynamics run serially on its lanes, whose chain of dependent loads sets the time; "
+                                "counters: 67% of wave cycles waiting on memory, a quarter of the lanes active per "
+                                "VALU instruction, one wave per SIMD at most at 256 VGPRs; measured traffic 1.12x "
+                                "the algorithmic bytes)",
+                     "kernel": f"StepCarArenasKernel<{'true' if stage == 2 else 'false'}, {lanes}>",
                      "kernel_avg_ms": avg_kernel_s * 1e3,
                      "bytes_per_arena_tick": b_arena,
                      "traffic_gbs": traffic / avg_kernel_s / 1e9 if traffic else None,

@@ -57,6 +57,8 @@ struct CarArenaRefs {
     int count;
     int env_idx;
     int car_base;  // batch index of this arena's car slot 0 (pad locks and respawns store base + slot + 1)
+    int ray_lanes = 1;  // lanes of a wavefront stepping this arena together, each with the same data (1, 2, 4)
+    int ray_sub = 0;    // this lane's index among them (the wheel rays it walks)
 };
 
 // ------------------------------------------------------------------------------------------------------------
@@ -820,6 +822,8 @@ RG_HD void StepCarArena(CarArenaRefs& a, const ArenaMeshView& mesh, const StepPa
         world.count = count;
         world.self = i;
         world.others_near = near;
+        world.ray_lanes = a.ray_lanes;
+        world.ray_sub = a.ray_sub;
         CarVehicleUpdate(car, a.controls[i], a.cfgs[i], m, a.wheels[i], world, unsupported);
     }
 

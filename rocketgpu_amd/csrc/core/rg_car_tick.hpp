@@ -557,17 +557,22 @@ RG_HD bool WheelRayCarHit(const Vec3& src, const Vec3& dir, float len_uu, float 
     return true;
 }
 
+// The wheel's segment against the arena meshes: its length and the nearest fraction a hit may have.
+RG_HD void WheelMeshSegment(const WheelRay& w, float min_hit_sq, Vec3& delta, float& min_frac_sq) {
+    delta = w.down * WheelRayLenBt(w);
+    min_frac_sq = min_hit_sq > 0.0f ? min_hit_sq / LengthSq((w.from + delta) - w.from) : 0.0f;
+}
+
 // The wheel's ray against the arena, the other cars and the ball; nearest wins (RayCastWheelAgainstArena, :641-844,
-// for a custom arena: meshes only, no analytic planes).  `others_near`: the other bodies whose last swept boxes the
-// car's rays reach (bit 0 ball, bit 1 + j car j); a car there cannot be cast here yet, which the arena reports.
-RG_HD void CastWheel(WheelRay& w, const CarState& car, const CarState* cars, const CarConfig* cfgs, int count, int self,
-                     const ArenaMeshView& mesh, const BallState& ball, float ball_radius, uint32_t others_near,
-                     float min_hit_sq, uint32_t& unsupported) {
+// for a custom arena: meshes only, no analytic planes).  `mh`: the ray's nearest mesh hit (CastArenaRay over the
+// segment WheelMeshSegment gives).  `others_near`: the other bodies whose last swept boxes the car's rays reach
+// (bit 0 ball, bit 1 + j car j); a car there cannot be cast here yet, which the arena reports.
+RG_HD void CastWheel(WheelRay& w, const RayHit& mh, const CarState& car, const CarState* cars, const CarConfig* cfgs,
+                     int count, int self, const ArenaMeshView& mesh, const BallState& ball, float ball_radius,
+                     uint32_t others_near, float min_hit_sq, uint32_t& unsupported) {
     const float len_bt = WheelRayLenBt(w);
     const float len_uu = ((w.rest + kWheelMaxTravel) + w.radius) - kWheelTravelTrim;
     const Vec3 delta = w.down * len_bt;
-    const float min_frac_sq = min_hit_sq > 0.0f ? min_hit_sq / LengthSq((w.from + delta) - w.from) : 0.0f;
-    const RayHit mh = CastArenaRay(mesh, w.from, delta, min_frac_sq);
     int kind = mh.hit ? kGroundMesh : kGroundNone;
     float best_t = mh.hit ? mh.fraction * len_uu : 0.0f;
     Vec3 p(0.0f, 0.0f, 0.0f), n = mh.normal;
@@ -925,7 +930,56 @@ struct VehicleWorld {
     int count;
     int self;
     uint32_t others_near;  // bodies the car's rays are cast against (bit 0 ball, 1 + j car j)
+    int ray_lanes = 1;     // lanes of a wavefront stepping this arena together (1, 2 or 4)
+    int ray_sub = 0;       // this lane's index among them
 };
+
+#if defined(__HIP_DEVICE_COMPILE__)
+// Ray hit `h` of lane `src` of this lane's group of `width` lanes (the mesh-hit fields CastWheel reads).
+__device__ inline RayHit ShuffleRayHit(const RayHit& h, int src, int width) {
+    RayHit o;
+    o.fraction = __shfl(h.fraction, src, width);
+    o.normal = Vec3(__shfl(h.normal.x, src, width), __shfl(h.normal.y, src, width), __shfl(h.normal.z, src, width));
+    o.tri = __shfl(h.tri, src, width);
+    o.hit = static_cast<uint8_t>(__shfl(static_cast<int>(h.hit), src, width));
+    o.min_fraction_sq = __shfl(h.min_fraction_sq, src, width);
+    return o;
+}
+#endif
+
+// The four wheel segments' nearest mesh hits, each CastArenaRay's.  When `lanes` (2 or 4) lanes of a wavefront
+// step the same arena with the same data, lane `sub` walks rays sub (and sub + 2 for two lanes) and the group's
+// lanes exchange the hits: the serial chain of tree loads per arena is a half or a quarter as long.
+RG_HD void CastWheelRays(const ArenaMeshView& mesh, const WheelRay* w, const Vec3* delta, const float* min_sq,
+                         int lanes, int sub, RayHit* out) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    if (lanes == 2 || lanes == 4) {
+        const int r0 = sub, r1 = sub + 2;
+        Vec3 f0 = w[0].from, d0 = delta[0], f1 = w[2].from, d1 = delta[2];
+        float m0 = min_sq[0], m1 = min_sq[2];
+        for (int q = 1; q < 4; ++q) {
+            if (q == r0) {
+                f0 = w[q].from;
+                d0 = delta[q];
+                m0 = min_sq[q];
+            }
+        }
+        if (r1 == 3) {
+            f1 = w[3].from;
+            d1 = delta[3];
+            m1 = min_sq[3];
+        }
+        const RayHit h0 = CastArenaRay(mesh, f0, d0, m0);
+        const RayHit h1 = lanes == 2 ? CastArenaRay(mesh, f1, d1, m1) : h0;
+        for (int r = 0; r < 4; ++r) out[r] = ShuffleRayHit(r < lanes ? h0 : h1, r % lanes, lanes);
+        return;
+    }
+#else
+    (void)lanes;
+    (void)sub;
+#endif
+    for (int r = 0; r < 4; ++r) out[r] = CastArenaRay(mesh, w[r].from, delta[r], min_sq[r]);
+}
 
 // The vehicle update of one car (PreTickCar after its pre-physics part, env_step.hpp:3115-3510, with
 // PreTickVehicleDynamics, raycast_wheels_and_contact_prep.hpp:2618-2975).  Controls are the step's raw controls.
@@ -970,14 +1024,22 @@ RG_HD void CarVehicleUpdate(CarState& car, const CarControls& raw, const CarConf
     const float min_hit_sq = min_hit * min_hit;
     const float front_bone = WheelBoneZ(cfg, true);
     const float back_bone = cfg.back_suspension_rest == cfg.front_suspension_rest ? front_bone : WheelBoneZ(cfg, false);
+    // The four rays against the meshes first (a ray's mesh hit depends only on the car's own pose).
     WheelRay w[4];
+    Vec3 seg_delta[4];
+    float seg_min_sq[4];
+    for (int i = 0; i < 4; ++i) {
+        InitWheel(w[i], car, cfg, k, i, i < 2 ? front_bone : back_bone);
+        WheelMeshSegment(w[i], min_hit_sq, seg_delta[i], seg_min_sq[i]);
+    }
+    RayHit mesh_hit[4];
+    CastWheelRays(*world.mesh, w, seg_delta, seg_min_sq, world.ray_lanes, world.ray_sub, mesh_hit);
     for (int kk = 0; kk < 4; ++kk) {
         const int i = kk ^ 1;
-        InitWheel(w[i], car, cfg, k, i, i < 2 ? front_bone : back_bone);
         w[i].force_scale = w[i].front ? front_share : back_share;
         w[i].pushback = ws.extra_pushback[i];
-        CastWheel(w[i], car, world.cars, world.cfgs, world.count, world.self, *world.mesh, *world.ball,
-                  world.ball_radius, world.others_near, min_hit_sq, unsupported);
+        CastWheel(w[i], mesh_hit[i], car, world.cars, world.cfgs, world.count, world.self, *world.mesh,
+                  *world.ball, world.ball_radius, world.others_near, min_hit_sq, unsupported);
         if (w[i].contact) w[i].pushback = 0.0f;
         if (w[i].on_world) WheelPushback(car, m.car_mass, k, w[i]);
         if (w[i].contact && !w[i].on_world && w[i].ground == kGroundCar) {

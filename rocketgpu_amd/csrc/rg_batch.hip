@@ -296,7 +296,7 @@ struct CarStepArgs {
 // reports are refreshed on the call's last tick.
 // Arena e's n car ticks with `scratch` as its per-tick working data.
 __device__ inline void RunCarArenaTicks(const StepArgs& a, const CarStepArgs& c, int e, int n, int count,
-                                        core::CarArenaTick& scratch) {
+                                        core::CarArenaTick& scratch, int lanes = 1, int sub = 0) {
     const int base = e * a.max_cars;
     const core::PadLayout pads{c.pad_pos, a.params.pad_count};
     for (int t = 0; t < n; ++t) {
@@ -313,6 +313,8 @@ __device__ inline void RunCarArenaTicks(const StepArgs& a, const CarStepArgs& c,
         r.count = count;
         r.env_idx = e;
         r.car_base = base;
+        r.ray_lanes = lanes;
+        r.ray_sub = sub;
         const core::CarRespawnContext rc{&c.rules, c.seed, e, 0u, 0u, 0};
         core::StepCarArena(r, a.mesh, a.params, c.m, pads, rc, t + 1 == n, scratch);
     }
@@ -323,17 +325,21 @@ __device__ inline void RunBallArenaTicks(const StepArgs& a, int e, int n, core::
 }
 
 // kLds: each arena's per-tick working data (CarArenaTick) lives in LDS, one slot per lane, instead of lane-private
-// memory.
-template <bool kLds>
+// memory.  G (1, 2 or 4): lanes per arena.  The G lanes of an arena step it together on the same data (their
+// stores write equal values) and split its wheel rays' tree walks between them; arenas_per_wave * G <= 64.
+template <bool kLds, int G>
 __global__ void __launch_bounds__(64) StepCarArenasKernel(StepArgs a, CarStepArgs c, int32_t ticks) {
+    static_assert(!kLds || G == 1, "LDS slots are one per lane");
     extern __shared__ __align__(16) unsigned char lds[];
-    if (static_cast<int>(threadIdx.x) >= a.arenas_per_wave) return;
-    const int i = static_cast<int>(blockIdx.x) * a.arenas_per_wave + static_cast<int>(threadIdx.x);
+    if (static_cast<int>(threadIdx.x) >= a.arenas_per_wave * G) return;
+    const int i = static_cast<int>(blockIdx.x) * a.arenas_per_wave + static_cast<int>(threadIdx.x) / G;
+    const int sub = static_cast<int>(threadIdx.x) % G;
     if (i >= a.count) return;
     const int e = a.arena_ids ? a.arena_ids[i] : i;
     if (e < 0 || e >= a.env_count) return;
     const int n = a.arena_ticks ? a.arena_ticks[i] : ticks;
     const int count = a.env_car_count[e];
+    if (count == 0 && sub != 0) return;  // a ball-only arena runs on one lane
     if constexpr (kLds) {
         core::CarArenaTick& slot = reinterpret_cast<core::CarArenaTick*>(lds)[threadIdx.x];
         if (count == 0) {
@@ -348,7 +354,7 @@ __global__ void __launch_bounds__(64) StepCarArenasKernel(StepArgs a, CarStepArg
             return;
         }
         core::CarArenaTick scratch;
-        RunCarArenaTicks(a, c, e, n, count, scratch);
+        RunCarArenaTicks(a, c, e, n, count, scratch, G, sub);
     }
 }
 
@@ -1231,7 +1237,9 @@ public:
 
     // Rejects a shape whose LDS slices do not fit one workgroup's LDS; allocates what the shape's launches need.
     void CheckLaunchShape() {
-        if (IsGroupStage(stage_) && per_wave_ != 16 && per_wave_ != 8 && per_wave_ != 4) {
+        // A batch with cars at stage 4 gives each arena 2 lanes (32 arenas per wave) or 4 (16, 8 or 4).
+        const bool car_pairs = has_cars_ && stage_ == kStageGroup && per_wave_ == 32;
+        if (IsGroupStage(stage_) && !car_pairs && per_wave_ != 16 && per_wave_ != 8 && per_wave_ != 4) {
             throw InvalidArgument("stages 4 and 5 need 16, 8 or 4 arenas per wave (4, 8 or 16 lanes per arena)");
         }
         if (IsSplitStage(stage_) && per_wave_ != 8) {
@@ -1657,14 +1665,18 @@ private:
                     throw InvalidArgument("car arenas with LDS staging need " + std::to_string(lds) +
                                           " B of LDS per workgroup; the device allows " + std::to_string(limit));
                 }
-                Check(hipFuncSetAttribute(reinterpret_cast<const void*>(&StepCarArenasKernel<true>),
+                Check(hipFuncSetAttribute(reinterpret_cast<const void*>(&StepCarArenasKernel<true, 1>),
                                           hipFuncAttributeMaxDynamicSharedMemorySize, limit),
                       "hipFuncSetAttribute");
                 car_lds_allowed_ = static_cast<size_t>(limit);
             }
-            hipLaunchKernelGGL(StepCarArenasKernel<true>, dim3(grid), dim3(64), lds, s, a, c, ticks);
+            hipLaunchKernelGGL((StepCarArenasKernel<true, 1>), dim3(grid), dim3(64), lds, s, a, c, ticks);
+        } else if (stage_ == kStageGroup && per_wave_ <= 16) {
+            hipLaunchKernelGGL((StepCarArenasKernel<false, 4>), dim3(grid), dim3(64), 0, s, a, c, ticks);
+        } else if (stage_ == kStageGroup && per_wave_ == 32) {
+            hipLaunchKernelGGL((StepCarArenasKernel<false, 2>), dim3(grid), dim3(64), 0, s, a, c, ticks);
         } else {
-            hipLaunchKernelGGL(StepCarArenasKernel<false>, dim3(grid), dim3(64), 0, s, a, c, ticks);
+            hipLaunchKernelGGL((StepCarArenasKernel<false, 1>), dim3(grid), dim3(64), 0, s, a, c, ticks);
         }
         Check(hipGetLastError(), "StepCarArenasKernel launch");
     }

@@ -209,7 +209,7 @@ class GpuEvents:
         self.sim.close()
 
 
-SHAPES = [(64, 0, 64), (64, 3, 16), (64, 4, 8), (64, 5, 8), (128, 5, 4), (64, 6, 8), (128, 7, 8)]
+SHAPES = [(64, 0, 64), (64, 3, 8), (64, 4, 8), (64, 5, 8), (128, 5, 4), (64, 6, 8), (128, 7, 8)]
 
 
 @pytest.mark.gpu

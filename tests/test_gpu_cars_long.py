@@ -3,7 +3,8 @@
 
 tests/golden/cars_mixed_4096_t10000.npz (written by tests/golden/make_car_fixtures.py with the reference CPU build)
 holds, per arena, a running FNV-1a chain over its body digest after every tick, at every 1,000 ticks.  The GPU run
-(one tick per launch through step_with_device_controls) keeps the same chains from the device's per-arena digests.
+(one tick per launch through step_with_device_controls, two lanes per arena) keeps the same chains from the
+device's per-arena digests.
 At each checkpoint, every arena whose unsupported word has stayed 0 up to it must have the oracle's chain: all its
 bodies' positions, velocities, angular velocities and orientations equal, bit for bit, on every tick so far.
 """
@@ -44,6 +45,8 @@ def test_gpu_cars_mixed_4096_10000_ticks_match_oracle_chains():
     sim.set_env_car_count(counts)
     sim.set_teams(teams)
     sim.set_car_configs(table[presets])
+    # The headline launch shape: two lanes per arena, which split each car's wheel-ray tree walks.
+    sim.set_launch_shape(64, 4, 32)
     sim.reset_to_kickoff(seed)
     controls = rga.workload_controls(seed, 0, n, 6, period)
     dev = torch.from_numpy(controls.reshape(period, -1)).to("cuda")

@@ -83,7 +83,7 @@ def test_graph_replays_match_reference(ticks_per_launch, launches):
 
 @pytest.mark.parametrize("block,stage,per_wave", [(64, 0, 64), (128, 0, 64), (256, 0, 32), (64, 1, 64),
                                                   (128, 1, 16), (256, 1, 8), (64, 0, 4), (64, 1, 8),
-                                                  (64, 2, 16), (128, 2, 8), (64, 3, 8), (128, 3, 4), (256, 3, 4),
+                                                  (64, 2, 16), (128, 2, 8), (64, 3, 8), (128, 3, 4), (64, 3, 4),
                                                   (64, 4, 16), (64, 4, 8), (128, 4, 8), (256, 4, 4),
                                                   (64, 5, 16), (64, 5, 8), (128, 5, 8), (256, 5, 4)])
 def test_launch_shapes_match_reference(block, stage, per_wave):

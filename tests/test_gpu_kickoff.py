@@ -133,10 +133,10 @@ def test_gpu_car_env_commands_match_reference(n_env, cars, mixed):
     ref.close()
 
 
-@pytest.mark.parametrize("stage,per_wave", [(0, 32), (0, 8), (0, 4), (2, 8), (2, 4)])
+@pytest.mark.parametrize("stage,per_wave", [(0, 32), (0, 8), (0, 4), (2, 8), (2, 4), (4, 32), (4, 16), (4, 8)])
 def test_gpu_car_arenas_per_wave_agree(stage, per_wave):
-    """The car kernel's launch knobs (arenas per wavefront; per-tick data in lane-private memory or LDS) change
-    nothing in the results."""
+    """The car kernel's launch knobs (arenas per wavefront; per-tick data in lane-private memory or LDS; 2 or 4 lanes
+    per arena sharing its wheel rays at stage 4) change nothing in the results."""
     torch = _torch()
     import rocketgpu_amd as rga
 
