@@ -316,7 +316,9 @@ def bench_cars(args, rga, np, torch, stream):
     if os.path.exists(traffic_file):
         with open(traffic_file) as fh:
             tr = json.load(fh)
-        if tr.get("arenas") == n and tr.get("ticks") == T and tr.get("seed") == args.seed:
+        shape_of = tr.get("measured_with", {})
+        if (tr.get("arenas") == n and tr.get("ticks") == T and tr.get("seed") == args.seed
+                and shape_of.get("stage") == stage and shape_of.get("arenas_per_wave") == per_wave):
             traffic = tr.get("hbm_bytes_per_launch")
 
     forms = None
