@@ -283,9 +283,11 @@ int rg_batch_step_graph(rg_batch* batch, int32_t ticks_per_launch, int32_t launc
  * in LDS; 3: records and scratch in LDS; 4: 64/arenas_per_wave lanes share each arena and split its contact walks;
  * 5: as 4, with the per-tick scratch in LDS, one slice per arena; 6 and 7: as 5 with 8 lanes per arena, and 32 or
  * 16 for the arenas with the longest contact walks, see rg_batch_set_split); arenas per 64-lane wavefront (64, 32,
- * 16, 8 or 4; stages 4 and 5 take 16, 8 or 4, stages 6 and 7 take 8).  A shape whose LDS exceeds the device's per-workgroup limit is rejected
- * (RG_EINVAL) and the previous shape kept.  Results are identical for every shape; the benchmark tunes it.
- * Defaults 64, 0, 64. */
+ * 16, 8 or 4; stages 4 and 5 take 16, 8 or 4, stages 6 and 7 take 8).  A batch with cars runs one lane per arena,
+ * except at stage 2 (per-tick data in LDS, one slot per lane) and stage 4 (2 lanes per arena at 32 arenas per wave,
+ * 4 at 16, 8 or 4, splitting each car's wheel-ray tree walks).  A shape whose LDS exceeds the device's
+ * per-workgroup limit is rejected (RG_EINVAL) and the previous shape kept.  Results are identical for every shape;
+ * the benchmark tunes it.  Defaults 64, 0, 64. */
 int rg_batch_set_launch_shape(rg_batch* batch, int32_t block, int32_t stage, int32_t arenas_per_wave);
 /* Stages 6 and 7 of the launch shape: before each whole-batch launch the arenas whose last contact walk loaded at
  * least `rounds` windows (default 26) go to a heavy list of at most env_count / cap_divisor arenas (default 16),
